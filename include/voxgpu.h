@@ -1,0 +1,232 @@
+/*
+ * voxgpu.h -- C ABI of libvoxgpu, the MI355X (gfx950) implementation of the
+ * voxtrees/voxrnd ray-casting path of shamazmazum/voxvision.
+ *
+ * Plain C: pointers, sizes and status codes, no C++ or torch types.  Device
+ * buffers are HIP device pointers (hipMalloc'd or torch CUDA tensors); all
+ * batched calls are asynchronous on the given stream unless stated.
+ *
+ * Reference interfaces replaced (paths relative to the reference root):
+ *   voxg_tree_upload / voxg_tree_build   vox_make_tree      src/voxtrees/tree.h:60
+ *   voxg_tree_plan(_reference)           vox_make_tree      src/voxtrees/tree.h:60
+ *                                        (host-side sizing, no device)
+ *   voxg_tree_release                    vox_destroy_tree   src/voxtrees/tree.h:67
+ *   voxg_tree_get_info                   vox_voxels_in_tree src/voxtrees/tree.h:72,
+ *                                        vox_bounding_box   src/voxtrees/tree.h:77
+ *   voxg_ray_batch          vox_ray_tree_intersection src/voxtrees/search.h:24-26
+ *   voxg_ball_batch         vox_tree_ball_collidep    src/voxtrees/search.h:36-37
+ *   voxg_render             vox_render per-pixel loop src/voxrnd/renderer.h:208,
+ *                           with screen2world of the simple camera
+ *                           src/voxrnd/simple-camera.c:21-31
+ */
+#ifndef VOXGPU_H
+#define VOXGPU_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* HIP stream handle (same declaration as hip_runtime_api.h). */
+typedef struct ihipStream_t *hipStream_t;
+
+/* Status codes: 0 ok, negative errno-style values on failure. */
+#define VOXG_OK 0
+#define VOXG_EIO (-5)      /* a HIP call failed */
+#define VOXG_ENOMEM (-12)  /* device or host allocation failed */
+#define VOXG_ENODEV (-19)  /* no usable gfx950 device */
+#define VOXG_EINVAL (-22)  /* bad argument */
+#define VOXG_ERANGE (-34)  /* size beyond a documented limit */
+
+/* Result code for a ray that hits nothing. */
+#define VOXG_MISS 0xffffffffu
+
+/* ---- reference tree layout (src/voxvision.h.in:45-56,79-83 with
+ *      SSE_INTRIN; src/voxtrees/tree.h:15-38) ----
+ * Only needed by callers that hand over a tree built by the reference's
+ * vox_make_tree; voxg_tree_upload reads it and never writes it. */
+#ifndef VOXGPU_NO_REFERENCE_LAYOUT
+typedef float vox_dot[4] __attribute__((aligned(16)));
+struct vox_box {
+    vox_dot min;
+    vox_dot max;
+};
+#define VOX_LEAF 1
+#define VOX_DENSE_LEAF 2
+#define VOX_LEAF_MASK 3
+#define VOX_OVERFLOW 4
+struct vox_node {
+    struct vox_box bounding_box;
+    unsigned int flags;
+    unsigned int dots_num;
+    union {
+        vox_dot *dots;
+        struct {
+            vox_dot center;
+            struct vox_node *children[8];
+        } inner;
+    } data;
+};
+#endif
+
+/* Opaque handle of a flattened, device-resident, immutable tree. */
+typedef struct voxg_tree voxg_tree;
+
+typedef struct {
+    uint64_t nodes, inner, leaves, dense, overflow, dots, voxel_ids;
+    uint32_t depth;          /* levels, root = 1 */
+    uint32_t stack_entries;  /* traversal stack bound: 3*(depth-1)+1 */
+    uint64_t device_bytes;   /* flattened tree size in HBM */
+    float bbox_min[3], bbox_max[3];
+    float voxel[3];          /* voxel size snapshot taken at build/upload */
+} voxg_tree_info;
+
+/* Select the HIP device used by later calls on this thread (default 0). */
+int voxg_init(int device);
+
+/* Build a tree from n voxel min-corners, exactly as vox_make_tree(set, n)
+ * would (tree.c:249-338), and upload it.  xyz holds n points of `stride`
+ * floats each (3 = packed, 4 = vox_dot layout); it is copied, never
+ * permuted.  voxel = vox_voxel at build time (NULL = {1,1,1}).
+ * n == 0 yields an empty tree on which every query misses. */
+int voxg_tree_build(const float *xyz, size_t n, size_t stride,
+                    const float voxel[3], voxg_tree **out);
+
+/* Flatten a tree built by the reference's vox_make_tree (reference layout
+ * above) and upload it.  root may be NULL (empty tree).  voxel must be the
+ * vox_voxel value the tree was built with (NULL = {1,1,1}). */
+int voxg_tree_upload(const struct vox_node *root, const float voxel[3],
+                     voxg_tree **out);
+
+/* Host-only planning: build / flatten exactly as voxg_tree_build /
+ * voxg_tree_upload would, without touching the device, and report the
+ * tree's statistics (device_bytes = HBM the upload will take) and its
+ * structure hash.  Either output may be NULL. */
+int voxg_tree_plan(const float *xyz, size_t n, size_t stride,
+                   const float voxel[3], voxg_tree_info *info,
+                   uint64_t *hash);
+int voxg_tree_plan_reference(const struct vox_node *root, const float voxel[3],
+                             voxg_tree_info *info, uint64_t *hash);
+
+int voxg_tree_release(voxg_tree *tree);
+int voxg_tree_get_info(const voxg_tree *tree, voxg_tree_info *out);
+
+/* Structure hash of the tree (identical for identical trees, whatever the
+ * origin: build, upload). */
+uint64_t voxg_tree_hash(const voxg_tree *tree);
+
+/* Host-side lookups.  Voxel ids: regular-leaf dots are numbered first, in
+ * depth-first preorder over children 0..7 and stored order inside a leaf;
+ * dense-leaf voxels follow, leaf by leaf in the same preorder, as
+ * base + (ix*dy + iy)*dz + iz.  Writes the min corner of each voxel. */
+int voxg_tree_voxel_coords(const voxg_tree *tree, size_t n,
+                           const uint32_t *ids, float *xyz);
+/* Node index (as returned in leaf_out) -> the reference node it came from
+ * (trees made by voxg_tree_upload only; NULL otherwise). */
+const struct vox_node *voxg_tree_node_ptr(const voxg_tree *tree,
+                                          uint32_t node);
+
+/* Batched vox_ray_tree_intersection over n rays (search.c:8-153).
+ * Inputs: SoA device arrays ox..dz (origin, direction).  Outputs (each may
+ * be NULL): voxel_out = voxel id or VOXG_MISS; leaf_out = node index of the
+ * returned leaf or VOXG_MISS; hx,hy,hz = hit point, left untouched on a
+ * miss (search.c leaves res untouched).  Results equal the reference's
+ * fp32 results bit for bit. */
+int voxg_ray_batch(const voxg_tree *tree, size_t n, const float *ox,
+                   const float *oy, const float *oz, const float *dx,
+                   const float *dy, const float *dz, uint32_t *voxel_out,
+                   uint32_t *leaf_out, float *hx, float *hy, float *hz,
+                   hipStream_t stream);
+
+/* Batched vox_tree_ball_collidep (search.c:155-183): out[i] = 1 when the
+ * ball (c_i, r_i) touches a voxel (squared distance strictly below r^2). */
+int voxg_ball_batch(const voxg_tree *tree, size_t n, const float *cx,
+                    const float *cy, const float *cz, const float *r,
+                    uint8_t *out, hipStream_t stream);
+
+/* Simple-camera descriptor (fields of struct vox_simple_camera,
+ * simple-camera.c:10-17).  mul, xsub, ysub as set by set_window_size
+ * (simple-camera.c:120-136); see voxg_camera_set_window_size. */
+typedef struct {
+    float position[3];
+    float rotation[4]; /* quaternion w, i, j, k */
+    float mul, xsub, ysub;
+    float fov;         /* tan of the half angle; only used by the setter */
+} voxg_camera;
+
+/* simple-camera.c:120-136, on the host. */
+void voxg_camera_set_window_size(voxg_camera *cam, int w, int h);
+
+/* Quality flags: renderer.h:81-131 */
+#define VOXG_QUALITY_BEST 0x0
+#define VOXG_QUALITY_FAST 0x1
+#define VOXG_QUALITY_ADAPTIVE 0x2
+#define VOXG_QUALITY_RAY_MERGE 0x4
+#define VOXG_QUALITY_RAY_MERGE_ACCURATE 0x8
+
+typedef struct {
+    unsigned quality;
+    /* Colour (renderer.c:27-52 without lights): device pointer to a 32^3
+     * byte texture indexed x*1024 + y*32 + z, mapped to a 32-bit pixel as
+     * (c<<rshift)|(c<<gshift)|(c<<bshift)|amask.  texture may be NULL when
+     * squares is NULL. */
+    const uint8_t *texture;
+    uint32_t rshift, gshift, bshift, amask;
+} voxg_render_params;
+
+/* The vox_render per-pixel loop for n_frames camera poses at W x H
+ * (W % 16 == 0, H % 4 == 0, renderer.c:97-100).  cams: device array of
+ * n_frames descriptors.  Pixel (x, y) of frame f lives at index
+ * f*W*H + cs*16 + p with block cs = (y/4)*(W/4) + x/4 and p = (y%4)*4 + x%4
+ * (the reference's square layout, renderer.h:14).  Outputs (each may be
+ * NULL): squares = colours (pixels that miss are untouched), voxel_px =
+ * voxel id or VOXG_MISS, hit_px = 3 planes of f*W*H floats (x, y, z),
+ * untouched on a miss. */
+int voxg_render(const voxg_tree *tree, size_t n_frames,
+                const voxg_camera *cams, int W, int H,
+                const voxg_render_params *params, uint32_t *squares,
+                uint32_t *voxel_px, float *hit_px, hipStream_t stream);
+
+/* Traversal counters of one counting run (node bounding-box tests and
+ * voxel-box tests), for the algorithmic-bytes model.  Synchronous. */
+typedef struct {
+    uint64_t node_visits;
+    uint64_t voxel_tests;
+    uint64_t rays;
+} voxg_counts;
+int voxg_ray_batch_count(const voxg_tree *tree, size_t n, const float *ox,
+                         const float *oy, const float *oz, const float *dx,
+                         const float *dy, const float *dz, voxg_counts *out,
+                         hipStream_t stream);
+int voxg_ball_batch_count(const voxg_tree *tree, size_t n, const float *cx,
+                          const float *cy, const float *cz, const float *r,
+                          voxg_counts *out, hipStream_t stream);
+int voxg_render_count(const voxg_tree *tree, size_t n_frames,
+                      const voxg_camera *cams, int W, int H,
+                      const voxg_render_params *params, voxg_counts *out,
+                      hipStream_t stream);
+
+/* Launch configuration of the traversal kernels (autotuned by
+ * voxvision_amd.autotune; defaults are sane).  stack_lds: stack entries per
+ * lane kept in LDS (power of two 2..32; deeper stacks spill to HBM and stay
+ * exact); block: threads per workgroup (64..1024, multiple of 64);
+ * waves_per_cu: persistent waves launched per CU; refill: minimum idle
+ * lanes before a wave fetches new rays (1..64). */
+typedef struct {
+    int stack_lds;
+    int block;
+    int waves_per_cu;
+    int refill;
+} voxg_launch_config;
+int voxg_set_launch_config(const voxg_launch_config *cfg);
+int voxg_get_launch_config(voxg_launch_config *cfg);
+
+/* Human-readable text of the last error on this thread. */
+const char *voxg_last_error(void);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

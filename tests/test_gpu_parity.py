@@ -1,0 +1,265 @@
+"""GPU parity: libvoxgpu against the CPU oracle on identical seeded inputs.
+
+Bar: bit-exact voxel ids, leaf identity and hit-point bits (the kernels
+perform the reference's fp32 operations in the reference's order), exact
+ball booleans, exact traversal counters.
+"""
+import ctypes as C
+
+import numpy as np
+import pytest
+import torch
+
+pytestmark = pytest.mark.gpu
+
+MISS = 0xFFFFFFFF
+
+
+def _c1():
+    from voxvision_amd import scenes
+    return scenes.config_c1()
+
+
+def _np_u32(t):
+    return (t.cpu().numpy().astype(np.int64) & 0xFFFFFFFF).astype(np.uint32)
+
+
+def _assert_rays_equal(vox_g, hit_g, vox_o, hit_o):
+    vg = _np_u32(vox_g)
+    assert np.array_equal(vg, vox_o), \
+        "voxel ids differ on %d rays" % int((vg != vox_o).sum())
+    m = vox_o != MISS
+    hg = hit_g.cpu().numpy()
+    assert np.array_equal(hg[:, m].view(np.uint32), hit_o[:, m].view(np.uint32))
+    assert np.isnan(hg[:, ~m]).all()  # untouched on a miss
+
+
+def _edge_rays(bmin, bmax):
+    """Axis-parallel rays, zero direction components, origins on faces, on
+    voxel corners, inside voxels, rays along dividing planes."""
+    o, d = [], []
+    lo, hi = np.floor(bmin), np.ceil(bmax)
+    c = np.floor((lo + hi) / 2)
+    for ax in range(3):
+        for sgn in (-1, 1):
+            for off in (0.0, 0.5, 1.0, 3.0):
+                oo = c.copy() + off
+                oo[ax] = lo[ax] - 5 if sgn > 0 else hi[ax] + 5
+                dd = np.zeros(3)
+                dd[ax] = sgn
+                o.append(oo)
+                d.append(dd)
+    rng = np.random.default_rng(99)
+    for _ in range(400):
+        oo = rng.integers(lo - 3, hi + 3).astype(np.float64)
+        dd = rng.integers(-2, 3, 3).astype(np.float64)
+        if not dd.any():
+            dd[0] = 1
+        o.append(oo)
+        d.append(dd)
+    for _ in range(400):  # origins on a face, inside, half-integers
+        oo = rng.integers(lo, hi).astype(np.float64) + \
+            rng.choice([0.0, 0.5, 1.0], 3)
+        dd = rng.standard_normal(3)
+        dd[rng.integers(0, 3)] = 0.0
+        o.append(oo)
+        d.append(dd)
+    return np.array(o, np.float32), np.array(d, np.float32)
+
+
+def test_ray_batch_random_and_edge_rays(gpu, oracle):
+    from voxvision_amd import scenes, voxtrees as VT
+    c1 = _c1()
+    dots = c1["dots"]
+    T = oracle.Tree(dots)
+    G = VT.make_tree(dots)
+    assert G.hash() == T.hash()
+    bmin, bmax = c1["bbox"]
+    o, d = scenes.random_rays(200000, bmin, bmax, seed=3)
+    eo, ed = _edge_rays(bmin, bmax)
+    o = np.concatenate([o, eo])
+    d = np.concatenate([d, ed])
+    vox_o, hit_o = T.rays(o, d)
+    vox_g, hit_g = VT.ray_tree_intersection(G, o, d)
+    torch.cuda.synchronize()
+    _assert_rays_equal(vox_g, hit_g, vox_o, hit_o)
+    assert (vox_o != MISS).mean() > 0.05
+
+
+def test_ray_counts_match_oracle(gpu, oracle):
+    from voxvision_amd import scenes, voxtrees as VT
+    c1 = _c1()
+    T = oracle.Tree(c1["dots"])
+    G = VT.make_tree(c1["dots"])
+    o, d = scenes.random_rays(50000, *c1["bbox"], seed=4)
+    _, _, (nv, nt) = T.rays(o, d, with_counts=True)
+    cnt = VT.ray_counts(G, o, d)
+    assert cnt["node_visits"] == nv and cnt["voxel_tests"] == nt
+    assert cnt["rays"] == len(o)
+
+
+def test_upload_reference_layout_tree(gpu, oracle):
+    """A tree in the reference's struct vox_node layout (built by the
+    oracle's vox_make_tree) flattens to the same results, and leaf indices
+    map back to the reference nodes the CPU search returns."""
+    from voxvision_amd import scenes, voxtrees as VT
+    rng = np.random.default_rng(8)
+    dots = scenes.volume_to_dots(rng.random((24, 24, 24)) < 0.2)
+    T = oracle.Tree(dots)
+    G = VT.Tree.upload(T.ptr)
+    assert G.hash() == T.hash()
+    o, d = scenes.random_rays(5000, *scenes.bbox_of(dots), seed=6)
+    vox_o, hit_o = T.rays(o, d)
+    vox_g, hit_g, leaf_g = VT.ray_tree_intersection(G, o, d, want_leaf=True)
+    _assert_rays_equal(vox_g, hit_g, vox_o, hit_o)
+    L = oracle.lib()
+    leaf_g = _np_u32(leaf_g)
+    for i in range(0, len(o), 97):
+        od = np.zeros(4, np.float32)
+        dd = np.zeros(4, np.float32)
+        res = np.zeros(4, np.float32)
+        od[:3] = o[i]
+        dd[:3] = d[i]
+        p = L.orc_ray_tree_intersection(T.ptr, oracle._fp(od), oracle._fp(dd),
+                                        oracle._fp(res))
+        if p is None:
+            assert leaf_g[i] == MISS
+        else:
+            assert G.node_ptr(int(leaf_g[i])) == p
+
+
+def test_voxel_ids_map_to_the_oracle_voxels(gpu, oracle):
+    from voxvision_amd import voxtrees as VT
+    c1 = _c1()
+    T = oracle.Tree(c1["dots"])
+    G = VT.make_tree(c1["dots"])
+    n = T.stats["voxel_ids"]
+    ids = np.arange(n, dtype=np.uint32)
+    assert np.array_equal(G.voxel_coords(ids), T.voxel_table())
+
+
+def test_ball_batch(gpu, oracle):
+    from voxvision_amd import scenes, voxtrees as VT
+    c1 = _c1()
+    T = oracle.Tree(c1["dots"])
+    G = VT.make_tree(c1["dots"])
+    c, r = c1["balls_c"], c1["balls_r"]
+    out_g = VT.tree_ball_collidep(G, c, r).cpu().numpy()
+    assert np.array_equal(out_g, T.balls(c, r))
+    assert np.array_equal(out_g, oracle.brute_balls(c1["dots"], c, r))
+    c, r = scenes.random_balls(100000, *c1["bbox"], seed=17)
+    out_o, (nv, nt) = T.balls(c, r, with_counts=True)
+    assert np.array_equal(VT.tree_ball_collidep(G, c, r).cpu().numpy(), out_o)
+    cnt = VT.ball_counts(G, c, r)
+    assert cnt["rays"] == len(c)
+    assert 0 < out_o.mean() < 1
+
+
+def test_render_best_matches_oracle(gpu, oracle):
+    from voxvision_amd import voxrnd
+    c1 = _c1()
+    T = oracle.Tree(c1["dots"])
+    from voxvision_amd import voxtrees as VT
+    G = VT.make_tree(c1["dots"])
+    tex = np.random.default_rng(2).integers(0, 256, 32 ** 3).astype(np.uint8)
+    W, H = c1["W"], c1["H"]
+    sq_o, vox_o, hit_o, _ = T.render(c1["camera"], W, H, oracle.QUALITY_BEST,
+                                     texture=tex)
+    out = voxrnd.render(G, [c1["camera"]], W, H, texture=tex)
+    torch.cuda.synchronize()
+    _assert_rays_equal(out["voxel"], out["hit"], vox_o, hit_o)
+    sq_g = _np_u32(out["squares"])
+    assert np.array_equal(sq_g, sq_o)
+
+
+def test_render_many_poses_ct(gpu, oracle):
+    """Two of the 64 orbit poses of the 256^3 scene at 1920x1080, compared
+    pixel by pixel with the oracle."""
+    from voxvision_amd import scenes, voxrnd, voxtrees as VT
+    c2 = scenes.config_c2(frames=64)
+    T = oracle.Tree(c2["dots"])
+    G = VT.make_tree(c2["dots"])
+    W, H = c2["W"], c2["H"]
+    picks = [0, 37]
+    cams = [c2["cameras"][k] for k in picks]
+    out = voxrnd.render(G, cams, W, H, want_colors=False)
+    torch.cuda.synchronize()
+    vg = _np_u32(out["voxel"]).reshape(len(picks), -1)
+    hg = out["hit"].cpu().numpy().reshape(3, len(picks), -1)
+    for i, cam in enumerate(cams):
+        _, vox_o, hit_o, _ = T.render(cam, W, H, oracle.QUALITY_BEST)
+        assert np.array_equal(vg[i], vox_o)
+        m = vox_o != MISS
+        assert np.array_equal(hg[:, i, m].view(np.uint32),
+                              hit_o[:, m].view(np.uint32))
+
+
+def test_stack_spill_is_exact(gpu, oracle):
+    """A 2-entry LDS ring forces the HBM spill path on a deep tree; the
+    results must not change."""
+    from voxvision_amd import scenes, voxtrees as VT
+    ii = np.arange(-50, 50)
+    I, J, K = np.meshgrid(ii, ii, ii, indexing="ij")
+    m = (I * I + J * J + K * K) < 2500
+    dots = np.stack([I[m], J[m], K[m]], 1).astype(np.float32)
+    G = VT.make_tree(dots)
+    T = oracle.Tree(dots)
+    o, d = scenes.random_rays(60000, *scenes.bbox_of(dots), seed=12)
+    vox_o, hit_o = T.rays(o, d)
+    old = VT.get_launch_config()
+    try:
+        for k in (2, 4, 32):
+            VT.set_launch_config(stack_lds=k, block=64 if k == 32 else 256)
+            vox_g, hit_g = VT.ray_tree_intersection(G, o, d)
+            torch.cuda.synchronize()
+            _assert_rays_equal(vox_g, hit_g, vox_o, hit_o)
+    finally:
+        VT.set_launch_config(**old)
+
+
+def test_empty_tree_and_empty_batches(gpu, oracle):
+    from voxvision_amd import voxtrees as VT
+    G = VT.make_tree(np.zeros((0, 3), np.float32))
+    o = np.zeros((100, 3), np.float32)
+    d = np.ones((100, 3), np.float32)
+    vox, hit = VT.ray_tree_intersection(G, o, d)
+    assert (_np_u32(vox) == MISS).all()
+    assert VT.tree_ball_collidep(G, o, np.ones(100)).sum().item() == 0
+    G2 = VT.make_tree(np.ones((5, 3), np.float32))
+    vox, _ = VT.ray_tree_intersection(G2, o[:0], d[:0])
+    assert vox.numel() == 0
+
+
+def test_single_voxel_known_answers(gpu, oracle):
+    """benchmarks/voxtrees/hit-box.c and ball-intersection.c closed forms."""
+    from voxvision_amd import voxtrees as VT
+    G = VT.make_tree(np.array([[0, 10, 0]], np.float32))
+    j = np.arange(10)
+    o = np.stack([j - 5, np.zeros(10), np.zeros(10)], 1).astype(np.float32)
+    d = np.stack([(j - 5) / 10.0, np.ones(10), np.zeros(10)],
+                 1).astype(np.float32)
+    vox, hit = VT.ray_tree_intersection(G, o, d)
+    v = _np_u32(vox)
+    assert int((v != MISS).sum()) == 1
+    k = int(np.nonzero(v != MISS)[0][0])
+    assert tuple(hit[:, k].cpu().numpy()) == (0.0, 10.0, 0.0)
+    G = VT.make_tree(np.zeros((1, 3), np.float32))
+    jj = np.arange(-100, 100)
+    c = np.stack([np.zeros(200), np.full(200, 10.0), jj], 1)
+    assert VT.tree_ball_collidep(G, c, np.full(200, 1000.0)).sum().item() == 200
+
+
+def test_invalid_arguments_are_rejected(gpu):
+    from voxvision_amd import _lib, voxtrees as VT
+    L = _lib.load()
+    G = VT.make_tree(np.ones((3, 3), np.float32))
+    x = torch.zeros(4, device="cuda")
+    rc = L.voxg_ray_batch(G.handle, 4, None, None, None, None, None, None,
+                          None, None, None, None, None, None)
+    assert rc == -22
+    cam = torch.zeros(11, device="cuda")
+    rc = L.voxg_render(G.handle, 1, cam.data_ptr(), 20, 8, None, None,
+                       x.data_ptr(), None, None)
+    assert rc == -22  # W % 16 != 0
+    assert L.voxg_ray_batch(None, 4, *([x.data_ptr()] * 6), None, None, None,
+                            None, None, None) == -22

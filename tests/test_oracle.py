@@ -1,0 +1,385 @@
+"""Pins for the CPU oracle (CPU only).
+
+The reference (shamazmazum/voxvision) cannot be compiled here -- its
+sources need the cmake-generated voxvision.h -- so the oracle is pinned by:
+  * the known answers of the reference's own tests (tests/tests.c) and
+    micro-benchmarks (benchmarks/voxtrees/*.c), kept as data in
+    tests/golden/reference_known_answers.json;
+  * an FP64 brute force over all voxels (hit/miss, hit voxel, hit point);
+  * a brute-force ball test with the reference fp32 formula (exact);
+  * closed forms of the geometry primitives.
+"""
+import json
+import math
+import os
+
+import numpy as np
+import pytest
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+GOLD = json.load(open(os.path.join(HERE, "golden",
+                                   "reference_known_answers.json")))
+PRECISE = GOLD["tolerances"]["precise_check"]
+APPROX = GOLD["tolerances"]["approx_check"]
+
+
+def ball50():
+    ii = np.arange(-50, 50)
+    I, J, K = np.meshgrid(ii, ii, ii, indexing="ij")
+    m = (I * I + J * J + K * K) < 2500
+    return np.stack([I[m], J[m], K[m]], 1).astype(np.float32)
+
+
+# ----------------------------------------------------------------------
+# vector operations: tests/tests.c:79-213, 303-363
+# ----------------------------------------------------------------------
+def test_quaternion_table(oracle):
+    for case in GOLD["quat_mul"]:
+        got = oracle.quat_mul(case["q1"], case["q2"])
+        assert np.max(np.abs(got - np.array(case["expect"]))) <= PRECISE, case
+
+
+def test_rotations(oracle):
+    for case in GOLD["rotate_vector"]:
+        q = [float(x) for x in case["q"]]
+        got = oracle.rotate_vector(q, case["v"])
+        exp = np.array(case["expect"] if case["expect"] != "v"
+                       else case["v"], np.float32)
+        assert np.max(np.abs(got - exp)) <= PRECISE, case
+
+
+def test_rotation_composition_and_isometry(oracle):
+    # tests.c:185-213
+    phi, psi = np.float32(0.1), np.float32(0.2)
+    c, s = np.cos, np.sin
+    b1 = [c(phi), s(phi), 0, 0]
+    b2 = [c(psi), 0, s(psi), 0]
+    br = [c(phi) * c(psi), s(phi) * c(psi), s(psi) * c(phi),
+          s(psi) * s(phi)]
+    v = [132, 2, 35]
+    r1 = oracle.rotate_vector(b1, oracle.rotate_vector(b2, v))
+    r2 = oracle.rotate_vector(br, v)
+    assert np.max(np.abs(r1 - r2)) <= PRECISE * 10  # tests.c uses vect_eq
+    base = [0.5, 0.5, 0.5, 0.5]
+    v = np.array([1, 5, 2], np.float32)
+    r = oracle.rotate_vector(base, v)
+    assert abs(float(v @ v) - float(r @ r)) < PRECISE * 10
+    # anticommutativity, tests.c:157-174
+    sc = math.sqrt(2.0) / 2.0
+    r1 = oracle.rotate_vector([sc, sc, 0, 0], [0, 1, 0])
+    r2 = oracle.rotate_vector([sc, 0, sc, 0], [1, 0, 0])
+    assert np.max(np.abs(r1 + r2)) <= PRECISE
+
+
+# ----------------------------------------------------------------------
+# cameras: tests/tests.c:661-738 (simple camera)
+# ----------------------------------------------------------------------
+def test_camera_center_ray(oracle):
+    from voxvision_amd import camera
+    g = GOLD["camera"]
+    mul, xs, ys = oracle.set_window_size(1.0, 100, 100)
+    assert (mul, xs, ys) == camera.set_window_size(1.0, 100, 100)
+    cam = dict(position=(0, 0, 0), rotation=(1, 0, 0, 0), mul=mul, xsub=xs,
+               ysub=ys)
+    d = oracle.screen2world(cam, 50, 50)
+    assert np.max(np.abs(d - np.array(g["center_ray"]))) <= PRECISE
+    # set_property_dot "rotation" {0,0,pi/2} (simple-camera.c:171-183)
+    ang = np.float32(math.pi / 2)
+    r = [[np.cos(0), np.sin(0), 0, 0], [np.cos(0), 0, np.sin(0), 0],
+         [np.cos(ang / 2), 0, 0, np.sin(ang / 2)]]
+    q = oracle.quat_mul(r[1], r[0])
+    q = oracle.quat_mul(r[2], q)
+    cam["rotation"] = tuple(q)
+    d = oracle.screen2world(cam, 50, 50)
+    assert np.max(np.abs(d - np.array(g["after_z_rotation"]))) <= PRECISE
+
+
+def test_camera_look_at(oracle):
+    # tests.c:706-724: the centre ray passes within approx_check of target
+    from voxvision_amd import camera
+    pos = np.array(GOLD["camera"]["look_at_position"], np.float64)
+    tgt = np.zeros(3)
+    c = camera.make_camera(pos, tgt, 1.0, 100, 100)
+    d = oracle.screen2world(c, 50, 50).astype(np.float64)
+    t = (tgt[0] - pos[0]) / d[0]
+    assert t >= 0
+    hit = pos + t * d
+    assert np.max(np.abs(hit - tgt)) <= APPROX
+
+
+# ----------------------------------------------------------------------
+# tree construction: tests/tests.c:214-300, 360-365
+# ----------------------------------------------------------------------
+def test_tree_construction_invariants(oracle):
+    T = oracle.Tree(ball50())
+    assert T.n == GOLD["ball50"]["voxels"]
+    assert T.check() == 0
+    assert T.stats["voxel_ids"] == T.n
+    assert T.stats["overflow"] == 0
+
+
+def test_tree_invariants_other_scenes(oracle):
+    from voxvision_amd import scenes
+    rng = np.random.default_rng(3)
+    for dots in (scenes.config_c1()["dots"],
+                 scenes.volume_to_dots(rng.random((20, 20, 20)) < 0.3)):
+        T = oracle.Tree(dots)
+        assert T.check() == 0
+        tab = T.voxel_table()
+        # the id table is a permutation of the input voxels
+        a = np.unique(tab, axis=0)
+        b = np.unique(dots, axis=0)
+        assert len(tab) == len(dots) and np.array_equal(a, b)
+
+
+def test_dense_and_overflow_leaves(oracle):
+    # 3x3x3 block is one dense leaf (tests.c:404-420 builds it by insertion)
+    g = np.stack(np.meshgrid(*[np.arange(3)] * 3, indexing="ij"),
+                 -1).reshape(-1, 3).astype(np.float32)
+    T = oracle.Tree(g)
+    assert T.stats["dense"] == 1 and T.stats["nodes"] == 1
+    # 10 copies of one voxel cannot be split -> overflow leaf
+    dup = np.zeros((10, 3), np.float32) + 4
+    T = oracle.Tree(dup)
+    assert T.stats["overflow"] == 1 and T.stats["dots"] == 10
+
+
+def test_dense_set_p_closed_form(oracle):
+    # |n*vol - bbvol| < vol (geom.c:157-167)
+    assert oracle.dense_set_p([0, 0, 0], [3, 3, 3], 27)
+    assert not oracle.dense_set_p([0, 0, 0], [3, 3, 3], 26)
+    assert oracle.dense_set_p([0, 0, 0], [17, 1, 1], 17)
+
+
+# ----------------------------------------------------------------------
+# geometry primitives (closed forms)
+# ----------------------------------------------------------------------
+def test_hit_box_cases(oracle):
+    h, p = oracle.hit_box([0, 0, 0], [1, 1, 1], [-1, 0.5, 0.5], [1, 0, 0])
+    assert h and tuple(p) == (0.0, 0.5, 0.5)
+    # origin inside (or on) the box is a hit at the origin, any direction
+    h, p = oracle.hit_box([0, 0, 0], [1, 1, 1], [0.5, 0.5, 0.5], [-1, 0, 0])
+    assert h and tuple(p) == (0.5, 0.5, 0.5)
+    h, p = oracle.hit_box([0, 0, 0], [1, 1, 1], [1, 0.5, 0.5], [1, 0, 0])
+    assert h and tuple(p) == (1.0, 0.5, 0.5)
+    # pointing away
+    h, _ = oracle.hit_box([0, 0, 0], [1, 1, 1], [-1, 0.5, 0.5], [-1, 0, 0])
+    assert not h
+    # parallel to a slab, outside it
+    h, _ = oracle.hit_box([0, 0, 0], [1, 1, 1], [-1, 2, 0.5], [1, 0, 0])
+    assert not h
+    # corner-tied maximum uses the clamped coordinates on both axes
+    h, p = oracle.hit_box([0, 0, 0], [1, 1, 1], [-1, -1, 0.5], [1, 1, 0])
+    assert h and tuple(p) == (0.0, 0.0, 0.5)
+
+
+def test_hit_plane_and_subspace(oracle):
+    h, p = oracle.hit_plane_within_box([0, 0, 0], [1, 2, 0], [3, 3, 3], 0,
+                                       [0, 0, 0], [8, 8, 8])
+    assert h and tuple(p) == (3.0, 6.0, 0.0)
+    # origin on the plane: no intersection (geom-sse.c:123-129)
+    h, _ = oracle.hit_plane_within_box([3, 0, 0], [1, 1, 0], [3, 3, 3], 0,
+                                       [0, 0, 0], [8, 8, 8])
+    assert not h
+    # moving away from the plane
+    h, _ = oracle.hit_plane_within_box([0, 0, 0], [-1, 1, 0], [3, 3, 3], 0,
+                                       [0, 0, 0], [8, 8, 8])
+    assert not h
+    assert oracle.subspace([1, 1, 1], [0, 2, 0]) == 0b101
+    assert oracle.corrected_subspace([1, 1, 1], [1, 1, 1], [-1, 1, -1]) == \
+        0b101
+    assert oracle.corrected_subspace([1, 1, 1], [0, 1, 2], [1, -1, 1]) == \
+        0b011
+    assert oracle.abs_metric([0, 0, 0], [1, -2, 3]) == 6.0
+
+
+def test_box_ball_closed_form(oracle):
+    assert oracle.box_ball_interp([0, 0, 0], [1, 1, 1], [2, 0.5, 0.5], 1.01)
+    assert not oracle.box_ball_interp([0, 0, 0], [1, 1, 1], [2, 0.5, 0.5],
+                                      1.0)  # strict <
+    assert oracle.box_ball_interp([0, 0, 0], [1, 1, 1], [0.5, 0.5, 0.5],
+                                  1e-3)
+
+
+# ----------------------------------------------------------------------
+# search known answers
+# ----------------------------------------------------------------------
+def test_search_g676d50c(oracle):
+    # tests.c:600-659: cross-shaped voxel set, ray (-2,-2,0) + t(1,1,-1)
+    g = GOLD["search_676d50c"]
+    dots = np.array(g["voxels"], np.float32)
+    T = oracle.Tree(dots)
+    vox, hit = T.rays(np.array([g["origin"]]), np.array([g["dir"]]))
+    assert vox[0] != oracle.MISS
+    idx, h64, tie = oracle.brute64_rays(dots, np.array([g["origin"]]),
+                                        np.array([g["dir"]]))
+    assert idx[0] >= 0
+    tab = T.voxel_table()
+    if not tie[0]:
+        assert np.array_equal(tab[vox[0]], dots[idx[0]])
+
+
+def test_hit_box_benchmark_closed_form(oracle):
+    # benchmarks/voxtrees/hit-box.c:5-31: one voxel at (0,10,0); of the 10
+    # rays exactly one hits, at (0,10,0)
+    g = GOLD["hit_box_benchmark"]
+    T = oracle.Tree(np.array([g["voxel"]], np.float32))
+    j = np.arange(10)
+    o = np.stack([j - 5, np.zeros(10), np.zeros(10)], 1).astype(np.float32)
+    d = np.stack([(j - 5) / 10.0, np.ones(10), np.zeros(10)],
+                 1).astype(np.float32)
+    vox, hit = T.rays(o, d)
+    assert int((vox != oracle.MISS).sum()) == g["hits_per_10"]
+    k = int(np.nonzero(vox != oracle.MISS)[0][0])
+    assert tuple(hit[:, k]) == tuple(g["hit_point"])
+
+
+def test_ball_benchmark_closed_form(oracle):
+    # benchmarks/voxtrees/ball-intersection.c:5-28: every query collides
+    g = GOLD["ball_benchmark"]
+    T = oracle.Tree(np.array([g["voxel"]], np.float32))
+    j = np.arange(-100, 100)
+    c = np.stack([np.zeros(200), np.full(200, 10.0), j], 1)
+    out = T.balls(c, np.full(200, g["radius"], np.float32))
+    assert int(out.sum()) == g["collisions_per_200"]
+
+
+# ----------------------------------------------------------------------
+# FP64 brute force
+# ----------------------------------------------------------------------
+def _rays_around(dots, n, seed, camera_like=True):
+    from voxvision_amd import scenes
+    bmin, bmax = scenes.bbox_of(dots)
+    o, d = scenes.random_rays(n, bmin, bmax, seed=seed)
+    return o, d
+
+
+@pytest.mark.parametrize("seed", [1, 2])
+def test_rays_against_fp64_brute_force(oracle, seed):
+    rng = np.random.default_rng(seed)
+    from voxvision_amd import scenes
+    occ = rng.random((14, 12, 10)) < 0.25
+    dots = scenes.volume_to_dots(occ)
+    T = oracle.Tree(dots)
+    o, d = _rays_around(dots, 4000, seed)
+    vox, hit = T.rays(o, d)
+    idx, h64, tie = oracle.brute64_rays(dots, o, d, eps=1e-5)
+    tab = T.voxel_table()
+    ok = ~tie
+    assert ok.mean() > 0.97
+    hit32 = vox != oracle.MISS
+    assert np.array_equal(hit32[ok], (idx >= 0)[ok])
+    both = ok & hit32
+    assert np.array_equal(tab[vox[both].astype(np.int64)], dots[idx[both]])
+    scale = max(1.0, float(np.abs(o).max()), float(np.abs(dots).max()))
+    err = np.abs(hit[:, both].T.astype(np.float64) - h64[both]).max()
+    assert err <= 1e-5 * scale
+
+
+def test_camera_rays_against_fp64_brute_force(oracle):
+    from voxvision_amd import scenes
+    rng = np.random.default_rng(5)
+    occ = rng.random((16, 16, 16)) < 0.15
+    dots = scenes.volume_to_dots(occ)
+    T = oracle.Tree(dots)
+    cams = scenes.orbit_cameras((8, 8, 8), 40.0, 2, 64, 48)
+    for cam in cams:
+        _, vox, hit, _ = T.render(cam, 64, 48, oracle.QUALITY_BEST)
+        # screen2world rays in the square layout
+        dirs = np.zeros((64 * 48, 3), np.float32)
+        for r in range(64 * 48):
+            cs, p = divmod(r, 16)
+            bx, by = cs % 16, cs // 16
+            dirs[r] = oracle.screen2world(cam, bx * 4 + p % 4, by * 4 + p // 4)
+        o = np.tile(np.array(cam["position"], np.float32), (len(dirs), 1))
+        idx, h64, tie = oracle.brute64_rays(dots, o, dirs, eps=1e-5)
+        ok = ~tie
+        hit32 = vox != oracle.MISS
+        assert np.array_equal(hit32[ok], (idx >= 0)[ok])
+        tab = T.voxel_table()
+        both = ok & hit32
+        assert np.array_equal(tab[vox[both].astype(np.int64)],
+                              dots[idx[both]])
+
+
+def test_balls_against_brute_force(oracle):
+    from voxvision_amd import scenes
+    rng = np.random.default_rng(9)
+    occ = rng.random((16, 16, 16)) < 0.1
+    dots = scenes.volume_to_dots(occ)
+    T = oracle.Tree(dots)
+    bmin, bmax = scenes.bbox_of(dots)
+    c, r = scenes.random_balls(5000, bmin - 3, bmax + 3, seed=4, rmin=0.1,
+                               rmax=3.0)
+    assert np.array_equal(T.balls(c, r), oracle.brute_balls(dots, c, r))
+
+
+def test_balls_non_unit_voxel(oracle):
+    from voxvision_amd import scenes
+    rng = np.random.default_rng(10)
+    V = (0.5, 2.0, 1.25)
+    occ = rng.random((10, 8, 12)) < 0.2
+    dots = scenes.volume_to_dots(occ, V)
+    T = oracle.Tree(dots, V)
+    assert T.check() == 0
+    bmin, bmax = scenes.bbox_of(dots, V)
+    c, r = scenes.random_balls(3000, bmin - 2, bmax + 2, seed=5, rmin=0.1,
+                               rmax=2.0)
+    assert np.array_equal(T.balls(c, r), oracle.brute_balls(dots, c, r, V))
+    oracle.set_voxel((1.0, 1.0, 1.0))
+
+
+# ----------------------------------------------------------------------
+# renderer loop
+# ----------------------------------------------------------------------
+def test_render_best_equals_ray_batch(oracle):
+    from voxvision_amd import scenes
+    c1 = scenes.config_c1()
+    T = oracle.Tree(c1["dots"])
+    W, H = 64, 48
+    cam = dict(c1["camera"])
+    from voxvision_amd.camera import set_window_size
+    cam["mul"], cam["xsub"], cam["ysub"] = set_window_size(cam["fov"], W, H)
+    _, vox, hit, st = T.render(cam, W, H, oracle.QUALITY_BEST)
+    assert st["root_traversals"] == W * H
+    dirs = np.zeros((W * H, 3), np.float32)
+    for r in range(W * H):
+        cs, p = divmod(r, 16)
+        bx, by = cs % (W // 4), cs // (W // 4)
+        dirs[r] = oracle.screen2world(cam, bx * 4 + p % 4, by * 4 + p // 4)
+    o = np.tile(np.array(cam["position"], np.float32), (W * H, 1))
+    v2, h2 = T.rays(o, dirs)
+    assert np.array_equal(vox, v2)
+    m = vox != oracle.MISS
+    assert np.array_equal(hit[:, m], h2[:, m])
+
+
+def test_render_rejects_bad_geometry_and_quality(oracle):
+    T = oracle.Tree(np.zeros((1, 3), np.float32))
+    cam = dict(position=(0, -5, 0), rotation=(1, 0, 0, 0), mul=0.1,
+               xsub=1, ysub=1)
+    with pytest.raises(ValueError):
+        T.render(cam, 20, 8)     # W % 16 != 0 (renderer.c:97-100)
+    with pytest.raises(ValueError):
+        T.render(cam, 16, 6)     # H % 4 != 0
+    with pytest.raises(ValueError):  # ray merge needs ADAPTIVE
+        T.render(cam, 16, 8, oracle.QUALITY_BEST | oracle.QUALITY_RAY_MERGE)
+
+
+def test_render_adaptive_modes_consistent(oracle):
+    """ADAPTIVE/FAST pixels that hit are hits of the same ray (the leaf
+    shortcut only changes which leaf is tried first)."""
+    from voxvision_amd import scenes
+    c1 = scenes.config_c1()
+    T = oracle.Tree(c1["dots"])
+    W, H = 320, 240
+    cam = dict(c1["camera"])
+    from voxvision_amd.camera import set_window_size
+    cam["mul"], cam["xsub"], cam["ysub"] = set_window_size(cam["fov"], W, H)
+    _, vb, hb, _ = T.render(cam, W, H, oracle.QUALITY_BEST)
+    for q in (oracle.QUALITY_FAST, oracle.QUALITY_ADAPTIVE):
+        _, va, ha, st = T.render(cam, W, H, q)
+        # same hit/miss set (a leaf hit is a hit of the ray)
+        assert np.array_equal(va != oracle.MISS, vb != oracle.MISS) or \
+            ((va != oracle.MISS) & (vb == oracle.MISS)).sum() == 0
+        assert st["root_traversals"] + st["leaf_traversals"] > 0

@@ -1,0 +1,121 @@
+"""ctypes binding of libvoxgpu.so (the C ABI in include/voxgpu.h).
+
+There is no fallback: if the library is missing or cannot be loaded this
+module raises, so nothing can silently run another implementation.
+"""
+import ctypes as C
+import os
+
+PKG = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(PKG, "libvoxgpu.so")
+
+VOXG_MISS = 0xFFFFFFFF
+QUALITY_BEST = 0x0
+QUALITY_FAST = 0x1
+QUALITY_ADAPTIVE = 0x2
+QUALITY_RAY_MERGE = 0x4
+QUALITY_RAY_MERGE_ACCURATE = 0x8
+
+# Every symbol include/voxgpu.h declares.
+EXPORTS = [
+    "voxg_init", "voxg_tree_build", "voxg_tree_upload", "voxg_tree_plan",
+    "voxg_tree_plan_reference", "voxg_tree_release",
+    "voxg_tree_get_info", "voxg_tree_hash", "voxg_tree_voxel_coords",
+    "voxg_tree_node_ptr", "voxg_ray_batch", "voxg_ball_batch",
+    "voxg_camera_set_window_size", "voxg_render", "voxg_ray_batch_count",
+    "voxg_ball_batch_count", "voxg_render_count", "voxg_set_launch_config",
+    "voxg_get_launch_config", "voxg_last_error",
+]
+
+
+class VoxgError(RuntimeError):
+    pass
+
+
+class TreeInfo(C.Structure):
+    _fields_ = [("nodes", C.c_uint64), ("inner", C.c_uint64),
+                ("leaves", C.c_uint64), ("dense", C.c_uint64),
+                ("overflow", C.c_uint64), ("dots", C.c_uint64),
+                ("voxel_ids", C.c_uint64), ("depth", C.c_uint32),
+                ("stack_entries", C.c_uint32), ("device_bytes", C.c_uint64),
+                ("bbox_min", C.c_float * 3), ("bbox_max", C.c_float * 3),
+                ("voxel", C.c_float * 3)]
+
+
+class Counts(C.Structure):
+    _fields_ = [("node_visits", C.c_uint64), ("voxel_tests", C.c_uint64),
+                ("rays", C.c_uint64)]
+
+
+class Camera(C.Structure):
+    _fields_ = [("position", C.c_float * 3), ("rotation", C.c_float * 4),
+                ("mul", C.c_float), ("xsub", C.c_float), ("ysub", C.c_float),
+                ("fov", C.c_float)]
+
+
+class RenderParams(C.Structure):
+    _fields_ = [("quality", C.c_uint), ("texture", C.c_void_p),
+                ("rshift", C.c_uint32), ("gshift", C.c_uint32),
+                ("bshift", C.c_uint32), ("amask", C.c_uint32)]
+
+
+class LaunchConfig(C.Structure):
+    _fields_ = [("stack_lds", C.c_int), ("block", C.c_int),
+                ("waves_per_cu", C.c_int), ("refill", C.c_int)]
+
+
+_lib = None
+
+
+def load():
+    """Load libvoxgpu.so (raises if it has not been built)."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(LIB_PATH):
+        raise ImportError(
+            "libvoxgpu.so is not built (%s); run "
+            "`python -m voxvision_amd.build`" % LIB_PATH)
+    L = C.CDLL(LIB_PATH)
+    P = C.c_void_p
+    I = C.c_int
+    S = C.c_size_t
+    L.voxg_init.argtypes = [I]
+    L.voxg_tree_build.argtypes = [P, S, S, P, C.POINTER(P)]
+    L.voxg_tree_upload.argtypes = [P, P, C.POINTER(P)]
+    L.voxg_tree_plan.argtypes = [P, S, S, P, C.POINTER(TreeInfo),
+                                 C.POINTER(C.c_uint64)]
+    L.voxg_tree_plan_reference.argtypes = [P, P, C.POINTER(TreeInfo),
+                                           C.POINTER(C.c_uint64)]
+    L.voxg_tree_release.argtypes = [P]
+    L.voxg_tree_get_info.argtypes = [P, C.POINTER(TreeInfo)]
+    L.voxg_tree_hash.argtypes = [P]
+    L.voxg_tree_hash.restype = C.c_uint64
+    L.voxg_tree_voxel_coords.argtypes = [P, S, P, P]
+    L.voxg_tree_node_ptr.argtypes = [P, C.c_uint32]
+    L.voxg_tree_node_ptr.restype = P
+    L.voxg_ray_batch.argtypes = [P, S] + [P] * 6 + [P] * 5 + [P]
+    L.voxg_ray_batch_count.argtypes = [P, S] + [P] * 6 + [
+        C.POINTER(Counts), P]
+    L.voxg_ball_batch.argtypes = [P, S, P, P, P, P, P, P]
+    L.voxg_ball_batch_count.argtypes = [P, S, P, P, P, P,
+                                        C.POINTER(Counts), P]
+    L.voxg_camera_set_window_size.argtypes = [C.POINTER(Camera), I, I]
+    L.voxg_camera_set_window_size.restype = None
+    L.voxg_render.argtypes = [P, S, P, I, I, C.POINTER(RenderParams), P, P,
+                              P, P]
+    L.voxg_render_count.argtypes = [P, S, P, I, I, C.POINTER(RenderParams),
+                                    C.POINTER(Counts), P]
+    L.voxg_set_launch_config.argtypes = [C.POINTER(LaunchConfig)]
+    L.voxg_get_launch_config.argtypes = [C.POINTER(LaunchConfig)]
+    L.voxg_last_error.restype = C.c_char_p
+    _lib = L
+    return L
+
+
+def check(rc, what="voxg call"):
+    if rc != 0:
+        msg = load().voxg_last_error()
+        raise VoxgError("%s failed (%d): %s" %
+                        (what, rc, msg.decode() if msg else ""))
+    return rc

@@ -1,0 +1,1160 @@
+// libvoxgpu: HIP kernels (gfx950) and the C ABI declared in include/voxgpu.h.
+//
+// Ray traversal (vox_ray_tree_intersection, src/voxtrees/search.c:8-153).
+// The reference recursion stops at the first leaf that reports a hit, so it
+// is a depth-first search over (node, origin) pairs in a fixed order:
+// the child holding the entry point first, then the children behind the
+// dividing planes in L1-distance order.  Each lane keeps the pair it is
+// working on in registers and the pending pairs on a stack.  Plane points
+// depend only on the node and its entry point, so they are computed when
+// the node is visited and pushed (last-visited first); the visit order,
+// every origin and every rounding step are the reference's, which makes
+// the results bit-identical.  The stack is a ring of K entries per lane in
+// LDS; older entries spill to an HBM buffer and come back when the ring
+// empties, so any depth stays exact.
+//
+// Work distribution: persistent waves.  A wave keeps 64 rays in flight and
+// refills idle lanes from a global ray counter (one atomic per refill,
+// lanes numbered with a ballot and mbcnt), so long rays do not hold a whole
+// wave.  The counter resets itself when the last wave leaves, so a launch
+// needs no memset and can be captured in a graph.
+//
+// Floating point: compiled with -ffp-contract=off; __builtin_fmaf marks the
+// reference's fused sites (geom-sse.c:109,135; simple-camera.c:29;
+// vect-ops-sse.c:24).  Division is IEEE correctly rounded (HIP default).
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <map>
+#include <memory>
+#include <mutex>
+#include <string>
+
+#include "tree_build.h"
+#include "voxg_layout.h"
+
+using namespace voxg;
+
+// Pending-entry bound of the traversal stack: at most three pending
+// children per inner level on the current path.
+static uint32_t stack_bound_of(const HostTree &h) {
+    return h.depth > 1 ? 3u * (h.depth - 1) + 1u : 1u;
+}
+
+static size_t device_bytes_of(const HostTree &h) {
+    size_t nb = h.nodes.size() * sizeof(NodeRec);
+    return ((nb + 255) & ~(size_t)255) + h.dots.size() * sizeof(float);
+}
+
+// ---------------------------------------------------------------------------
+// errors
+// ---------------------------------------------------------------------------
+static thread_local std::string g_err;
+
+static int fail(int code, const std::string &msg) {
+    g_err = msg;
+    return code;
+}
+
+#define HIPCHK(expr)                                                       \
+    do {                                                                   \
+        hipError_t e_ = (expr);                                            \
+        if (e_ != hipSuccess)                                              \
+            return fail(VOXG_EIO, std::string(#expr ": ") +                \
+                                      hipGetErrorString(e_));              \
+    } while (0)
+
+// ---------------------------------------------------------------------------
+// device helpers
+// ---------------------------------------------------------------------------
+namespace dev {
+
+// MINPS / MAXPS lane semantics: the second operand wins unless the first
+// compares strictly smaller / larger.
+__device__ __forceinline__ float smin(float a, float b) { return a < b ? a : b; }
+__device__ __forceinline__ float smax(float a, float b) { return a > b ? a : b; }
+
+struct f3 {
+    float x, y, z;
+};
+
+// geom-sse.c:84-116 (fit_into_box :34-40)
+__device__ __forceinline__ bool hit_box(f3 bmin, f3 bmax, f3 o, f3 d, f3 &r) {
+    float fx = smin(smax(o.x, bmin.x), bmax.x);
+    float fy = smin(smax(o.y, bmin.y), bmax.y);
+    float fz = smin(smax(o.z, bmin.z), bmax.z);
+    float sx = fx - o.x, sy = fy - o.y, sz = fz - o.z;
+    if (sx == 0.0f && sy == 0.0f && sz == 0.0f) {
+        r = o;
+        return true;
+    }
+    float tx = (d.x == 0.0f) ? 0.0f : sx / d.x;
+    float ty = (d.y == 0.0f) ? 0.0f : sy / d.y;
+    float tz = (d.z == 0.0f) ? 0.0f : sz / d.z;
+    // lane-wise result of the shuffle/max network (:101-105)
+    float m0 = smax(smax(tx, tz), smax(tx, ty));
+    if (m0 <= 0.0f) return false;
+    float m1 = smax(smax(ty, tx), smax(ty, tz));
+    float m2 = smax(smax(tz, ty), smax(tz, tx));
+    float hx = (m0 == tx) ? fx : __builtin_fmaf(m0, d.x, o.x);
+    float hy = (m1 == ty) ? fy : __builtin_fmaf(m1, d.y, o.y);
+    float hz = (m2 == tz) ? fz : __builtin_fmaf(m2, d.z, o.z);
+    if (hx < bmin.x || hx > bmax.x || hy < bmin.y || hy > bmax.y ||
+        hz < bmin.z || hz > bmax.z)
+        return false;
+    r = f3{hx, hy, hz};
+    return true;
+}
+
+__device__ __forceinline__ float comp(f3 v, int i) {
+    return i == 0 ? v.x : (i == 1 ? v.y : v.z);
+}
+
+// geom-sse.c:118-142
+__device__ __forceinline__ bool hit_plane(f3 o, f3 d, f3 c, int p, f3 bmin,
+                                          f3 bmax, f3 &r) {
+    float cp = comp(c, p), op = comp(o, p), dp = comp(d, p);
+    float k = cp - op;
+    if (k == 0.0f || ((dp < 0.0f) != (k < 0.0f))) return false;
+    k = k / dp;
+    float rx = __builtin_fmaf(k, d.x, o.x);
+    float ry = __builtin_fmaf(k, d.y, o.y);
+    float rz = __builtin_fmaf(k, d.z, o.z);
+    if (rx < bmin.x || rx > bmax.x || ry < bmin.y || ry > bmax.y ||
+        rz < bmin.z || rz > bmax.z)
+        return false;
+    r = f3{rx, ry, rz};
+    return true;
+}
+
+// geom-sse.c:4-10
+__device__ __forceinline__ float abs_metric(f3 a, f3 b) {
+    return (fabsf(a.x - b.x) + fabsf(a.y - b.y)) + fabsf(a.z - b.z);
+}
+
+// geom-sse.c:70-82
+__device__ __forceinline__ int corrected_subspace(f3 c, f3 p, f3 d) {
+    int bx = (p.x == c.x) ? (d.x < 0.0f) : (p.x < c.x);
+    int by = (p.y == c.y) ? (d.y < 0.0f) : (p.y < c.y);
+    int bz = (p.z == c.z) ? (d.z < 0.0f) : (p.z < c.z);
+    return bx | (by << 1) | (bz << 2);
+}
+
+// geom-sse.c:144-149 with the radius squared precomputed (radius*radius,
+// one rounding, as in the reference expression)
+__device__ __forceinline__ bool box_ball(f3 bmin, f3 bmax, f3 c, float rr) {
+    float fx = smin(smax(c.x, bmin.x), bmax.x);
+    float fy = smin(smax(c.y, bmin.y), bmax.y);
+    float fz = smin(smax(c.z, bmin.z), bmax.z);
+    float dx = fx - c.x, dy = fy - c.y, dz = fz - c.z;
+    dx = dx * dx;
+    dy = dy * dy;
+    dz = dz * dz;
+    return ((dx + dy) + dz) < rr;
+}
+
+struct Tree {
+    const NodeRec *__restrict__ nodes;
+    const float *__restrict__ dots;
+    float vx, vy, vz;
+    uint32_t empty;
+};
+
+struct Node {
+    f3 bmin, bmax, c;
+    uint32_t ref, info, escape;
+};
+
+__device__ __forceinline__ Node load_node(const NodeRec *__restrict__ nodes,
+                                          uint32_t i) {
+    const float4 *p = reinterpret_cast<const float4 *>(nodes + i);
+    float4 a = p[0], b = p[1], c = p[2];
+    Node n;
+    n.bmin = f3{a.x, a.y, a.z};
+    n.bmax = f3{a.w, b.x, b.y};
+    n.ref = __float_as_uint(b.z);
+    n.info = __float_as_uint(b.w);
+    n.c = f3{c.x, c.y, c.z};
+    n.escape = __float_as_uint(c.w);
+    return n;
+}
+
+// Voxel id of a dense-leaf hit: floor((res - min)/V) clamped to the leaf's
+// dimensions (get_dimensions, geom.c:192-196).
+__device__ __forceinline__ uint32_t dense_voxel(const Node &n, f3 r,
+                                                const Tree &t) {
+    float dxf = (n.bmax.x - n.bmin.x) / t.vx;
+    float dyf = (n.bmax.y - n.bmin.y) / t.vy;
+    float dzf = (n.bmax.z - n.bmin.z) / t.vz;
+    int dx = dxf > 0.0f ? (int)(uint32_t)dxf : 0;
+    int dy = dyf > 0.0f ? (int)(uint32_t)dyf : 0;
+    int dz = dzf > 0.0f ? (int)(uint32_t)dzf : 0;
+    int ix = (int)floorf((r.x - n.bmin.x) / t.vx);
+    int iy = (int)floorf((r.y - n.bmin.y) / t.vy);
+    int iz = (int)floorf((r.z - n.bmin.z) / t.vz);
+    ix = ix < 0 ? 0 : (ix > dx - 1 ? dx - 1 : ix);
+    iy = iy < 0 ? 0 : (iy > dy - 1 ? dy - 1 : iy);
+    iz = iz < 0 ? 0 : (iz > dz - 1 ? dz - 1 : iz);
+    return n.ref + (uint32_t)((ix * dy + iy) * dz + iz);
+}
+
+__device__ __forceinline__ uint32_t child_of(uint32_t ref, uint32_t mask,
+                                             int s) {
+    return ((mask >> s) & 1u) ? ref + __popc(mask & ((1u << s) - 1u)) : kEnd;
+}
+
+__device__ __forceinline__ uint32_t lane_id() {
+    return __builtin_amdgcn_mbcnt_hi(~0u, __builtin_amdgcn_mbcnt_lo(~0u, 0u));
+}
+
+__device__ __forceinline__ uint32_t lanes_below(uint64_t mask) {
+    return __builtin_amdgcn_mbcnt_hi((uint32_t)(mask >> 32),
+                                     __builtin_amdgcn_mbcnt_lo((uint32_t)mask,
+                                                               0u));
+}
+
+// ---------------------------------------------------------------------------
+// ray sources and sinks
+// ---------------------------------------------------------------------------
+struct SoaSrc {
+    const float *ox, *oy, *oz, *dx, *dy, *dz;
+    __device__ __forceinline__ void get(uint32_t r, f3 &o, f3 &d) const {
+        o = f3{ox[r], oy[r], oz[r]};
+        d = f3{dx[r], dy[r], dz[r]};
+    }
+};
+
+struct SoaSink {
+    uint32_t *voxel, *leaf;
+    float *hx, *hy, *hz;
+    __device__ __forceinline__ void put(uint32_t r, bool hit, uint32_t vid,
+                                        uint32_t leafi, f3 p) const {
+        if (voxel) voxel[r] = hit ? vid : kEnd;
+        if (leaf) leaf[r] = hit ? leafi : kEnd;
+        if (hit && hx) {
+            hx[r] = p.x;
+            hy[r] = p.y;
+            hz[r] = p.z;
+        }
+    }
+};
+
+struct CamDev {
+    float px, py, pz, qw, qi, qj, qk, mul, xsub, ysub, fov;
+};
+
+// Camera rays in the square layout: ray r of a frame is pixel index
+// cs*16 + p, i.e. 16x4-pixel tiles of four 4x4 blocks per 64 rays.
+struct CamSrc {
+    const CamDev *cams;
+    uint32_t W, per_frame;
+    __device__ __forceinline__ void get(uint32_t r, f3 &o, f3 &d) const {
+        uint32_t f = r / per_frame, q = r - f * per_frame;
+        uint32_t row = 4u * W;
+        uint32_t ty = q / row, rem = q - ty * row;
+        uint32_t tx = rem >> 6, l = rem & 63u;
+        uint32_t b = l >> 4, p = l & 15u;
+        int sx = (int)((tx * 4u + b) * 4u + (p & 3u));
+        int sy = (int)(ty * 4u + (p >> 2));
+        const CamDev &c = cams[f];
+        o = f3{c.px, c.py, c.pz};
+        // simple-camera.c:29, then vect-ops-sse.c:16-26 with the vector in
+        // quaternion lanes (0, vx, vy, vz)
+        float vx = __builtin_fmaf(c.mul, (float)sx, -c.xsub);
+        float vy = 1.0f;
+        float vz = __builtin_fmaf(-c.mul, (float)sy, c.ysub);
+        float t1 = (c.qj * vz - vy * c.qk) * 2.0f;
+        float t2 = (c.qk * vx - vz * c.qi) * 2.0f;
+        float t3 = (c.qi * vy - vx * c.qj) * 2.0f;
+        float c1 = c.qj * t3 - t2 * c.qk;
+        float c2 = c.qk * t1 - t3 * c.qi;
+        float c3 = c.qi * t2 - t1 * c.qj;
+        d = f3{__builtin_fmaf(c.qw, t1, vx) + c1,
+               __builtin_fmaf(c.qw, t2, vy) + c2,
+               __builtin_fmaf(c.qw, t3, vz) + c3};
+    }
+};
+
+struct Shade {
+    const uint8_t *tex;
+    uint32_t rshift, gshift, bshift, amask;
+    float vx, vy, vz;
+    // renderer.c:27-52 without a light manager
+    __device__ __forceinline__ uint32_t color(f3 p) const {
+        int x = abs((int)(p.x / vx)) & 31;
+        int y = abs((int)(p.y / vy)) & 31;
+        int z = abs((int)(p.z / vz)) & 31;
+        uint32_t c = tex[x * 1024 + y * 32 + z];
+        return (c << rshift) | (c << gshift) | (c << bshift) | amask;
+    }
+};
+
+struct PixSink {
+    uint32_t *squares, *voxel;
+    float *hit;
+    size_t plane;  // floats per hit plane
+    Shade sh;
+    __device__ __forceinline__ void put(uint32_t r, bool hit, uint32_t vid,
+                                        uint32_t, f3 p) const {
+        if (voxel) voxel[r] = hit ? vid : kEnd;
+        if (hit) {
+            if (this->hit) {
+                this->hit[r] = p.x;
+                this->hit[plane + r] = p.y;
+                this->hit[2 * plane + r] = p.z;
+            }
+            if (squares) squares[r] = sh.color(p);
+        }
+    }
+};
+
+struct Ctrl {
+    uint32_t next;     // next ray to hand out
+    uint32_t left;     // waves that have finished
+    uint32_t overflow; // set when a stack bound was exceeded (never expected)
+    uint32_t pad;
+    unsigned long long visits, tests, rays;
+};
+
+// ---------------------------------------------------------------------------
+// persistent traversal kernel
+// ---------------------------------------------------------------------------
+template <int K, class Src, class Sink, bool COUNT>
+__global__ void __launch_bounds__(256)
+rti_kernel(Tree t, Src src, Sink sink, uint32_t n, Ctrl *ctrl,
+           uint4 *__restrict__ spill, uint32_t spill_slots, uint32_t refill) {
+    extern __shared__ uint4 ring[];  // [K][blockDim.x]
+    const uint32_t tid = threadIdx.x;
+    const uint32_t bdim = blockDim.x;
+    const uint32_t gtid = blockIdx.x * bdim + tid;
+    const uint32_t gthreads = gridDim.x * bdim;
+    const uint32_t lane = lane_id();
+    const f3 V = f3{t.vx, t.vy, t.vz};
+
+    bool active = false;
+    bool exhausted = (t.empty != 0);
+    uint32_t ray = 0, cur = 0;
+    f3 o{0, 0, 0}, d{0, 0, 0}, org{0, 0, 0};
+    int sp = 0, lo = 0;
+    unsigned long long nv = 0, nt = 0, nr = 0;
+
+    if (t.empty) {
+        // every query misses; hand out all rays as misses
+        for (uint32_t r = gtid; r < n; r += gthreads)
+            sink.put(r, false, kEnd, kEnd, o);
+    }
+
+    for (;;) {
+        uint64_t idle = __ballot(!active);
+        if (!exhausted && idle != 0 &&
+            (__popcll(idle) >= refill || idle == ~0ull)) {
+            uint32_t cnt = (uint32_t)__popcll(idle);
+            uint32_t leader = (uint32_t)(__ffsll((long long)idle) - 1);
+            uint32_t base = 0;
+            if (lane == leader) base = atomicAdd(&ctrl->next, cnt);
+            base = __shfl(base, (int)leader);
+            base = __builtin_amdgcn_readfirstlane(base);
+            if (base + cnt >= n) exhausted = true;
+            if (!active) {
+                uint32_t my = base + lanes_below(idle);
+                if (my < n) {
+                    ray = my;
+                    src.get(ray, o, d);
+                    org = o;
+                    cur = 0;
+                    sp = lo = 0;
+                    active = true;
+                    if (COUNT) nr++;
+                }
+            }
+        }
+        if (__ballot(active) == 0) {
+            if (exhausted) break;
+            continue;
+        }
+        if (!active) continue;
+
+        // ---- process (cur, org) ----
+        Node nd = load_node(t.nodes, cur);
+        if (COUNT) nv++;
+        f3 e;
+        bool finished = false, hit = false, descend = false;
+        uint32_t vid = kEnd;
+        if (hit_box(nd.bmin, nd.bmax, org, d, e)) {
+            uint32_t type = nd.info & NT_TYPE_MASK;
+            if (type == NT_DENSE) {
+                // search.c:31-37
+                hit = true;
+                vid = dense_voxel(nd, e, t);
+            } else if (type == NT_LEAF) {
+                // search.c:39-81
+                uint32_t cnt = nd.info >> 8;
+                float best = INFINITY;
+                f3 bres = e;
+                for (uint32_t k = 0; k < cnt; k++) {
+                    const float *p = t.dots + 3u * (nd.ref + k);
+                    f3 vmin = f3{p[0], p[1], p[2]};
+                    f3 vmax = f3{vmin.x + V.x, vmin.y + V.y, vmin.z + V.z};
+                    f3 far;
+                    if (COUNT) nt++;
+                    if (hit_box(vmin, vmax, e, d, far)) {
+                        float m = abs_metric(e, far);
+                        if (m == 0.0f) {
+                            bres = far;
+                            vid = nd.ref + k;
+                            hit = true;
+                            break;
+                        }
+                        if (m < best) {
+                            best = m;
+                            bres = far;
+                            vid = nd.ref + k;
+                            hit = true;
+                        }
+                    }
+                }
+                e = bres;
+            } else {
+                // search.c:84-148
+                int s = corrected_subspace(nd.c, e, d);
+                f3 P0, P1, P2;
+                bool h0 = hit_plane(e, d, nd.c, 0, nd.bmin, nd.bmax, P0);
+                bool h1 = hit_plane(e, d, nd.c, 1, nd.bmin, nd.bmax, P1);
+                bool h2 = hit_plane(e, d, nd.c, 2, nd.bmin, nd.bmax, P2);
+                // compact the hits in plane order (:108-113)
+                int cntp = 0, i0 = 0, i1 = 0, i2 = 0;
+                f3 Q0 = P0, Q1 = P1, Q2 = P2;
+                if (h0) {
+                    i0 = 0;
+                    Q0 = P0;
+                    cntp = 1;
+                }
+                if (h1) {
+                    if (cntp == 0) {
+                        i0 = 1;
+                        Q0 = P1;
+                    } else {
+                        i1 = 1;
+                        Q1 = P1;
+                    }
+                    cntp++;
+                }
+                if (h2) {
+                    if (cntp == 0) {
+                        i0 = 2;
+                        Q0 = P2;
+                    } else if (cntp == 1) {
+                        i1 = 2;
+                        Q1 = P2;
+                    } else {
+                        i2 = 2;
+                        Q2 = P2;
+                    }
+                    cntp++;
+                }
+                // selection sort by L1 distance from the entry (:115-134)
+                float m0 = abs_metric(e, Q0), m1 = abs_metric(e, Q1),
+                      m2 = abs_metric(e, Q2);
+                if (cntp >= 2 && m1 < m0) {
+                    f3 tq = Q0; Q0 = Q1; Q1 = tq;
+                    int ti = i0; i0 = i1; i1 = ti;
+                    float tm = m0; m0 = m1; m1 = tm;
+                }
+                if (cntp >= 3 && m2 < m0) {
+                    f3 tq = Q0; Q0 = Q2; Q2 = tq;
+                    int ti = i0; i0 = i2; i2 = ti;
+                    float tm = m0; m0 = m2; m2 = tm;
+                }
+                if (cntp >= 3 && m2 < m1) {
+                    f3 tq = Q1; Q1 = Q2; Q2 = tq;
+                    int ti = i1; i1 = i2; i2 = ti;
+                }
+                uint32_t mask = (nd.info >> 8) & 0xffu;
+                int s1 = s ^ (1 << i0);
+                int s2 = s1 ^ (1 << i1);
+                int s3 = s2 ^ (1 << i2);
+                uint32_t cs0 = child_of(nd.ref, mask, s);
+                uint32_t cs1 = cntp >= 1 ? child_of(nd.ref, mask, s1) : kEnd;
+                uint32_t cs2 = cntp >= 2 ? child_of(nd.ref, mask, s2) : kEnd;
+                uint32_t cs3 = cntp >= 3 ? child_of(nd.ref, mask, s3) : kEnd;
+                // push in reverse visiting order (:137-147)
+#define VOXG_PUSH(CI, Q)                                                      \
+    if ((CI) != kEnd) {                                                      \
+        if (sp - lo == K) {                                                  \
+            if ((uint32_t)lo >= spill_slots) {                               \
+                atomicOr(&ctrl->overflow, 1u);                               \
+            } else {                                                         \
+                spill[(size_t)lo * gthreads + gtid] =                        \
+                    ring[(lo & (K - 1)) * bdim + tid];                       \
+            }                                                                \
+            lo++;                                                            \
+        }                                                                    \
+        ring[(sp & (K - 1)) * bdim + tid] =                                  \
+            make_uint4((CI), __float_as_uint((Q).x), __float_as_uint((Q).y), \
+                       __float_as_uint((Q).z));                              \
+        sp++;                                                                \
+    }
+                VOXG_PUSH(cs3, Q2)
+                VOXG_PUSH(cs2, Q1)
+                VOXG_PUSH(cs1, Q0)
+#undef VOXG_PUSH
+                if (cs0 != kEnd) {
+                    cur = cs0;
+                    org = e;
+                    descend = true;
+                }
+            }
+        }
+        if (hit) {
+            sink.put(ray, true, vid, cur, e);
+            finished = true;
+        } else if (!descend) {
+            if (sp == 0) {
+                sink.put(ray, false, kEnd, kEnd, e);
+                finished = true;
+            } else {
+                uint4 v;
+                if (sp == lo) {
+                    sp--;
+                    lo--;
+                    v = ((uint32_t)lo < spill_slots)
+                            ? spill[(size_t)lo * gthreads + gtid]
+                            : make_uint4(kEnd, 0, 0, 0);
+                } else {
+                    sp--;
+                    v = ring[(sp & (K - 1)) * bdim + tid];
+                }
+                cur = v.x;
+                org = f3{__uint_as_float(v.y), __uint_as_float(v.z),
+                         __uint_as_float(v.w)};
+                if (cur == kEnd) {  // only after a reported overflow
+                    sink.put(ray, false, kEnd, kEnd, e);
+                    finished = true;
+                }
+            }
+        }
+        if (finished) active = false;
+    }
+
+    if (COUNT) {
+        atomicAdd(&ctrl->visits, nv);
+        atomicAdd(&ctrl->tests, nt);
+        atomicAdd(&ctrl->rays, nr);
+    }
+    // self-reset of the ray counter by the last wave to leave
+    __threadfence();
+    if (lane == 0) {
+        uint32_t total = gridDim.x * (bdim >> 6);
+        uint32_t done = atomicAdd(&ctrl->left, 1u);
+        if (done == total - 1) {
+            atomicExch(&ctrl->next, 0u);
+            atomicExch(&ctrl->left, 0u);
+        }
+    }
+}
+
+// ---------------------------------------------------------------------------
+// ball kernel (search.c:155-183): any-hit depth-first walk over children
+// 0..7 with escape links; the answer does not depend on the order.
+// ---------------------------------------------------------------------------
+template <bool COUNT>
+__global__ void __launch_bounds__(256)
+ball_kernel(Tree t, uint32_t n, const float *__restrict__ cx,
+            const float *__restrict__ cy, const float *__restrict__ cz,
+            const float *__restrict__ rad, uint8_t *__restrict__ out,
+            Ctrl *ctrl) {
+    uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    unsigned long long nv = 0, nt = 0;
+    if (i < n) {
+        f3 c = f3{cx[i], cy[i], cz[i]};
+        float r = rad[i];
+        float rr = r * r;
+        const f3 V = f3{t.vx, t.vy, t.vz};
+        uint8_t res = 0;
+        uint32_t node = t.empty ? kEnd : 0u;
+        while (node != kEnd) {
+            Node nd = load_node(t.nodes, node);
+            if (COUNT) nv++;
+            if (box_ball(nd.bmin, nd.bmax, c, rr)) {
+                uint32_t type = nd.info & NT_TYPE_MASK;
+                if (type == NT_DENSE) {
+                    res = 1;
+                    break;
+                }
+                if (type == NT_LEAF) {
+                    uint32_t cnt = nd.info >> 8;
+                    for (uint32_t k = 0; k < cnt; k++) {
+                        const float *p = t.dots + 3u * (nd.ref + k);
+                        f3 vmin = f3{p[0], p[1], p[2]};
+                        f3 vmax = f3{vmin.x + V.x, vmin.y + V.y, vmin.z + V.z};
+                        if (COUNT) nt++;
+                        if (box_ball(vmin, vmax, c, rr)) {
+                            res = 1;
+                            break;
+                        }
+                    }
+                    if (res) break;
+                    node = nd.escape;
+                } else {
+                    node = (nd.ref != kEnd) ? nd.ref : nd.escape;
+                }
+            } else {
+                node = nd.escape;
+            }
+        }
+        if (out) out[i] = res;
+    }
+    if (COUNT) {
+        atomicAdd(&ctrl->visits, nv);
+        atomicAdd(&ctrl->tests, nt);
+        if (i < n) atomicAdd(&ctrl->rays, 1ull);
+    }
+}
+
+}  // namespace dev
+
+// ---------------------------------------------------------------------------
+// host side
+// ---------------------------------------------------------------------------
+struct voxg_tree {
+    HostTree host;
+    NodeRec *d_nodes = nullptr;
+    float *d_dots = nullptr;
+    void *d_mem = nullptr;
+    size_t bytes = 0;
+    int device = 0;
+    uint32_t stack_bound = 0;
+};
+
+namespace {
+
+std::mutex g_mu;
+voxg_launch_config g_cfg = {8, 256, 16, 16};
+thread_local int g_device = 0;
+
+struct Workspace {
+    dev::Ctrl *ctrl = nullptr;
+    uint4 *spill = nullptr;
+    size_t spill_bytes = 0;
+};
+std::map<std::pair<int, hipStream_t>, std::unique_ptr<Workspace>> g_ws;
+
+int cu_count(int device) {
+    static std::map<int, int> cache;
+    std::lock_guard<std::mutex> g(g_mu);
+    auto it = cache.find(device);
+    if (it != cache.end()) return it->second;
+    int n = 0;
+    if (hipDeviceGetAttribute(&n, hipDeviceAttributeMultiprocessorCount,
+                              device) != hipSuccess || n <= 0)
+        n = 256;
+    cache[device] = n;
+    return n;
+}
+
+int get_ws(hipStream_t s, size_t spill_bytes, Workspace **out) {
+    int devid = 0;
+    HIPCHK(hipGetDevice(&devid));
+    std::lock_guard<std::mutex> g(g_mu);
+    auto &slot = g_ws[{devid, s}];
+    if (!slot) {
+        slot.reset(new Workspace());
+        HIPCHK(hipMalloc(&slot->ctrl, sizeof(dev::Ctrl)));
+        HIPCHK(hipMemset(slot->ctrl, 0, sizeof(dev::Ctrl)));
+    }
+    if (spill_bytes > slot->spill_bytes) {
+        if (slot->spill) HIPCHK(hipFree(slot->spill));
+        slot->spill = nullptr;
+        slot->spill_bytes = 0;
+        HIPCHK(hipMalloc(&slot->spill, spill_bytes));
+        slot->spill_bytes = spill_bytes;
+    }
+    *out = slot.get();
+    return 0;
+}
+
+dev::Tree dev_tree(const voxg_tree *t) {
+    dev::Tree d;
+    d.nodes = t->d_nodes;
+    d.dots = t->d_dots;
+    d.vx = t->host.voxel[0];
+    d.vy = t->host.voxel[1];
+    d.vz = t->host.voxel[2];
+    d.empty = t->host.nodes.empty() ? 1u : 0u;
+    return d;
+}
+
+template <int K, class Src, class Sink, bool COUNT>
+int launch_k(const voxg_tree *t, const Src &src, const Sink &sink, size_t n,
+             hipStream_t stream, const voxg_launch_config &cfg,
+             voxg_counts *counts) {
+    int devid = 0;
+    HIPCHK(hipGetDevice(&devid));
+    int cus = cu_count(devid);
+    int block = cfg.block;
+    int waves = std::max(1, cfg.waves_per_cu);
+    long long threads = (long long)cus * waves * 64;
+    int grid = (int)std::max<long long>(1, threads / block);
+    // never launch more lanes than rays need (rounded up to whole blocks)
+    long long need_blocks = ((long long)n + block - 1) / block;
+    if (need_blocks < 1) need_blocks = 1;
+    if (grid > need_blocks) grid = (int)need_blocks;
+    uint32_t gthreads = (uint32_t)grid * (uint32_t)block;
+    uint32_t slots = t->stack_bound > (uint32_t)K ? t->stack_bound - K : 0u;
+    Workspace *ws = nullptr;
+    int rc = get_ws(stream, (size_t)std::max<uint32_t>(slots, 1u) * gthreads *
+                                sizeof(uint4),
+                    &ws);
+    if (rc) return rc;
+    size_t lds = (size_t)K * block * sizeof(uint4);
+    auto kern = dev::rti_kernel<K, Src, Sink, COUNT>;
+    if (lds > 65536) {
+        HIPCHK(hipFuncSetAttribute((const void *)kern,
+                                   hipFuncAttributeMaxDynamicSharedMemorySize,
+                                   (int)lds));
+    }
+    if (COUNT) {
+        HIPCHK(hipMemsetAsync(&ws->ctrl->visits, 0, 3 * sizeof(unsigned long long),
+                              stream));
+    }
+    hipLaunchKernelGGL(kern, dim3(grid), dim3(block), lds, stream,
+                       dev_tree(t), src, sink, (uint32_t)n, ws->ctrl,
+                       ws->spill, slots, (uint32_t)cfg.refill);
+    HIPCHK(hipGetLastError());
+    if (COUNT) {
+        dev::Ctrl h;
+        HIPCHK(hipMemcpyAsync(&h, ws->ctrl, sizeof(h), hipMemcpyDeviceToHost,
+                              stream));
+        HIPCHK(hipStreamSynchronize(stream));
+        if (h.overflow)
+            return fail(VOXG_EIO, "traversal stack bound exceeded");
+        counts->node_visits = h.visits;
+        counts->voxel_tests = h.tests;
+        counts->rays = h.rays;
+    }
+    return 0;
+}
+
+template <class Src, class Sink, bool COUNT>
+int launch(const voxg_tree *t, const Src &src, const Sink &sink, size_t n,
+           hipStream_t stream, voxg_counts *counts) {
+    voxg_launch_config cfg;
+    {
+        std::lock_guard<std::mutex> g(g_mu);
+        cfg = g_cfg;
+    }
+    switch (cfg.stack_lds) {
+    case 2: return launch_k<2, Src, Sink, COUNT>(t, src, sink, n, stream, cfg, counts);
+    case 4: return launch_k<4, Src, Sink, COUNT>(t, src, sink, n, stream, cfg, counts);
+    case 8: return launch_k<8, Src, Sink, COUNT>(t, src, sink, n, stream, cfg, counts);
+    case 16: return launch_k<16, Src, Sink, COUNT>(t, src, sink, n, stream, cfg, counts);
+    case 32: return launch_k<32, Src, Sink, COUNT>(t, src, sink, n, stream, cfg, counts);
+    default: return fail(VOXG_EINVAL, "stack_lds must be 2, 4, 8, 16 or 32");
+    }
+}
+
+int check_tree(const voxg_tree *t) {
+    if (!t) return fail(VOXG_EINVAL, "tree is NULL");
+    int devid = 0;
+    HIPCHK(hipGetDevice(&devid));
+    if (devid != t->device)
+        return fail(VOXG_EINVAL, "tree lives on another device");
+    return 0;
+}
+
+int upload(voxg_tree *t) {
+    const HostTree &h = t->host;
+    size_t nb = h.nodes.size() * sizeof(NodeRec);
+    size_t db = h.dots.size() * sizeof(float);
+    size_t nb_al = (nb + 255) & ~(size_t)255;
+    t->bytes = nb_al + db;
+    HIPCHK(hipGetDevice(&t->device));
+    if (t->bytes == 0) return 0;
+    HIPCHK(hipMalloc(&t->d_mem, t->bytes));
+    t->d_nodes = (NodeRec *)t->d_mem;
+    t->d_dots = (float *)((char *)t->d_mem + nb_al);
+    if (nb) HIPCHK(hipMemcpy(t->d_nodes, h.nodes.data(), nb, hipMemcpyHostToDevice));
+    if (db) HIPCHK(hipMemcpy(t->d_dots, h.dots.data(), db, hipMemcpyHostToDevice));
+    return 0;
+}
+
+int finish_tree(voxg_tree *t, voxg_tree **out) {
+    t->stack_bound = stack_bound_of(t->host);
+    int rc = upload(t);
+    if (rc) {
+        if (t->d_mem) (void)hipFree(t->d_mem);
+        delete t;
+        return rc;
+    }
+    *out = t;
+    return 0;
+}
+
+}  // namespace
+
+extern "C" {
+
+const char *voxg_last_error(void) { return g_err.c_str(); }
+
+int voxg_init(int device) {
+    int n = 0;
+    HIPCHK(hipGetDeviceCount(&n));
+    if (device < 0 || device >= n)
+        return fail(VOXG_ENODEV, "no such HIP device");
+    HIPCHK(hipSetDevice(device));
+    hipDeviceProp_t p;
+    HIPCHK(hipGetDeviceProperties(&p, device));
+    if (std::strncmp(p.gcnArchName, "gfx950", 6) != 0)
+        return fail(VOXG_ENODEV, std::string("device is ") + p.gcnArchName +
+                                     ", this library is built for gfx950");
+    g_device = device;
+    return 0;
+}
+
+int voxg_tree_build(const float *xyz, size_t n, size_t stride,
+                    const float voxel[3], voxg_tree **out) {
+    if (!out) return fail(VOXG_EINVAL, "out is NULL");
+    *out = nullptr;
+    if (n && !xyz) return fail(VOXG_EINVAL, "xyz is NULL");
+    if (stride < 3) return fail(VOXG_EINVAL, "stride must be >= 3");
+    float V[3] = {1.0f, 1.0f, 1.0f};
+    if (voxel) std::memcpy(V, voxel, sizeof(V));
+    if (!(V[0] > 0.0f && V[1] > 0.0f && V[2] > 0.0f))
+        return fail(VOXG_EINVAL, "voxel size must be positive");
+    std::unique_ptr<voxg_tree> t(new (std::nothrow) voxg_tree());
+    if (!t) return fail(VOXG_ENOMEM, "host allocation failed");
+    std::string err;
+    int rc;
+    try {
+        rc = build_tree(xyz, n, stride, V, t->host, err);
+    } catch (const std::bad_alloc &) {
+        return fail(VOXG_ENOMEM, "host allocation failed");
+    }
+    if (rc) return fail(rc, err);
+    return finish_tree(t.release(), out);
+}
+
+int voxg_tree_upload(const struct vox_node *root, const float voxel[3],
+                     voxg_tree **out) {
+    if (!out) return fail(VOXG_EINVAL, "out is NULL");
+    *out = nullptr;
+    float V[3] = {1.0f, 1.0f, 1.0f};
+    if (voxel) std::memcpy(V, voxel, sizeof(V));
+    if (!(V[0] > 0.0f && V[1] > 0.0f && V[2] > 0.0f))
+        return fail(VOXG_EINVAL, "voxel size must be positive");
+    std::unique_ptr<voxg_tree> t(new (std::nothrow) voxg_tree());
+    if (!t) return fail(VOXG_ENOMEM, "host allocation failed");
+    std::string err;
+    int rc;
+    try {
+        rc = flatten_reference(root, V, t->host, err);
+    } catch (const std::bad_alloc &) {
+        return fail(VOXG_ENOMEM, "host allocation failed");
+    }
+    if (rc) return fail(rc, err);
+    return finish_tree(t.release(), out);
+}
+
+int voxg_tree_release(voxg_tree *t) {
+    if (!t) return 0;
+    if (t->d_mem) HIPCHK(hipFree(t->d_mem));
+    delete t;
+    return 0;
+}
+
+static void fill_info(const HostTree &h, voxg_tree_info *o) {
+    std::memset(o, 0, sizeof(*o));
+    o->nodes = h.nodes.size();
+    o->inner = h.inner;
+    o->leaves = h.leaves;
+    o->dense = h.dense;
+    o->overflow = h.overflow;
+    o->dots = h.ndots;
+    o->voxel_ids = h.voxel_ids;
+    o->depth = h.depth;
+    o->stack_entries = stack_bound_of(h);
+    o->device_bytes = h.nodes.empty() ? 0 : device_bytes_of(h);
+    if (!h.nodes.empty()) {
+        for (int l = 0; l < 3; l++) {
+            o->bbox_min[l] = h.nodes[0].bmin[l];
+            o->bbox_max[l] = h.nodes[0].bmax[l];
+        }
+    }
+    for (int l = 0; l < 3; l++) o->voxel[l] = h.voxel[l];
+}
+
+int voxg_tree_get_info(const voxg_tree *t, voxg_tree_info *o) {
+    if (!t || !o) return fail(VOXG_EINVAL, "NULL argument");
+    fill_info(t->host, o);
+    return 0;
+}
+
+int voxg_tree_plan(const float *xyz, size_t n, size_t stride,
+                   const float voxel[3], voxg_tree_info *info,
+                   uint64_t *hash) {
+    if (n && !xyz) return fail(VOXG_EINVAL, "xyz is NULL");
+    if (stride < 3) return fail(VOXG_EINVAL, "stride must be >= 3");
+    float V[3] = {1.0f, 1.0f, 1.0f};
+    if (voxel) std::memcpy(V, voxel, sizeof(V));
+    if (!(V[0] > 0.0f && V[1] > 0.0f && V[2] > 0.0f))
+        return fail(VOXG_EINVAL, "voxel size must be positive");
+    HostTree h;
+    std::string err;
+    int rc;
+    try {
+        rc = build_tree(xyz, n, stride, V, h, err);
+    } catch (const std::bad_alloc &) {
+        return fail(VOXG_ENOMEM, "host allocation failed");
+    }
+    if (rc) return fail(rc, err);
+    if (info) fill_info(h, info);
+    if (hash) *hash = tree_hash(h);
+    return 0;
+}
+
+int voxg_tree_plan_reference(const struct vox_node *root, const float voxel[3],
+                             voxg_tree_info *info, uint64_t *hash) {
+    float V[3] = {1.0f, 1.0f, 1.0f};
+    if (voxel) std::memcpy(V, voxel, sizeof(V));
+    if (!(V[0] > 0.0f && V[1] > 0.0f && V[2] > 0.0f))
+        return fail(VOXG_EINVAL, "voxel size must be positive");
+    HostTree h;
+    std::string err;
+    int rc;
+    try {
+        rc = flatten_reference(root, V, h, err);
+    } catch (const std::bad_alloc &) {
+        return fail(VOXG_ENOMEM, "host allocation failed");
+    }
+    if (rc) return fail(rc, err);
+    if (info) fill_info(h, info);
+    if (hash) *hash = tree_hash(h);
+    return 0;
+}
+
+uint64_t voxg_tree_hash(const voxg_tree *t) {
+    return t ? tree_hash(t->host) : 0;
+}
+
+int voxg_tree_voxel_coords(const voxg_tree *t, size_t n, const uint32_t *ids,
+                           float *xyz) {
+    if (!t || (n && (!ids || !xyz))) return fail(VOXG_EINVAL, "NULL argument");
+    return voxel_coords(t->host, n, ids, xyz);
+}
+
+const struct vox_node *voxg_tree_node_ptr(const voxg_tree *t, uint32_t i) {
+    if (!t || i >= t->host.src.size()) return nullptr;
+    return (const struct vox_node *)t->host.src[i];
+}
+
+void voxg_camera_set_window_size(voxg_camera *cam, int w, int h) {
+    // simple-camera.c:120-136
+    if (!cam) return;
+    if (w > h) {
+        cam->xsub = cam->fov;
+        cam->ysub = (cam->fov * (float)h) / (float)w;
+        cam->mul = (cam->fov * 2.0f) / (float)w;
+    } else {
+        cam->ysub = cam->fov;
+        cam->xsub = (cam->fov * (float)w) / (float)h;
+        cam->mul = (cam->fov * 2.0f) / (float)h;
+    }
+}
+
+int voxg_set_launch_config(const voxg_launch_config *cfg) {
+    if (!cfg) return fail(VOXG_EINVAL, "cfg is NULL");
+    int k = cfg->stack_lds;
+    if (k != 2 && k != 4 && k != 8 && k != 16 && k != 32)
+        return fail(VOXG_EINVAL, "stack_lds must be 2, 4, 8, 16 or 32");
+    if (cfg->block < 64 || cfg->block > 256 || cfg->block % 64)
+        return fail(VOXG_EINVAL, "block must be 64, 128, 192 or 256");
+    if ((size_t)k * cfg->block * 16 > 160 * 1024)
+        return fail(VOXG_EINVAL, "stack_lds * block * 16 exceeds 160 KiB of LDS");
+    if (cfg->waves_per_cu < 1 || cfg->waves_per_cu > 64)
+        return fail(VOXG_EINVAL, "waves_per_cu must be in 1..64");
+    if (cfg->refill < 1 || cfg->refill > 64)
+        return fail(VOXG_EINVAL, "refill must be in 1..64");
+    std::lock_guard<std::mutex> g(g_mu);
+    g_cfg = *cfg;
+    return 0;
+}
+
+int voxg_get_launch_config(voxg_launch_config *cfg) {
+    if (!cfg) return fail(VOXG_EINVAL, "cfg is NULL");
+    std::lock_guard<std::mutex> g(g_mu);
+    *cfg = g_cfg;
+    return 0;
+}
+
+static int ray_common(const voxg_tree *t, size_t n, const float *ox,
+                      const float *oy, const float *oz, const float *dx,
+                      const float *dy, const float *dz) {
+    int rc = check_tree(t);
+    if (rc) return rc;
+    if (n >= 0x80000000ull) return fail(VOXG_ERANGE, "n must be < 2^31");
+    if (n && (!ox || !oy || !oz || !dx || !dy || !dz))
+        return fail(VOXG_EINVAL, "ray input pointer is NULL");
+    return 0;
+}
+
+int voxg_ray_batch(const voxg_tree *t, size_t n, const float *ox,
+                   const float *oy, const float *oz, const float *dx,
+                   const float *dy, const float *dz, uint32_t *voxel_out,
+                   uint32_t *leaf_out, float *hx, float *hy, float *hz,
+                   hipStream_t stream) {
+    int rc = ray_common(t, n, ox, oy, oz, dx, dy, dz);
+    if (rc) return rc;
+    if ((hx || hy || hz) && !(hx && hy && hz))
+        return fail(VOXG_EINVAL, "hx, hy, hz must be all set or all NULL");
+    if (n == 0) return 0;
+    dev::SoaSrc src{ox, oy, oz, dx, dy, dz};
+    dev::SoaSink sink{voxel_out, leaf_out, hx, hy, hz};
+    return launch<dev::SoaSrc, dev::SoaSink, false>(t, src, sink, n, stream,
+                                                    nullptr);
+}
+
+int voxg_ray_batch_count(const voxg_tree *t, size_t n, const float *ox,
+                         const float *oy, const float *oz, const float *dx,
+                         const float *dy, const float *dz, voxg_counts *out,
+                         hipStream_t stream) {
+    int rc = ray_common(t, n, ox, oy, oz, dx, dy, dz);
+    if (rc) return rc;
+    if (!out) return fail(VOXG_EINVAL, "out is NULL");
+    std::memset(out, 0, sizeof(*out));
+    if (n == 0) return 0;
+    dev::SoaSrc src{ox, oy, oz, dx, dy, dz};
+    dev::SoaSink sink{nullptr, nullptr, nullptr, nullptr, nullptr};
+    return launch<dev::SoaSrc, dev::SoaSink, true>(t, src, sink, n, stream,
+                                                   out);
+}
+
+static int ball_launch(const voxg_tree *t, size_t n, const float *cx,
+                       const float *cy, const float *cz, const float *r,
+                       uint8_t *out, voxg_counts *counts, hipStream_t stream) {
+    int rc = check_tree(t);
+    if (rc) return rc;
+    if (n >= 0x80000000ull) return fail(VOXG_ERANGE, "n must be < 2^31");
+    if (n && (!cx || !cy || !cz || !r))
+        return fail(VOXG_EINVAL, "ball input pointer is NULL");
+    if (n && !out && !counts) return fail(VOXG_EINVAL, "out is NULL");
+    if (n == 0) return 0;
+    Workspace *ws = nullptr;
+    rc = get_ws(stream, 16, &ws);
+    if (rc) return rc;
+    int block = 256;
+    int grid = (int)((n + block - 1) / block);
+    if (counts) {
+        HIPCHK(hipMemsetAsync(&ws->ctrl->visits, 0,
+                              3 * sizeof(unsigned long long), stream));
+        hipLaunchKernelGGL(dev::ball_kernel<true>, dim3(grid), dim3(block), 0,
+                           stream, dev_tree(t), (uint32_t)n, cx, cy, cz, r,
+                           out, ws->ctrl);
+        HIPCHK(hipGetLastError());
+        dev::Ctrl h;
+        HIPCHK(hipMemcpyAsync(&h, ws->ctrl, sizeof(h), hipMemcpyDeviceToHost,
+                              stream));
+        HIPCHK(hipStreamSynchronize(stream));
+        counts->node_visits = h.visits;
+        counts->voxel_tests = h.tests;
+        counts->rays = h.rays;
+        return 0;
+    }
+    hipLaunchKernelGGL(dev::ball_kernel<false>, dim3(grid), dim3(block), 0,
+                       stream, dev_tree(t), (uint32_t)n, cx, cy, cz, r, out,
+                       ws->ctrl);
+    HIPCHK(hipGetLastError());
+    return 0;
+}
+
+int voxg_ball_batch(const voxg_tree *t, size_t n, const float *cx,
+                    const float *cy, const float *cz, const float *r,
+                    uint8_t *out, hipStream_t stream) {
+    if (n && !out) return fail(VOXG_EINVAL, "out is NULL");
+    return ball_launch(t, n, cx, cy, cz, r, out, nullptr, stream);
+}
+
+int voxg_ball_batch_count(const voxg_tree *t, size_t n, const float *cx,
+                          const float *cy, const float *cz, const float *r,
+                          voxg_counts *out, hipStream_t stream) {
+    if (!out) return fail(VOXG_EINVAL, "out is NULL");
+    std::memset(out, 0, sizeof(*out));
+    return ball_launch(t, n, cx, cy, cz, r, nullptr, out, stream);
+}
+
+static int render_common(const voxg_tree *t, size_t n_frames,
+                         const voxg_camera *cams, int W, int H,
+                         const voxg_render_params *params, uint32_t *squares,
+                         size_t *nrays) {
+    int rc = check_tree(t);
+    if (rc) return rc;
+    if (W <= 0 || H <= 0 || (W & 0xf) || (H & 0x3))
+        return fail(VOXG_EINVAL,
+                    "W must be a positive multiple of 16 and H of 4");
+    if (n_frames && !cams) return fail(VOXG_EINVAL, "cams is NULL");
+    unsigned q = params ? params->quality : VOXG_QUALITY_BEST;
+    // renderer.c:179-195
+    if ((q & 0xc) && (q & 3) != VOXG_QUALITY_ADAPTIVE)
+        return fail(VOXG_EINVAL, "ray merging needs the adaptive mode");
+    if ((q & 3) > 2 || (q & 0xc) > 8 || (q & ~0xfu))
+        return fail(VOXG_EINVAL, "reserved quality bits set");
+    if ((q & 3) != VOXG_QUALITY_BEST)
+        return fail(VOXG_EINVAL, "only VOXG_QUALITY_BEST is implemented");
+    if (squares && (!params || !params->texture))
+        return fail(VOXG_EINVAL, "colour output needs a texture");
+    size_t per = (size_t)W * (size_t)H;
+    if (n_frames * per >= 0x80000000ull)
+        return fail(VOXG_ERANGE, "n_frames * W * H must be < 2^31");
+    *nrays = n_frames * per;
+    return 0;
+}
+
+int voxg_render(const voxg_tree *t, size_t n_frames, const voxg_camera *cams,
+                int W, int H, const voxg_render_params *params,
+                uint32_t *squares, uint32_t *voxel_px, float *hit_px,
+                hipStream_t stream) {
+    size_t n = 0;
+    int rc = render_common(t, n_frames, cams, W, H, params, squares, &n);
+    if (rc) return rc;
+    if (n == 0) return 0;
+    static_assert(sizeof(voxg_camera) == sizeof(dev::CamDev), "camera layout");
+    dev::CamSrc src{(const dev::CamDev *)cams, (uint32_t)W,
+                    (uint32_t)W * (uint32_t)H};
+    dev::PixSink sink;
+    sink.squares = squares;
+    sink.voxel = voxel_px;
+    sink.hit = hit_px;
+    sink.plane = n;
+    sink.sh.tex = params ? params->texture : nullptr;
+    sink.sh.rshift = params ? params->rshift : 0;
+    sink.sh.gshift = params ? params->gshift : 0;
+    sink.sh.bshift = params ? params->bshift : 0;
+    sink.sh.amask = params ? params->amask : 0;
+    sink.sh.vx = t->host.voxel[0];
+    sink.sh.vy = t->host.voxel[1];
+    sink.sh.vz = t->host.voxel[2];
+    return launch<dev::CamSrc, dev::PixSink, false>(t, src, sink, n, stream,
+                                                    nullptr);
+}
+
+int voxg_render_count(const voxg_tree *t, size_t n_frames,
+                      const voxg_camera *cams, int W, int H,
+                      const voxg_render_params *params, voxg_counts *out,
+                      hipStream_t stream) {
+    if (!out) return fail(VOXG_EINVAL, "out is NULL");
+    std::memset(out, 0, sizeof(*out));
+    size_t n = 0;
+    int rc = render_common(t, n_frames, cams, W, H, params, nullptr, &n);
+    if (rc) return rc;
+    if (n == 0) return 0;
+    dev::CamSrc src{(const dev::CamDev *)cams, (uint32_t)W,
+                    (uint32_t)W * (uint32_t)H};
+    dev::PixSink sink;
+    std::memset(&sink, 0, sizeof(sink));
+    return launch<dev::CamSrc, dev::PixSink, true>(t, src, sink, n, stream,
+                                                   out);
+}
+
+}  // extern "C"

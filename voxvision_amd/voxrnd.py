@@ -1,0 +1,87 @@
+"""Python mirror of the vox_render per-pixel loop, backed by libvoxgpu.
+
+Reference: vox_render (src/voxrnd/renderer.c:200-319) over 4x4-pixel
+squares (renderer.h:14) with the simple camera's screen2world
+(src/voxrnd/simple-camera.c:21-31).  Cameras are data (see camera.py);
+colours use a caller-supplied 32^3 texture (the reference generates it
+with vn3d) and a 32-bit pixel format (SDL_MapRGB).
+"""
+import ctypes as C
+
+import numpy as np
+import torch
+
+from . import _lib
+from ._lib import check, QUALITY_BEST
+from .camera import camera_array
+from .voxtrees import _stream
+
+ARGB8888 = (16, 8, 0, 0xFF000000)
+
+
+def cameras_tensor(cams):
+    """List of camera dicts -> device tensor in the voxg_camera layout."""
+    return torch.from_numpy(camera_array(cams)).to("cuda")
+
+
+def _params(quality, texture, fmt):
+    p = _lib.RenderParams()
+    p.quality = quality
+    p.texture = texture.data_ptr() if texture is not None else None
+    p.rshift, p.gshift, p.bshift, p.amask = fmt
+    return p
+
+
+def render(tree, cams, W, H, quality=QUALITY_BEST, texture=None,
+           fmt=ARGB8888, want_colors=True, want_voxels=True, want_hits=True,
+           out=None, stream=None):
+    """Render len(cams) frames.  Returns dict of device tensors in the
+    square layout (index f*W*H + cs*16 + p): squares (uint32 colours, 0
+    where nothing was hit), voxel (int32, -1 = miss), hit (3, f*W*H)."""
+    L = _lib.load()
+    if not torch.is_tensor(cams):
+        cams = cameras_tensor(cams)
+    nf = cams.shape[0]
+    n = nf * W * H
+    if out is None:
+        out = {}
+        if want_colors:
+            out["squares"] = torch.zeros(n, dtype=torch.int32, device="cuda")
+        if want_voxels:
+            out["voxel"] = torch.empty(n, dtype=torch.int32, device="cuda")
+        if want_hits:
+            out["hit"] = torch.full((3, n), float("nan"), dtype=torch.float32,
+                                    device="cuda")
+    tex = None
+    if texture is not None:
+        tex = torch.as_tensor(texture, dtype=torch.uint8).to("cuda")
+    p = _params(quality, tex, fmt)
+    sq = out.get("squares")
+    vx = out.get("voxel")
+    hv = out.get("hit")
+    check(L.voxg_render(tree.handle, nf, cams.data_ptr(), W, H, C.byref(p),
+                        sq.data_ptr() if sq is not None else None,
+                        vx.data_ptr() if vx is not None else None,
+                        hv.data_ptr() if hv is not None else None,
+                        _stream(stream)), "voxg_render")
+    return out
+
+
+def render_counts(tree, cams, W, H, quality=QUALITY_BEST, stream=None):
+    L = _lib.load()
+    if not torch.is_tensor(cams):
+        cams = cameras_tensor(cams)
+    p = _params(quality, None, ARGB8888)
+    c = _lib.Counts()
+    check(L.voxg_render_count(tree.handle, cams.shape[0], cams.data_ptr(), W,
+                              H, C.byref(p), C.byref(c), _stream(stream)),
+          "voxg_render_count")
+    return dict(node_visits=c.node_visits, voxel_tests=c.voxel_tests,
+                rays=c.rays)
+
+
+def squares_to_image(squares, W, H):
+    """Host helper: square layout -> (H, W) image (copy_squares,
+    src/voxrnd/copy-helper.c:5-40)."""
+    s = np.asarray(squares).reshape(H // 4, W // 4, 4, 4)
+    return s.transpose(0, 2, 1, 3).reshape(H, W)

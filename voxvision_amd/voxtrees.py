@@ -1,0 +1,232 @@
+"""Python mirror of the voxtrees interface, backed by libvoxgpu.
+
+Reference interface (src/voxtrees/tree.h, search.h):
+  vox_make_tree(set, n)                       -> make_tree(dots, voxel)
+  vox_destroy_tree(tree)                      -> Tree.release()
+  vox_voxels_in_tree / vox_bounding_box       -> Tree.info
+  vox_ray_tree_intersection(tree, o, d, res)  -> ray_tree_intersection(tree, o, d)
+  vox_tree_ball_collidep(tree, c, r)          -> tree_ball_collidep(tree, c, r)
+The searches are batched: they take (n, 3) device tensors and return
+device tensors; a ray that misses gets voxel id VOXG_MISS and an untouched
+(NaN-initialised) hit point, as res is untouched in the reference.
+"""
+import ctypes as C
+
+import numpy as np
+import torch
+
+from . import _lib
+from ._lib import VOXG_MISS, check
+
+
+def _stream(stream=None):
+    s = stream if stream is not None else torch.cuda.current_stream()
+    return C.c_void_p(s.cuda_stream)
+
+
+def _dev_f32(x, n=None, cols=None):
+    t = torch.as_tensor(x)
+    if t.device.type != "cuda":
+        t = t.to("cuda")
+    t = t.to(torch.float32).contiguous()
+    return t
+
+
+class Tree:
+    """A flattened tree resident in device memory (immutable)."""
+
+    def __init__(self, handle, voxel):
+        self._h = handle
+        self.voxel = tuple(float(v) for v in voxel)
+        info = _lib.TreeInfo()
+        check(_lib.load().voxg_tree_get_info(self._h, C.byref(info)),
+              "voxg_tree_get_info")
+        self.info = {k: (list(getattr(info, k)) if k in
+                         ("bbox_min", "bbox_max", "voxel")
+                         else getattr(info, k)) for k, _ in info._fields_}
+
+    @property
+    def handle(self):
+        if self._h is None:
+            raise ValueError("tree was released")
+        return self._h
+
+    @classmethod
+    def build(cls, dots, voxel=(1.0, 1.0, 1.0)):
+        """vox_make_tree over an (n, 3) or (n, 4) float array (host)."""
+        L = _lib.load()
+        d = np.ascontiguousarray(np.asarray(dots, dtype=np.float32))
+        if d.ndim != 2 or d.shape[1] not in (3, 4):
+            raise ValueError("dots must be (n, 3) or (n, 4)")
+        v = (C.c_float * 3)(*voxel)
+        h = C.c_void_p()
+        check(L.voxg_tree_build(d.ctypes.data if len(d) else None, len(d),
+                                d.shape[1], v, C.byref(h)),
+              "voxg_tree_build")
+        return cls(h, voxel)
+
+    @classmethod
+    def upload(cls, root_ptr, voxel=(1.0, 1.0, 1.0)):
+        """Flatten a tree in the reference's struct vox_node layout."""
+        L = _lib.load()
+        v = (C.c_float * 3)(*voxel)
+        h = C.c_void_p()
+        check(L.voxg_tree_upload(root_ptr, v, C.byref(h)), "voxg_tree_upload")
+        return cls(h, voxel)
+
+    def release(self):
+        if self._h is not None:
+            _lib.load().voxg_tree_release(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.release()
+        except Exception:
+            pass
+
+    def hash(self):
+        return int(_lib.load().voxg_tree_hash(self.handle))
+
+    def voxel_coords(self, ids):
+        ids = np.ascontiguousarray(np.asarray(ids, dtype=np.uint32))
+        out = np.zeros((len(ids), 3), np.float32)
+        check(_lib.load().voxg_tree_voxel_coords(
+            self.handle, len(ids), ids.ctypes.data, out.ctypes.data),
+            "voxg_tree_voxel_coords")
+        return out
+
+    def node_ptr(self, idx):
+        return _lib.load().voxg_tree_node_ptr(self.handle, int(idx))
+
+
+def _info_dict(info):
+    return {k: (list(getattr(info, k)) if k in ("bbox_min", "bbox_max",
+                                                "voxel")
+                else getattr(info, k)) for k, _ in info._fields_}
+
+
+def plan(dots, voxel=(1.0, 1.0, 1.0)):
+    """Host-only build: (info dict, structure hash); no device needed."""
+    L = _lib.load()
+    d = np.ascontiguousarray(np.asarray(dots, dtype=np.float32))
+    v = (C.c_float * 3)(*voxel)
+    info = _lib.TreeInfo()
+    h = C.c_uint64()
+    check(L.voxg_tree_plan(d.ctypes.data if len(d) else None, len(d),
+                           d.shape[1] if d.ndim == 2 else 3, v,
+                           C.byref(info), C.byref(h)), "voxg_tree_plan")
+    return _info_dict(info), int(h.value)
+
+
+def plan_reference(root_ptr, voxel=(1.0, 1.0, 1.0)):
+    """Host-only flattening of a reference-layout tree."""
+    L = _lib.load()
+    v = (C.c_float * 3)(*voxel)
+    info = _lib.TreeInfo()
+    h = C.c_uint64()
+    check(L.voxg_tree_plan_reference(root_ptr, v, C.byref(info),
+                                     C.byref(h)), "voxg_tree_plan_reference")
+    return _info_dict(info), int(h.value)
+
+
+def make_tree(dots, voxel=(1.0, 1.0, 1.0)):
+    return Tree.build(dots, voxel)
+
+
+def ray_tree_intersection(tree, o, d, want_leaf=False, hit_init=None,
+                          stream=None):
+    """Batched vox_ray_tree_intersection.  o, d: (n, 3) arrays or tensors.
+    Returns (voxel_id uint32-as-int64 tensor, hit (3, n) float32 tensor
+    [, leaf node index tensor])."""
+    L = _lib.load()
+    o = _dev_f32(o).reshape(-1, 3)
+    d = _dev_f32(d).reshape(-1, 3)
+    n = o.shape[0]
+    oc = o.t().contiguous()
+    dc = d.t().contiguous()
+    vox = torch.empty(n, dtype=torch.int32, device="cuda")
+    leaf = torch.empty(n, dtype=torch.int32, device="cuda") if want_leaf \
+        else None
+    if hit_init is None:
+        hit = torch.full((3, n), float("nan"), dtype=torch.float32,
+                         device="cuda")
+    else:
+        hit = hit_init
+    check(L.voxg_ray_batch(tree.handle, n, oc[0].data_ptr(),
+                           oc[1].data_ptr(), oc[2].data_ptr(),
+                           dc[0].data_ptr(), dc[1].data_ptr(),
+                           dc[2].data_ptr(), vox.data_ptr(),
+                           leaf.data_ptr() if leaf is not None else None,
+                           hit[0].data_ptr(), hit[1].data_ptr(),
+                           hit[2].data_ptr(), _stream(stream)),
+          "voxg_ray_batch")
+    vox = vox.to(torch.int64) & 0xFFFFFFFF
+    if want_leaf:
+        return vox, hit, leaf.to(torch.int64) & 0xFFFFFFFF
+    return vox, hit
+
+
+def ray_counts(tree, o, d, stream=None):
+    """Node visits and voxel tests of the traversal of these rays."""
+    L = _lib.load()
+    o = _dev_f32(o).reshape(-1, 3).t().contiguous()
+    d = _dev_f32(d).reshape(-1, 3).t().contiguous()
+    c = _lib.Counts()
+    check(L.voxg_ray_batch_count(tree.handle, o.shape[1], o[0].data_ptr(),
+                                 o[1].data_ptr(), o[2].data_ptr(),
+                                 d[0].data_ptr(), d[1].data_ptr(),
+                                 d[2].data_ptr(), C.byref(c),
+                                 _stream(stream)), "voxg_ray_batch_count")
+    return dict(node_visits=c.node_visits, voxel_tests=c.voxel_tests,
+                rays=c.rays)
+
+
+def tree_ball_collidep(tree, c, r, stream=None):
+    """Batched vox_tree_ball_collidep: (n,3) centres, (n,) radii -> uint8."""
+    L = _lib.load()
+    c = _dev_f32(c).reshape(-1, 3).t().contiguous()
+    r = _dev_f32(r).reshape(-1).contiguous()
+    n = c.shape[1]
+    out = torch.empty(n, dtype=torch.uint8, device="cuda")
+    check(L.voxg_ball_batch(tree.handle, n, c[0].data_ptr(), c[1].data_ptr(),
+                            c[2].data_ptr(), r.data_ptr(), out.data_ptr(),
+                            _stream(stream)), "voxg_ball_batch")
+    return out
+
+
+def ball_counts(tree, c, r, stream=None):
+    L = _lib.load()
+    c = _dev_f32(c).reshape(-1, 3).t().contiguous()
+    r = _dev_f32(r).reshape(-1).contiguous()
+    cnt = _lib.Counts()
+    check(L.voxg_ball_batch_count(tree.handle, c.shape[1], c[0].data_ptr(),
+                                  c[1].data_ptr(), c[2].data_ptr(),
+                                  r.data_ptr(), C.byref(cnt),
+                                  _stream(stream)), "voxg_ball_batch_count")
+    return dict(node_visits=cnt.node_visits, voxel_tests=cnt.voxel_tests,
+                rays=cnt.rays)
+
+
+def set_launch_config(stack_lds=None, block=None, waves_per_cu=None,
+                      refill=None):
+    L = _lib.load()
+    cfg = _lib.LaunchConfig()
+    check(L.voxg_get_launch_config(C.byref(cfg)))
+    if stack_lds is not None:
+        cfg.stack_lds = stack_lds
+    if block is not None:
+        cfg.block = block
+    if waves_per_cu is not None:
+        cfg.waves_per_cu = waves_per_cu
+    if refill is not None:
+        cfg.refill = refill
+    check(L.voxg_set_launch_config(C.byref(cfg)), "voxg_set_launch_config")
+    return get_launch_config()
+
+
+def get_launch_config():
+    cfg = _lib.LaunchConfig()
+    check(_lib.load().voxg_get_launch_config(C.byref(cfg)))
+    return dict(stack_lds=cfg.stack_lds, block=cfg.block,
+                waves_per_cu=cfg.waves_per_cu, refill=cfg.refill)
