@@ -1,0 +1,227 @@
+"""Benchmark of the voxvision ray-casting path on one MI355X.
+
+Step = one pass of the hot path over one batch: 64 orbit poses of the
+256^3 CT-like scene (1.34 M voxels) at 1920x1080, i.e. 132,710,400 primary
+rays traced with the vox_render per-pixel loop in BEST mode, writing the
+hit voxel id and hit point of every pixel.  One kernel launch per step,
+inputs (tree, cameras) resident in HBM before the timed region.
+
+    python bench.py [--steps K] [--warmup W] [--gpus 1] [--no-cpu] [--quick]
+
+Prints one JSON line.  `roofline` follows the algorithmic-bytes model
+(40 B per ray + 32 B per node visit + 12 B per voxel test, visits and
+tests counted by the counting variant of the same kernel); `cpu_baseline`
+times the CPU oracle (a C restatement of the reference, one task per 4x4
+block on a thread pool) on a bounded sample of the same frames.
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+METRIC = ("Mrays/s coherent+incoherent on 256³ scene @1080p; "
+          "batched vs plain; p99 latency")
+HBM_PEAK_GBS = 8000.0
+BYTES_PER_RAY, BYTES_PER_VISIT, BYTES_PER_TEST = 40, 32, 12
+
+
+def log(*a):
+    print(*a, file=sys.stderr, flush=True)
+
+
+def cpu_baseline(dots, cams, W, H, budget_s):
+    """Oracle render (BEST) on all usable host cores over consecutive poses
+    until `budget_s` seconds of work are done (at least one frame)."""
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import oracle as O
+    O.build()
+    T = O.Tree(dots)
+    cores = O.cpu_count()
+    frames, rays = 0, 0
+    t0 = time.perf_counter()
+    while True:
+        T.render(cams[frames % len(cams)], W, H, O.QUALITY_BEST,
+                 nthreads=cores)
+        frames += 1
+        rays += W * H
+        if time.perf_counter() - t0 >= budget_s:
+            break
+    dt = time.perf_counter() - t0
+    t1 = time.perf_counter()
+    T.render(cams[0], W, H, O.QUALITY_BEST, nthreads=1)
+    dt1 = time.perf_counter() - t1
+    T.close()
+    return {"value": rays / dt / 1e6, "unit": "Mrays/s", "cores": cores,
+            "kind": "port",
+            "sample": "%d of the 64 poses at %dx%d (%.1f s, %d threads, "
+                      "oracle vox_render loop, BEST); 1 core: %.2f Mrays/s"
+                      % (frames, W, H, dt, cores, W * H / dt1 / 1e6)}
+
+
+def load_traffic(name):
+    p = os.path.join(ROOT, "profiles", name)
+    if not os.path.exists(p):
+        return None
+    try:
+        return json.load(open(p))
+    except Exception:
+        return None
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=10)
+    ap.add_argument("--warmup", type=int, default=3)
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--cpu-seconds", type=float, default=12.0)
+    ap.add_argument("--quick", action="store_true",
+                    help="skip the incoherent and ball extras")
+    ap.add_argument("--copy-calib-mib", type=int, default=0,
+                    help="run a plain device copy of this size first (the "
+                         "reference kernel for HBM counter calibration)")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print("bench.py measures one GPU: the ray-casting path shards rays "
+              "across GPUs with no communication, so only --gpus 1 is run "
+              "here", file=sys.stderr)
+        sys.exit(2)
+
+    import numpy as np
+    import torch
+    from voxvision_amd import _lib, scenes, voxrnd, voxtrees as VT
+
+    L = _lib.load()
+    _lib.check(L.voxg_init(0), "voxg_init")
+    torch.cuda.set_device(0)
+    stream = torch.cuda.current_stream()
+
+    log("[bench] generating the 256^3 scene and 64 poses")
+    c2 = scenes.config_c2()
+    dots, W, H = c2["dots"], c2["W"], c2["H"]
+    G = VT.make_tree(dots)
+    cams = voxrnd.cameras_tensor(c2["cameras"])
+    nf = cams.shape[0]
+    n = nf * W * H
+    out = {"voxel": torch.empty(n, dtype=torch.int32, device="cuda"),
+           "hit": torch.empty((3, n), dtype=torch.float32, device="cuda")}
+    log("[bench] tree: %s" % json.dumps(G.info))
+
+    def step():
+        voxrnd.render(G, cams, W, H, out=out)
+
+    if args.copy_calib_mib:
+        nb = args.copy_calib_mib << 20
+        src = torch.ones(nb // 4, dtype=torch.float32, device="cuda")
+        dst = torch.empty_like(src)
+        dst.copy_(src)
+        torch.cuda.synchronize()
+        log("[bench] copy calibration: %d bytes read, %d written" % (nb, nb))
+        del src, dst
+
+    for _ in range(args.warmup):
+        step()
+    torch.cuda.synchronize()
+    ev0 = torch.cuda.Event(enable_timing=True)
+    ev1 = torch.cuda.Event(enable_timing=True)
+    t0 = time.perf_counter()
+    ev0.record(stream)
+    for _ in range(args.steps):
+        step()
+    ev1.record(stream)
+    torch.cuda.synchronize()
+    wall = time.perf_counter() - t0
+    ms_total = ev0.elapsed_time(ev1)
+    ms = ms_total / args.steps
+    value = n / (ms / 1e3) / 1e6
+    log("[bench] %d steps: %.3f ms/step (events), wall %.3f s"
+        % (args.steps, ms, wall))
+
+    hits = int((out["voxel"] != -1).sum().item())
+    cnt = voxrnd.render_counts(G, cams, W, H)
+    alg_bytes = (BYTES_PER_RAY * n + BYTES_PER_VISIT * cnt["node_visits"] +
+                 BYTES_PER_TEST * cnt["voxel_tests"])
+    achieved = alg_bytes / (ms / 1e3) / 1e9
+    tr = load_traffic("traffic_ct256_render.json")
+    roofline = {"bound": "hbm", "achieved": round(achieved, 2),
+                "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                "frac": round(achieved / HBM_PEAK_GBS, 5),
+                "traffic": tr.get("hbm_bytes_per_launch") if tr else None,
+                "algorithmic_bytes_per_launch": alg_bytes,
+                "node_visits_per_ray": round(cnt["node_visits"] / n, 4),
+                "voxel_tests_per_ray": round(cnt["voxel_tests"] / n, 4)}
+
+    extra = {"hit_rate": round(hits / n, 4)}
+    if not args.quick:
+        bmin, bmax = c2["bbox"]
+        o, d = scenes.random_rays(1 << 24, bmin, bmax, seed=11)
+        so = [torch.from_numpy(np.ascontiguousarray(o[:, k])).cuda()
+              for k in range(3)]
+        sd = [torch.from_numpy(np.ascontiguousarray(d[:, k])).cuda()
+              for k in range(3)]
+        vo = torch.empty(1 << 24, dtype=torch.int32, device="cuda")
+        ho = [torch.empty(1 << 24, device="cuda") for _ in range(3)]
+        import ctypes as C
+
+        def inc():
+            _lib.check(L.voxg_ray_batch(
+                G.handle, 1 << 24, *[t.data_ptr() for t in so + sd],
+                vo.data_ptr(), None, *[t.data_ptr() for t in ho],
+                C.c_void_p(stream.cuda_stream)), "voxg_ray_batch")
+        inc()
+        torch.cuda.synchronize()
+        ev0.record(stream)
+        for _ in range(5):
+            inc()
+        ev1.record(stream)
+        torch.cuda.synchronize()
+        ims = ev0.elapsed_time(ev1) / 5
+        extra["incoherent_16M_mrays_s"] = round((1 << 24) / ims / 1e3, 1)
+        cb, rb = scenes.random_balls(1 << 22, bmin, bmax, seed=13)
+        cbt = [torch.from_numpy(np.ascontiguousarray(cb[:, k])).cuda()
+               for k in range(3)]
+        rbt = torch.from_numpy(rb).cuda()
+        bo = torch.empty(1 << 22, dtype=torch.uint8, device="cuda")
+
+        def balls():
+            _lib.check(L.voxg_ball_batch(
+                G.handle, 1 << 22, *[t.data_ptr() for t in cbt],
+                rbt.data_ptr(), bo.data_ptr(),
+                C.c_void_p(stream.cuda_stream)), "voxg_ball_batch")
+        balls()
+        torch.cuda.synchronize()
+        ev0.record(stream)
+        for _ in range(5):
+            balls()
+        ev1.record(stream)
+        torch.cuda.synchronize()
+        bms = ev0.elapsed_time(ev1) / 5
+        extra["balls_4M_mqueries_s"] = round((1 << 22) / bms / 1e3, 1)
+
+    cpu = None
+    if not args.no_cpu:
+        log("[bench] CPU baseline (%.0f s budget)" % args.cpu_seconds)
+        cpu = cpu_baseline(dots, c2["cameras"], W, H, args.cpu_seconds)
+
+    line = {
+        "metric": METRIC, "value": round(value, 1), "unit": "Mrays/s",
+        "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
+        "ms_per_step": round(ms, 4), "higher_is_better": True,
+        "scaling": "weak", "vs_baseline": None, "dtype": "f32",
+        "data": "synthetic",
+        "config": {"workload": "ct256_orbit64_1080p_best",
+                   "scene": "256^3 4-octave value-noise CT shell+interior, "
+                            "8%% occupancy, %d voxels" % len(dots),
+                   "frames": nf, "W": W, "H": H, "rays_per_step": n,
+                   "outputs": "voxel id + hit point per pixel"},
+        "roofline": roofline, "cpu_baseline": cpu, "extra": extra,
+    }
+    print(json.dumps(line), flush=True)
+
+
+if __name__ == "__main__":
+    main()

@@ -632,7 +632,7 @@ struct voxg_tree {
 namespace {
 
 std::mutex g_mu;
-voxg_launch_config g_cfg = {8, 256, 16, 16};
+voxg_launch_config g_cfg = {4, 256, 32, 32};
 thread_local int g_device = 0;
 
 struct Workspace {
