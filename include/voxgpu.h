@@ -177,6 +177,7 @@ typedef struct {
 } voxg_render_params;
 
 /* The vox_render per-pixel loop for n_frames camera poses at W x H
+ * in any quality mode (BEST, FAST, ADAPTIVE with or without ray merging).
  * (W % 16 == 0, H % 4 == 0, renderer.c:97-100).  cams: device array of
  * n_frames descriptors.  Pixel (x, y) of frame f lives at index
  * f*W*H + cs*16 + p with block cs = (y/4)*(W/4) + x/4 and p = (y%4)*4 + x%4
@@ -194,7 +195,9 @@ int voxg_render(const voxg_tree *tree, size_t n_frames,
 typedef struct {
     uint64_t node_visits;
     uint64_t voxel_tests;
-    uint64_t rays;
+    uint64_t rays;            /* searches started at the root */
+    uint64_t leaf_traversals; /* FAST-mode searches started at a leaf */
+    uint64_t fast_blocks, best_blocks, merge_blocks; /* render modes */
 } voxg_counts;
 int voxg_ray_batch_count(const voxg_tree *tree, size_t n, const float *ox,
                          const float *oy, const float *oz, const float *dx,
@@ -213,12 +216,15 @@ int voxg_render_count(const voxg_tree *tree, size_t n_frames,
  * lane kept in LDS (power of two 2..32; deeper stacks spill to HBM and stay
  * exact); block: threads per workgroup (64..1024, multiple of 64);
  * waves_per_cu: persistent waves launched per CU; refill: minimum idle
- * lanes before a wave fetches new rays (1..64). */
+ * lanes before a wave fetches new work (1..64); chunk: consecutive rays a
+ * lane traces before it needs new work in voxg_ray_batch (1..4096; a
+ * render lane always takes one 4x4 block). */
 typedef struct {
     int stack_lds;
     int block;
     int waves_per_cu;
     int refill;
+    int chunk;
 } voxg_launch_config;
 int voxg_set_launch_config(const voxg_launch_config *cfg);
 int voxg_get_launch_config(voxg_launch_config *cfg);

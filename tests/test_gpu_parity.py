@@ -263,3 +263,53 @@ def test_invalid_arguments_are_rejected(gpu):
     assert rc == -22  # W % 16 != 0
     assert L.voxg_ray_batch(None, 4, *([x.data_ptr()] * 6), None, None, None,
                             None, None, None) == -22
+
+
+@pytest.mark.parametrize("quality", [1, 2, 2 | 4, 2 | 8])
+def test_render_fast_adaptive_merge_match_oracle(gpu, oracle, quality):
+    """FAST, ADAPTIVE and both ray-merge modes (renderer.c:200-319): the
+    per-block decisions, the leaf-reuse chain, merged pixels and colours
+    all match the oracle; so do the mode and traversal counters."""
+    from voxvision_amd import voxrnd, voxtrees as VT
+    c1 = _c1()
+    T = oracle.Tree(c1["dots"])
+    G = VT.make_tree(c1["dots"])
+    tex = np.random.default_rng(4).integers(0, 256, 32 ** 3).astype(np.uint8)
+    W, H = c1["W"], c1["H"]
+    sq_o, vox_o, hit_o, st = T.render(c1["camera"], W, H, quality,
+                                      texture=tex)
+    out = voxrnd.render(G, [c1["camera"]], W, H, quality=quality,
+                        texture=tex)
+    torch.cuda.synchronize()
+    _assert_rays_equal(out["voxel"], out["hit"], vox_o, hit_o)
+    assert np.array_equal(_np_u32(out["squares"]), sq_o)
+    cnt = voxrnd.render_counts(G, [c1["camera"]], W, H, quality=quality)
+    assert cnt["fast_blocks"] == st["fast_blocks"]
+    assert cnt["best_blocks"] == st["best_blocks"]
+    assert cnt["merge_blocks"] == st["merge_blocks"]
+    assert cnt["rays"] == st["root_traversals"]
+    assert cnt["leaf_traversals"] == st["leaf_traversals"]
+    assert cnt["node_visits"] == st["node_visits"]
+    assert cnt["voxel_tests"] == st["voxel_tests"]
+
+
+def test_render_adaptive_ct_frames(gpu, oracle):
+    """ADAPTIVE with accurate ray merging on two 1080p poses of the 256^3
+    scene (the reference's default quality is ADAPTIVE, renderer.c:78)."""
+    from voxvision_amd import scenes, voxrnd, voxtrees as VT
+    c2 = scenes.config_c2(frames=64)
+    T = oracle.Tree(c2["dots"])
+    G = VT.make_tree(c2["dots"])
+    W, H = c2["W"], c2["H"]
+    cams = [c2["cameras"][k] for k in (5, 50)]
+    q = 2 | 8
+    out = voxrnd.render(G, cams, W, H, quality=q, want_colors=False)
+    torch.cuda.synchronize()
+    vg = _np_u32(out["voxel"]).reshape(2, -1)
+    hg = out["hit"].cpu().numpy().reshape(3, 2, -1)
+    for i, cam in enumerate(cams):
+        _, vox_o, hit_o, _ = T.render(cam, W, H, q)
+        assert np.array_equal(vg[i], vox_o)
+        m = vox_o != MISS
+        assert np.array_equal(hg[:, i, m].view(np.uint32),
+                              hit_o[:, m].view(np.uint32))

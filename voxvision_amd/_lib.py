@@ -44,7 +44,9 @@ class TreeInfo(C.Structure):
 
 class Counts(C.Structure):
     _fields_ = [("node_visits", C.c_uint64), ("voxel_tests", C.c_uint64),
-                ("rays", C.c_uint64)]
+                ("rays", C.c_uint64), ("leaf_traversals", C.c_uint64),
+                ("fast_blocks", C.c_uint64), ("best_blocks", C.c_uint64),
+                ("merge_blocks", C.c_uint64)]
 
 
 class Camera(C.Structure):
@@ -61,7 +63,8 @@ class RenderParams(C.Structure):
 
 class LaunchConfig(C.Structure):
     _fields_ = [("stack_lds", C.c_int), ("block", C.c_int),
-                ("waves_per_cu", C.c_int), ("refill", C.c_int)]
+                ("waves_per_cu", C.c_int), ("refill", C.c_int),
+                ("chunk", C.c_int)]
 
 
 _lib = None

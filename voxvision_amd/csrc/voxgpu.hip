@@ -246,6 +246,22 @@ struct CamDev {
     float px, py, pz, qw, qi, qj, qk, mul, xsub, ysub, fov;
 };
 
+// simple-camera.c:21-31 + vect-ops-sse.c:16-26,58-64
+__device__ __forceinline__ f3 screen2world(const CamDev &c, int sx, int sy) {
+    float vx = __builtin_fmaf(c.mul, (float)sx, -c.xsub);
+    float vy = 1.0f;
+    float vz = __builtin_fmaf(-c.mul, (float)sy, c.ysub);
+    float t1 = (c.qj * vz - vy * c.qk) * 2.0f;
+    float t2 = (c.qk * vx - vz * c.qi) * 2.0f;
+    float t3 = (c.qi * vy - vx * c.qj) * 2.0f;
+    float c1 = c.qj * t3 - t2 * c.qk;
+    float c2 = c.qk * t1 - t3 * c.qi;
+    float c3 = c.qi * t2 - t1 * c.qj;
+    return f3{__builtin_fmaf(c.qw, t1, vx) + c1,
+              __builtin_fmaf(c.qw, t2, vy) + c2,
+              __builtin_fmaf(c.qw, t3, vz) + c3};
+}
+
 // Camera rays in the square layout: ray r of a frame is pixel index
 // cs*16 + p, i.e. 16x4-pixel tiles of four 4x4 blocks per 64 rays.
 struct CamSrc {
@@ -261,20 +277,7 @@ struct CamSrc {
         int sy = (int)(ty * 4u + (p >> 2));
         const CamDev &c = cams[f];
         o = f3{c.px, c.py, c.pz};
-        // simple-camera.c:29, then vect-ops-sse.c:16-26 with the vector in
-        // quaternion lanes (0, vx, vy, vz)
-        float vx = __builtin_fmaf(c.mul, (float)sx, -c.xsub);
-        float vy = 1.0f;
-        float vz = __builtin_fmaf(-c.mul, (float)sy, c.ysub);
-        float t1 = (c.qj * vz - vy * c.qk) * 2.0f;
-        float t2 = (c.qk * vx - vz * c.qi) * 2.0f;
-        float t3 = (c.qi * vy - vx * c.qj) * 2.0f;
-        float c1 = c.qj * t3 - t2 * c.qk;
-        float c2 = c.qk * t1 - t3 * c.qi;
-        float c3 = c.qi * t2 - t1 * c.qj;
-        d = f3{__builtin_fmaf(c.qw, t1, vx) + c1,
-               __builtin_fmaf(c.qw, t2, vy) + c2,
-               __builtin_fmaf(c.qw, t3, vz) + c3};
+        d = screen2world(c, sx, sy);
     }
 };
 
@@ -312,19 +315,248 @@ struct PixSink {
 };
 
 struct Ctrl {
-    uint32_t next;     // next ray to hand out
+    uint32_t next;     // next work unit to hand out
     uint32_t left;     // waves that have finished
     uint32_t overflow; // set when a stack bound was exceeded (never expected)
     uint32_t pad;
-    unsigned long long visits, tests, rays;
+    unsigned long long visits, tests, rays, leaf_trav;
+    unsigned long long fast_blocks, best_blocks, merge_blocks, pad2;
+};
+
+// ---------------------------------------------------------------------------
+// work units driven by the persistent traversal kernel
+//
+//   begin(unit, st, cur, org, d)  -> starts the unit's first traversal
+//   finish(st, hit, vid, node, e, cur, org, d) -> consumes a traversal's
+//       result; returns true when it started another traversal
+// ---------------------------------------------------------------------------
+
+// One ray per unit (vox_ray_tree_intersection, or BEST-mode pixels).
+template <class Src, class Sink>
+struct RayWork {
+    Src src;
+    Sink sink;
+    struct State {
+        uint32_t ray;
+    };
+    __device__ __forceinline__ void begin(uint32_t unit, State &st,
+                                          uint32_t &cur, f3 &org, f3 &d) const {
+        st.ray = unit;
+        src.get(unit, org, d);
+        cur = 0;
+    }
+    __device__ __forceinline__ bool finish(State &st, bool hit, uint32_t vid,
+                                           uint32_t node, f3 e, uint32_t &,
+                                           f3 &, f3 &) const {
+        sink.put(st.ray, hit, vid, node, e);
+        return false;
+    }
+    __device__ __forceinline__ void flush(const State &, Ctrl *) const {}
+};
+
+// A run of `chunk` consecutive rays per unit, traced one after the other
+// by the same lane: a lane whose ray ends moves on at once instead of
+// waiting for a wave-wide refill.
+template <class Src, class Sink>
+struct ChunkWork {
+    Src src;
+    Sink sink;
+    uint32_t n, chunk;
+    struct State {
+        uint32_t ray, end;
+    };
+    __device__ __forceinline__ void begin(uint32_t unit, State &st,
+                                          uint32_t &cur, f3 &org, f3 &d) const {
+        st.ray = unit * chunk;
+        st.end = min(st.ray + chunk, n);
+        src.get(st.ray, org, d);
+        cur = 0;
+    }
+    __device__ __forceinline__ bool finish(State &st, bool hit, uint32_t vid,
+                                           uint32_t node, f3 e, uint32_t &cur,
+                                           f3 &org, f3 &d) const {
+        sink.put(st.ray, hit, vid, node, e);
+        if (++st.ray >= st.end) return false;
+        src.get(st.ray, org, d);
+        cur = 0;
+        return true;
+    }
+    __device__ __forceinline__ void flush(const State &, Ctrl *) const {}
+};
+
+// renderer.c:23-25
+__device__ __forceinline__ int render_order(int i) {
+    // 0, 1, 4, 5, 2, 3, 6, 7, 8, 9, 12, 13, 10, 11, 14, 15 packed by nibble
+    const unsigned long long tab = 0xfebadc9876325410ull;
+    return (int)((tab >> (4 * i)) & 0xfull);
+}
+
+__device__ __forceinline__ float sqr_metric(f3 a, f3 b) {
+    float x = a.x - b.x, y = a.y - b.y, z = a.z - b.z;
+    x = x * x;
+    y = y * y;
+    z = z * z;
+    return (x + y) + z;
+}
+
+// One 4x4 block per unit in FAST or ADAPTIVE mode (renderer.c:214-318):
+// the leaf found for one pixel is tried first for the next one, so the
+// pixels of a block are traced in order by one lane.
+struct BlockWork {
+    const CamDev *cams;
+    uint32_t W, ws, bpf;     // width, blocks per row, blocks per frame
+    int rnd_mode, merge_mode;
+    PixSink sink;
+    enum { CORNER1 = 0, CORNER2 = 1, LOOP = 2 };
+    struct State {
+        uint32_t blk, base;  // global block, output index of its pixel 0
+        int i, iend, stage, mode, merge, leaf_try, p;
+        uint32_t leaf, vid_last;
+        f3 inter_last, c1_inter, c1_dir;
+        int xs, ys;
+        uint32_t fast, best, merged;
+    };
+
+    __device__ __forceinline__ const CamDev &cam(const State &st) const {
+        return cams[st.blk / bpf];
+    }
+    __device__ __forceinline__ void emit(const State &st, int p, bool hit,
+                                         uint32_t vid, f3 e) const {
+        sink.put(st.base + (uint32_t)p, hit, vid, 0, e);
+    }
+    __device__ __forceinline__ void trace_pixel(const State &st, int p,
+                                                uint32_t start, uint32_t &cur,
+                                                f3 &org, f3 &d) const {
+        const CamDev &c = cam(st);
+        d = screen2world(c, st.xs + (p & 3), st.ys + (p >> 2));
+        org = f3{c.px, c.py, c.pz};
+        cur = start;
+    }
+    // renderer.c:286-316: advance to the next pixel that needs a traversal,
+    // writing merged pixels on the way; false when the block is done.
+    __device__ __forceinline__ bool next(State &st, uint32_t &cur, f3 &org,
+                                         f3 &d) const {
+        while (st.i < st.iend) {
+            int p = render_order(st.i);
+            bool m = st.merge && (st.i & 1);
+            if (m) {
+                emit(st, p, st.leaf != kEnd, st.vid_last, st.inter_last);
+                st.i++;
+                continue;
+            }
+            st.p = p;
+            st.leaf_try = (st.mode == VOXG_QUALITY_FAST && st.leaf != kEnd);
+            trace_pixel(st, p, st.leaf_try ? st.leaf : 0u, cur, org, d);
+            return true;
+        }
+        if (st.mode == VOXG_QUALITY_FAST) st.fast++;
+        else st.best++;
+        if (st.merge) st.merged++;
+        return false;
+    }
+    __device__ __forceinline__ void begin(uint32_t unit, State &st,
+                                          uint32_t &cur, f3 &org, f3 &d) const {
+        uint32_t cs = unit % bpf;
+        st.blk = unit;
+        st.base = (unit / bpf) * bpf * 16u + cs * 16u;
+        st.xs = (int)(cs % ws) * 4;
+        st.ys = (int)(cs / ws) * 4;
+        st.mode = rnd_mode;
+        st.merge = 0;
+        st.leaf = kEnd;
+        st.vid_last = kEnd;
+        st.inter_last = f3{0, 0, 0};
+        st.i = 0;
+        st.iend = 16;
+        if (st.mode == VOXG_QUALITY_ADAPTIVE) {
+            // renderer.c:240-253
+            st.merge = (merge_mode == VOXG_QUALITY_RAY_MERGE);
+            st.i = 1;
+            st.stage = CORNER1;
+            st.leaf_try = 0;
+            st.p = 0;
+            trace_pixel(st, 0, 0u, cur, org, d);
+            st.c1_dir = d;
+            return;
+        }
+        st.stage = LOOP;
+        next(st, cur, org, d);  // a block always has a first pixel to trace
+    }
+    __device__ __forceinline__ bool finish(State &st, bool hit, uint32_t vid,
+                                           uint32_t node, f3 e, uint32_t &cur,
+                                           f3 &org, f3 &d) const {
+        if (st.stage == CORNER1) {
+            // renderer.c:252-259
+            st.mode = VOXG_QUALITY_FAST;
+            st.leaf = hit ? node : kEnd;
+            emit(st, 0, hit, vid, e);
+            if (hit) {
+                st.vid_last = vid;
+                st.inter_last = e;
+                st.c1_inter = e;
+                st.iend = 15;
+                st.stage = CORNER2;
+                st.p = 15;
+                trace_pixel(st, 15, 0u, cur, org, d);
+                return true;
+            }
+            st.stage = LOOP;
+            return next(st, cur, org, d);
+        }
+        if (st.stage == CORNER2) {
+            // renderer.c:260-279
+            emit(st, 15, hit, vid, e);
+            if (hit) {
+                const CamDev &c = cam(st);
+                f3 origin = f3{c.px, c.py, c.pz};
+                float d1 = sqr_metric(st.c1_inter, origin);
+                f3 dir1 = st.c1_dir;
+                float d2 = (dir1.x * dir1.x + dir1.y * dir1.y) + dir1.z * dir1.z;
+                float criteria = d1 / d2 * sqr_metric(dir1, d);
+                float dist = sqr_metric(st.c1_inter, e);
+                if ((double)(dist / criteria) > 1.69) st.mode = VOXG_QUALITY_BEST;
+                if (d1 > 22500.0f) {
+                    st.merge = (st.mode == VOXG_QUALITY_FAST &&
+                                merge_mode == VOXG_QUALITY_RAY_MERGE_ACCURATE)
+                                   ? 1
+                                   : st.merge;
+                } else {
+                    st.merge = 0;
+                }
+            }
+            st.stage = LOOP;
+            return next(st, cur, org, d);
+        }
+        // renderer.c:293-315
+        if (!hit && st.leaf_try) {
+            st.leaf_try = 0;
+            cur = 0;  // the leaf missed: search from the root
+            const CamDev &c = cam(st);
+            org = f3{c.px, c.py, c.pz};
+            return true;
+        }
+        st.leaf = hit ? node : kEnd;
+        if (hit) {
+            st.vid_last = vid;
+            st.inter_last = e;
+        }
+        emit(st, st.p, hit, vid, e);
+        st.i++;
+        return next(st, cur, org, d);
+    }
+    __device__ __forceinline__ void flush(const State &st, Ctrl *ctrl) const {
+        atomicAdd(&ctrl->fast_blocks, (unsigned long long)st.fast);
+        atomicAdd(&ctrl->best_blocks, (unsigned long long)st.best);
+        atomicAdd(&ctrl->merge_blocks, (unsigned long long)st.merged);
+    }
 };
 
 // ---------------------------------------------------------------------------
 // persistent traversal kernel
 // ---------------------------------------------------------------------------
-template <int K, class Src, class Sink, bool COUNT>
+template <int K, class Work, bool COUNT>
 __global__ void __launch_bounds__(256)
-rti_kernel(Tree t, Src src, Sink sink, uint32_t n, Ctrl *ctrl,
+rti_kernel(Tree t, Work work, uint32_t n, Ctrl *ctrl,
            uint4 *__restrict__ spill, uint32_t spill_slots, uint32_t refill) {
     extern __shared__ uint4 ring[];  // [K][blockDim.x]
     const uint32_t tid = threadIdx.x;
@@ -335,17 +567,13 @@ rti_kernel(Tree t, Src src, Sink sink, uint32_t n, Ctrl *ctrl,
     const f3 V = f3{t.vx, t.vy, t.vz};
 
     bool active = false;
-    bool exhausted = (t.empty != 0);
-    uint32_t ray = 0, cur = 0;
-    f3 o{0, 0, 0}, d{0, 0, 0}, org{0, 0, 0};
+    bool exhausted = false;
+    uint32_t cur = 0;
+    f3 d{0, 0, 0}, org{0, 0, 0};
+    typename Work::State st;
+    std::memset(&st, 0, sizeof(st));
     int sp = 0, lo = 0;
-    unsigned long long nv = 0, nt = 0, nr = 0;
-
-    if (t.empty) {
-        // every query misses; hand out all rays as misses
-        for (uint32_t r = gtid; r < n; r += gthreads)
-            sink.put(r, false, kEnd, kEnd, o);
-    }
+    unsigned long long nv = 0, nt = 0, nr = 0, nl = 0;
 
     for (;;) {
         uint64_t idle = __ballot(!active);
@@ -361,13 +589,13 @@ rti_kernel(Tree t, Src src, Sink sink, uint32_t n, Ctrl *ctrl,
             if (!active) {
                 uint32_t my = base + lanes_below(idle);
                 if (my < n) {
-                    ray = my;
-                    src.get(ray, o, d);
-                    org = o;
-                    cur = 0;
+                    work.begin(my, st, cur, org, d);
                     sp = lo = 0;
                     active = true;
-                    if (COUNT) nr++;
+                    if (COUNT) {
+                        if (cur == 0) nr++;
+                        else nl++;
+                    }
                 }
             }
         }
@@ -378,12 +606,15 @@ rti_kernel(Tree t, Src src, Sink sink, uint32_t n, Ctrl *ctrl,
         if (!active) continue;
 
         // ---- process (cur, org) ----
-        Node nd = load_node(t.nodes, cur);
-        if (COUNT) nv++;
-        f3 e;
-        bool finished = false, hit = false, descend = false;
+        f3 e = org;
+        bool hit = false, descend = false;
         uint32_t vid = kEnd;
-        if (hit_box(nd.bmin, nd.bmax, org, d, e)) {
+        Node nd;
+        if (!t.empty) {
+            nd = load_node(t.nodes, cur);
+            if (COUNT) nv++;
+        }
+        if (!t.empty && hit_box(nd.bmin, nd.bmax, org, d, e)) {
             uint32_t type = nd.info & NT_TYPE_MASK;
             if (type == NT_DENSE) {
                 // search.c:31-37
@@ -508,13 +739,12 @@ rti_kernel(Tree t, Src src, Sink sink, uint32_t n, Ctrl *ctrl,
                 }
             }
         }
+        bool done = false;
         if (hit) {
-            sink.put(ray, true, vid, cur, e);
-            finished = true;
+            done = true;
         } else if (!descend) {
             if (sp == 0) {
-                sink.put(ray, false, kEnd, kEnd, e);
-                finished = true;
+                done = true;
             } else {
                 uint4 v;
                 if (sp == lo) {
@@ -530,21 +760,33 @@ rti_kernel(Tree t, Src src, Sink sink, uint32_t n, Ctrl *ctrl,
                 cur = v.x;
                 org = f3{__uint_as_float(v.y), __uint_as_float(v.z),
                          __uint_as_float(v.w)};
-                if (cur == kEnd) {  // only after a reported overflow
-                    sink.put(ray, false, kEnd, kEnd, e);
-                    finished = true;
-                }
+                if (cur == kEnd) done = true;  // only after a reported overflow
             }
         }
-        if (finished) active = false;
+        if (done) {
+            // the traversal ended: hand the result to the work unit
+            uint32_t node = cur;
+            sp = lo = 0;
+            if (work.finish(st, hit, hit ? vid : kEnd, hit ? node : kEnd, e,
+                            cur, org, d)) {
+                if (COUNT) {
+                    if (cur == 0) nr++;
+                    else nl++;
+                }
+            } else {
+                active = false;
+            }
+        }
     }
 
     if (COUNT) {
         atomicAdd(&ctrl->visits, nv);
         atomicAdd(&ctrl->tests, nt);
         atomicAdd(&ctrl->rays, nr);
+        atomicAdd(&ctrl->leaf_trav, nl);
+        work.flush(st, ctrl);
     }
-    // self-reset of the ray counter by the last wave to leave
+    // self-reset of the unit counter by the last wave to leave
     __threadfence();
     if (lane == 0) {
         uint32_t total = gridDim.x * (bdim >> 6);
@@ -632,7 +874,12 @@ struct voxg_tree {
 namespace {
 
 std::mutex g_mu;
-voxg_launch_config g_cfg = {4, 256, 32, 32};
+voxg_launch_config g_cfg = {8, 256, 20, 8, 4};
+
+voxg_launch_config current_config() {
+    std::lock_guard<std::mutex> g(g_mu);
+    return g_cfg;
+}
 thread_local int g_device = 0;
 
 struct Workspace {
@@ -687,8 +934,8 @@ dev::Tree dev_tree(const voxg_tree *t) {
     return d;
 }
 
-template <int K, class Src, class Sink, bool COUNT>
-int launch_k(const voxg_tree *t, const Src &src, const Sink &sink, size_t n,
+template <int K, class Work, bool COUNT>
+int launch_k(const voxg_tree *t, const Work &work, size_t n,
              hipStream_t stream, const voxg_launch_config &cfg,
              voxg_counts *counts) {
     int devid = 0;
@@ -710,19 +957,19 @@ int launch_k(const voxg_tree *t, const Src &src, const Sink &sink, size_t n,
                     &ws);
     if (rc) return rc;
     size_t lds = (size_t)K * block * sizeof(uint4);
-    auto kern = dev::rti_kernel<K, Src, Sink, COUNT>;
+    auto kern = dev::rti_kernel<K, Work, COUNT>;
     if (lds > 65536) {
         HIPCHK(hipFuncSetAttribute((const void *)kern,
                                    hipFuncAttributeMaxDynamicSharedMemorySize,
                                    (int)lds));
     }
     if (COUNT) {
-        HIPCHK(hipMemsetAsync(&ws->ctrl->visits, 0, 3 * sizeof(unsigned long long),
-                              stream));
+        HIPCHK(hipMemsetAsync(&ws->ctrl->visits, 0,
+                              8 * sizeof(unsigned long long), stream));
     }
     hipLaunchKernelGGL(kern, dim3(grid), dim3(block), lds, stream,
-                       dev_tree(t), src, sink, (uint32_t)n, ws->ctrl,
-                       ws->spill, slots, (uint32_t)cfg.refill);
+                       dev_tree(t), work, (uint32_t)n, ws->ctrl, ws->spill,
+                       slots, (uint32_t)cfg.refill);
     HIPCHK(hipGetLastError());
     if (COUNT) {
         dev::Ctrl h;
@@ -734,12 +981,16 @@ int launch_k(const voxg_tree *t, const Src &src, const Sink &sink, size_t n,
         counts->node_visits = h.visits;
         counts->voxel_tests = h.tests;
         counts->rays = h.rays;
+        counts->leaf_traversals = h.leaf_trav;
+        counts->fast_blocks = h.fast_blocks;
+        counts->best_blocks = h.best_blocks;
+        counts->merge_blocks = h.merge_blocks;
     }
     return 0;
 }
 
-template <class Src, class Sink, bool COUNT>
-int launch(const voxg_tree *t, const Src &src, const Sink &sink, size_t n,
+template <class Work, bool COUNT>
+int launch(const voxg_tree *t, const Work &work, size_t n,
            hipStream_t stream, voxg_counts *counts) {
     voxg_launch_config cfg;
     {
@@ -747,11 +998,11 @@ int launch(const voxg_tree *t, const Src &src, const Sink &sink, size_t n,
         cfg = g_cfg;
     }
     switch (cfg.stack_lds) {
-    case 2: return launch_k<2, Src, Sink, COUNT>(t, src, sink, n, stream, cfg, counts);
-    case 4: return launch_k<4, Src, Sink, COUNT>(t, src, sink, n, stream, cfg, counts);
-    case 8: return launch_k<8, Src, Sink, COUNT>(t, src, sink, n, stream, cfg, counts);
-    case 16: return launch_k<16, Src, Sink, COUNT>(t, src, sink, n, stream, cfg, counts);
-    case 32: return launch_k<32, Src, Sink, COUNT>(t, src, sink, n, stream, cfg, counts);
+    case 2: return launch_k<2, Work, COUNT>(t, work, n, stream, cfg, counts);
+    case 4: return launch_k<4, Work, COUNT>(t, work, n, stream, cfg, counts);
+    case 8: return launch_k<8, Work, COUNT>(t, work, n, stream, cfg, counts);
+    case 16: return launch_k<16, Work, COUNT>(t, work, n, stream, cfg, counts);
+    case 32: return launch_k<32, Work, COUNT>(t, work, n, stream, cfg, counts);
     default: return fail(VOXG_EINVAL, "stack_lds must be 2, 4, 8, 16 or 32");
     }
 }
@@ -977,6 +1228,8 @@ int voxg_set_launch_config(const voxg_launch_config *cfg) {
         return fail(VOXG_EINVAL, "waves_per_cu must be in 1..64");
     if (cfg->refill < 1 || cfg->refill > 64)
         return fail(VOXG_EINVAL, "refill must be in 1..64");
+    if (cfg->chunk < 1 || cfg->chunk > 4096)
+        return fail(VOXG_EINVAL, "chunk must be in 1..4096");
     std::lock_guard<std::mutex> g(g_mu);
     g_cfg = *cfg;
     return 0;
@@ -1010,10 +1263,12 @@ int voxg_ray_batch(const voxg_tree *t, size_t n, const float *ox,
     if ((hx || hy || hz) && !(hx && hy && hz))
         return fail(VOXG_EINVAL, "hx, hy, hz must be all set or all NULL");
     if (n == 0) return 0;
-    dev::SoaSrc src{ox, oy, oz, dx, dy, dz};
-    dev::SoaSink sink{voxel_out, leaf_out, hx, hy, hz};
-    return launch<dev::SoaSrc, dev::SoaSink, false>(t, src, sink, n, stream,
-                                                    nullptr);
+    uint32_t c = (uint32_t)current_config().chunk;
+    dev::ChunkWork<dev::SoaSrc, dev::SoaSink> work{
+        dev::SoaSrc{ox, oy, oz, dx, dy, dz},
+        dev::SoaSink{voxel_out, leaf_out, hx, hy, hz}, (uint32_t)n, c};
+    return launch<decltype(work), false>(t, work, (n + c - 1) / c, stream,
+                                         nullptr);
 }
 
 int voxg_ray_batch_count(const voxg_tree *t, size_t n, const float *ox,
@@ -1025,10 +1280,13 @@ int voxg_ray_batch_count(const voxg_tree *t, size_t n, const float *ox,
     if (!out) return fail(VOXG_EINVAL, "out is NULL");
     std::memset(out, 0, sizeof(*out));
     if (n == 0) return 0;
-    dev::SoaSrc src{ox, oy, oz, dx, dy, dz};
-    dev::SoaSink sink{nullptr, nullptr, nullptr, nullptr, nullptr};
-    return launch<dev::SoaSrc, dev::SoaSink, true>(t, src, sink, n, stream,
-                                                   out);
+    uint32_t c = (uint32_t)current_config().chunk;
+    dev::ChunkWork<dev::SoaSrc, dev::SoaSink> work{
+        dev::SoaSrc{ox, oy, oz, dx, dy, dz},
+        dev::SoaSink{nullptr, nullptr, nullptr, nullptr, nullptr},
+        (uint32_t)n, c};
+    return launch<decltype(work), true>(t, work, (n + c - 1) / c, stream,
+                                        out);
 }
 
 static int ball_launch(const voxg_tree *t, size_t n, const float *cx,
@@ -1048,7 +1306,7 @@ static int ball_launch(const voxg_tree *t, size_t n, const float *cx,
     int grid = (int)((n + block - 1) / block);
     if (counts) {
         HIPCHK(hipMemsetAsync(&ws->ctrl->visits, 0,
-                              3 * sizeof(unsigned long long), stream));
+                              8 * sizeof(unsigned long long), stream));
         hipLaunchKernelGGL(dev::ball_kernel<true>, dim3(grid), dim3(block), 0,
                            stream, dev_tree(t), (uint32_t)n, cx, cy, cz, r,
                            out, ws->ctrl);
@@ -1087,7 +1345,7 @@ int voxg_ball_batch_count(const voxg_tree *t, size_t n, const float *cx,
 static int render_common(const voxg_tree *t, size_t n_frames,
                          const voxg_camera *cams, int W, int H,
                          const voxg_render_params *params, uint32_t *squares,
-                         size_t *nrays) {
+                         size_t *npix) {
     int rc = check_tree(t);
     if (rc) return rc;
     if (W <= 0 || H <= 0 || (W & 0xf) || (H & 0x3))
@@ -1100,16 +1358,62 @@ static int render_common(const voxg_tree *t, size_t n_frames,
         return fail(VOXG_EINVAL, "ray merging needs the adaptive mode");
     if ((q & 3) > 2 || (q & 0xc) > 8 || (q & ~0xfu))
         return fail(VOXG_EINVAL, "reserved quality bits set");
-    if ((q & 3) != VOXG_QUALITY_BEST)
-        return fail(VOXG_EINVAL, "only VOXG_QUALITY_BEST is implemented");
     if (squares && (!params || !params->texture))
         return fail(VOXG_EINVAL, "colour output needs a texture");
     size_t per = (size_t)W * (size_t)H;
     if (n_frames * per >= 0x80000000ull)
         return fail(VOXG_ERANGE, "n_frames * W * H must be < 2^31");
-    *nrays = n_frames * per;
+    *npix = n_frames * per;
     return 0;
 }
+
+extern "C++" {
+static dev::PixSink make_sink(const voxg_tree *t,
+                              const voxg_render_params *params,
+                              uint32_t *squares, uint32_t *voxel_px,
+                              float *hit_px, size_t npix) {
+    dev::PixSink sink;
+    std::memset(&sink, 0, sizeof(sink));
+    sink.squares = squares;
+    sink.voxel = voxel_px;
+    sink.hit = hit_px;
+    sink.plane = npix;
+    sink.sh.tex = params ? params->texture : nullptr;
+    sink.sh.rshift = params ? params->rshift : 0;
+    sink.sh.gshift = params ? params->gshift : 0;
+    sink.sh.bshift = params ? params->bshift : 0;
+    sink.sh.amask = params ? params->amask : 0;
+    sink.sh.vx = t->host.voxel[0];
+    sink.sh.vy = t->host.voxel[1];
+    sink.sh.vz = t->host.voxel[2];
+    return sink;
+}
+
+template <bool COUNT>
+static int render_launch(const voxg_tree *t, size_t n_frames,
+                         const voxg_camera *cams, int W, int H,
+                         const voxg_render_params *params,
+                         const dev::PixSink &sink, size_t npix,
+                         voxg_counts *counts, hipStream_t stream) {
+    static_assert(sizeof(voxg_camera) == sizeof(dev::CamDev), "camera layout");
+    unsigned q = params ? params->quality : VOXG_QUALITY_BEST;
+    const dev::CamDev *cd = (const dev::CamDev *)cams;
+    // one 4x4 block per lane (renderer.c:214): in FAST / ADAPTIVE mode the
+    // leaf of one pixel is tried first for the next; in BEST mode every
+    // pixel is a root search, traced one after the other by the lane
+    dev::BlockWork work;
+    std::memset(&work, 0, sizeof(work));
+    work.cams = cd;
+    work.W = (uint32_t)W;
+    work.ws = (uint32_t)W / 4u;
+    work.bpf = (uint32_t)(W / 4) * (uint32_t)(H / 4);
+    work.rnd_mode = (int)(q & 3);
+    work.merge_mode = (int)(q & 0xc);
+    work.sink = sink;
+    return launch<dev::BlockWork, COUNT>(t, work, npix / 16, stream, counts);
+}
+
+}  // extern "C++"
 
 int voxg_render(const voxg_tree *t, size_t n_frames, const voxg_camera *cams,
                 int W, int H, const voxg_render_params *params,
@@ -1119,24 +1423,9 @@ int voxg_render(const voxg_tree *t, size_t n_frames, const voxg_camera *cams,
     int rc = render_common(t, n_frames, cams, W, H, params, squares, &n);
     if (rc) return rc;
     if (n == 0) return 0;
-    static_assert(sizeof(voxg_camera) == sizeof(dev::CamDev), "camera layout");
-    dev::CamSrc src{(const dev::CamDev *)cams, (uint32_t)W,
-                    (uint32_t)W * (uint32_t)H};
-    dev::PixSink sink;
-    sink.squares = squares;
-    sink.voxel = voxel_px;
-    sink.hit = hit_px;
-    sink.plane = n;
-    sink.sh.tex = params ? params->texture : nullptr;
-    sink.sh.rshift = params ? params->rshift : 0;
-    sink.sh.gshift = params ? params->gshift : 0;
-    sink.sh.bshift = params ? params->bshift : 0;
-    sink.sh.amask = params ? params->amask : 0;
-    sink.sh.vx = t->host.voxel[0];
-    sink.sh.vy = t->host.voxel[1];
-    sink.sh.vz = t->host.voxel[2];
-    return launch<dev::CamSrc, dev::PixSink, false>(t, src, sink, n, stream,
-                                                    nullptr);
+    dev::PixSink sink = make_sink(t, params, squares, voxel_px, hit_px, n);
+    return render_launch<false>(t, n_frames, cams, W, H, params, sink, n,
+                                nullptr, stream);
 }
 
 int voxg_render_count(const voxg_tree *t, size_t n_frames,
@@ -1149,12 +1438,9 @@ int voxg_render_count(const voxg_tree *t, size_t n_frames,
     int rc = render_common(t, n_frames, cams, W, H, params, nullptr, &n);
     if (rc) return rc;
     if (n == 0) return 0;
-    dev::CamSrc src{(const dev::CamDev *)cams, (uint32_t)W,
-                    (uint32_t)W * (uint32_t)H};
-    dev::PixSink sink;
-    std::memset(&sink, 0, sizeof(sink));
-    return launch<dev::CamSrc, dev::PixSink, true>(t, src, sink, n, stream,
-                                                   out);
+    dev::PixSink sink = make_sink(t, nullptr, nullptr, nullptr, nullptr, n);
+    return render_launch<true>(t, n_frames, cams, W, H, params, sink, n, out,
+                               stream);
 }
 
 }  // extern "C"

@@ -76,8 +76,7 @@ def render_counts(tree, cams, W, H, quality=QUALITY_BEST, stream=None):
     check(L.voxg_render_count(tree.handle, cams.shape[0], cams.data_ptr(), W,
                               H, C.byref(p), C.byref(c), _stream(stream)),
           "voxg_render_count")
-    return dict(node_visits=c.node_visits, voxel_tests=c.voxel_tests,
-                rays=c.rays)
+    return {k: getattr(c, k) for k, _ in c._fields_}
 
 
 def squares_to_image(squares, W, H):

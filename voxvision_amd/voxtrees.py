@@ -209,7 +209,7 @@ def ball_counts(tree, c, r, stream=None):
 
 
 def set_launch_config(stack_lds=None, block=None, waves_per_cu=None,
-                      refill=None):
+                      refill=None, chunk=None):
     L = _lib.load()
     cfg = _lib.LaunchConfig()
     check(L.voxg_get_launch_config(C.byref(cfg)))
@@ -221,6 +221,8 @@ def set_launch_config(stack_lds=None, block=None, waves_per_cu=None,
         cfg.waves_per_cu = waves_per_cu
     if refill is not None:
         cfg.refill = refill
+    if chunk is not None:
+        cfg.chunk = chunk
     check(L.voxg_set_launch_config(C.byref(cfg)), "voxg_set_launch_config")
     return get_launch_config()
 
@@ -229,4 +231,5 @@ def get_launch_config():
     cfg = _lib.LaunchConfig()
     check(_lib.load().voxg_get_launch_config(C.byref(cfg)))
     return dict(stack_lds=cfg.stack_lds, block=cfg.block,
-                waves_per_cu=cfg.waves_per_cu, refill=cfg.refill)
+                waves_per_cu=cfg.waves_per_cu, refill=cfg.refill,
+                chunk=cfg.chunk)
