@@ -62,6 +62,77 @@ def cpu_baseline(dots, cams, W, H, budget_s):
                       % (frames, W, H, dt, cores, W * H / dt1 / 1e6)}
 
 
+def batched_leg(ev0, ev1, stream, log):
+    """32 uneven scenes (edges 32..512, noise / shells / Menger sponges,
+    64K..8M rays each, half camera rays, half random): one plain
+    voxg_ray_batch launch per scene versus one persistent batched launch
+    (and its HIP-graph replay), each form at its own autotuned
+    configuration."""
+    import ctypes as C
+    import numpy as np
+    import torch
+    from voxvision_amd import _lib, autotune, batch, scenes, voxtrees as VT
+    L = _lib.load()
+    t0 = time.perf_counter()
+    c4 = scenes.config_c4()
+    trees = [VT.make_tree(s["dots"]) for s in c4]
+    log("[bench] batched leg: 32 scenes ready in %.1f s"
+        % (time.perf_counter() - t0))
+    o = np.concatenate([s["o"] for s in c4])
+    d = np.concatenate([s["d"] for s in c4])
+    first = np.cumsum([0] + [len(s["o"]) for s in c4])[:-1]
+    counts = [len(s["o"]) for s in c4]
+    n = len(o)
+    so, sd = batch.soa(o), batch.soa(d)
+    vox = torch.empty(n, dtype=torch.int32, device="cuda")
+    hit = [torch.empty(n, device="cuda") for _ in range(3)]
+    sp = C.c_void_p(stream.cuda_stream)
+
+    def plain_runner(cfg):
+        def run():
+            for t, f, cnt in zip(trees, first, counts):
+                off = int(f) * 4
+                _lib.check(L.voxg_ray_batch(
+                    t.handle, cnt, *[x.data_ptr() + off for x in so + sd],
+                    vox.data_ptr() + off, None,
+                    *[h.data_ptr() + off for h in hit], sp))
+        return run
+
+    def batch_runner(cfg):
+        B = batch.Batch(list(zip(trees, first, counts)))
+        return lambda: B.ray(so, sd, vox, None, hit)
+
+    space = ["chunk", "waves_per_cu", "stack_lds", "refill"]
+    pcfg, pms, _ = autotune.autotune(plain_runner, params=space, reps=2)
+    bcfg, bms, _ = autotune.autotune(batch_runner, params=space, reps=2)
+    VT.set_launch_config(**bcfg)
+    B = batch.Batch(list(zip(trees, first, counts)))
+    g = torch.cuda.CUDAGraph()
+    gs = torch.cuda.Stream()
+    with torch.cuda.stream(gs):
+        B.ray(so, sd, vox, None, hit, stream=gs)
+    gs.synchronize()
+    with torch.cuda.graph(g, stream=gs):
+        B.ray(so, sd, vox, None, hit, stream=gs)
+    g.replay()
+    torch.cuda.synchronize()
+    ev0.record(stream)
+    for _ in range(5):
+        g.replay()
+    ev1.record(stream)
+    torch.cuda.synchronize()
+    gms = ev0.elapsed_time(ev1) / 5
+    log("[bench] batched leg: plain %.2f ms, batched %.2f ms, graph %.2f ms"
+        % (pms, bms, gms))
+    return {"scenes": len(c4), "rays": n,
+            "voxels": int(sum(len(s["dots"]) for s in c4)),
+            "plain_mrays_s": round(n / pms / 1e3, 1),
+            "batched_mrays_s": round(n / bms / 1e3, 1),
+            "graph_mrays_s": round(n / gms / 1e3, 1),
+            "speedup_batched_vs_plain": round(pms / min(bms, gms), 3),
+            "plain_config": pcfg, "batched_config": bcfg}
+
+
 def load_traffic(name):
     p = os.path.join(ROOT, "profiles", name)
     if not os.path.exists(p):
@@ -81,6 +152,10 @@ def main():
     ap.add_argument("--cpu-seconds", type=float, default=12.0)
     ap.add_argument("--quick", action="store_true",
                     help="skip the incoherent and ball extras")
+    ap.add_argument("--no-batched", action="store_true",
+                    help="skip the 32-scene batched-vs-plain leg")
+    ap.add_argument("--no-tune", action="store_true",
+                    help="use the library's default launch configuration")
     ap.add_argument("--copy-calib-mib", type=int, default=0,
                     help="run a plain device copy of this size first (the "
                          "reference kernel for HBM counter calibration)")
@@ -113,6 +188,16 @@ def main():
 
     def step():
         voxrnd.render(G, cams, W, H, out=out)
+
+    tuned = None
+    if not args.no_tune:
+        from voxvision_amd import autotune
+        cfg, best_ms, hist = autotune.autotune(
+            lambda cfg: step, params=["stack_lds", "waves_per_cu", "refill",
+                                      "block"], reps=2)
+        tuned = {"config": cfg, "ms": round(best_ms, 3),
+                 "candidates": len(hist)}
+        log("[bench] autotuned render config %s (%.3f ms)" % (cfg, best_ms))
 
     if args.copy_calib_mib:
         nb = args.copy_calib_mib << 20
@@ -202,6 +287,9 @@ def main():
         bms = ev0.elapsed_time(ev1) / 5
         extra["balls_4M_mqueries_s"] = round((1 << 22) / bms / 1e3, 1)
 
+    if not args.no_batched:
+        extra["batched"] = batched_leg(ev0, ev1, stream, log)
+
     cpu = None
     if not args.no_cpu:
         log("[bench] CPU baseline (%.0f s budget)" % args.cpu_seconds)
@@ -220,6 +308,8 @@ def main():
                    "outputs": "voxel id + hit point per pixel"},
         "roofline": roofline, "cpu_baseline": cpu, "extra": extra,
     }
+    if tuned:
+        line["config"]["launch"] = tuned["config"]
     print(json.dumps(line), flush=True)
 
 

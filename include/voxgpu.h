@@ -211,6 +211,28 @@ int voxg_render_count(const voxg_tree *tree, size_t n_frames,
                       const voxg_render_params *params, voxg_counts *out,
                       hipStream_t stream);
 
+/* ---- batched form: many trees in one persistent launch ----
+ * A batch is a device work queue of items, each a tree and a run of
+ * consecutive rays [first, first+count) of the ray arrays passed at launch.
+ * voxg_batch_create uploads the queue once (it allocates; call it outside
+ * graph capture); voxg_batch_ray is a single kernel launch that needs no
+ * memset, so it can be captured in a HIP graph and replayed.  Results are
+ * those of voxg_ray_batch on each item's tree.  Trees must outlive the
+ * batch; the ray chunk of the launch configuration is fixed at creation. */
+typedef struct voxg_batch voxg_batch;
+typedef struct {
+    const voxg_tree *tree;
+    size_t first;
+    size_t count;
+} voxg_batch_item;
+int voxg_batch_create(size_t n_items, const voxg_batch_item *items,
+                      voxg_batch **out);
+int voxg_batch_ray(const voxg_batch *batch, const float *ox, const float *oy,
+                   const float *oz, const float *dx, const float *dy,
+                   const float *dz, uint32_t *voxel_out, uint32_t *leaf_out,
+                   float *hx, float *hy, float *hz, hipStream_t stream);
+int voxg_batch_release(voxg_batch *batch);
+
 /* Launch configuration of the traversal kernels (autotuned by
  * voxvision_amd.autotune; defaults are sane).  stack_lds: stack entries per
  * lane kept in LDS (power of two 2..32; deeper stacks spill to HBM and stay

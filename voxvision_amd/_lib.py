@@ -24,7 +24,8 @@ EXPORTS = [
     "voxg_tree_node_ptr", "voxg_ray_batch", "voxg_ball_batch",
     "voxg_camera_set_window_size", "voxg_render", "voxg_ray_batch_count",
     "voxg_ball_batch_count", "voxg_render_count", "voxg_set_launch_config",
-    "voxg_get_launch_config", "voxg_last_error",
+    "voxg_get_launch_config", "voxg_last_error", "voxg_batch_create",
+    "voxg_batch_ray", "voxg_batch_release",
 ]
 
 
@@ -59,6 +60,11 @@ class RenderParams(C.Structure):
     _fields_ = [("quality", C.c_uint), ("texture", C.c_void_p),
                 ("rshift", C.c_uint32), ("gshift", C.c_uint32),
                 ("bshift", C.c_uint32), ("amask", C.c_uint32)]
+
+
+class BatchItem(C.Structure):
+    _fields_ = [("tree", C.c_void_p), ("first", C.c_size_t),
+                ("count", C.c_size_t)]
 
 
 class LaunchConfig(C.Structure):
@@ -112,6 +118,9 @@ def load():
     L.voxg_set_launch_config.argtypes = [C.POINTER(LaunchConfig)]
     L.voxg_get_launch_config.argtypes = [C.POINTER(LaunchConfig)]
     L.voxg_last_error.restype = C.c_char_p
+    L.voxg_batch_create.argtypes = [S, C.POINTER(BatchItem), C.POINTER(P)]
+    L.voxg_batch_ray.argtypes = [P] + [P] * 6 + [P] * 5 + [P]
+    L.voxg_batch_release.argtypes = [P]
     _lib = L
     return L
 

@@ -1,0 +1,88 @@
+"""Autotuner for the traversal launch configuration.
+
+The tuned parameters are those of voxg_launch_config: entries of the
+per-lane LDS stack ring (`stack_lds`), threads per workgroup (`block`),
+persistent waves per CU (`waves_per_cu`), idle lanes that trigger a refill
+(`refill`) and rays a lane takes per fetch (`chunk`).  The search is
+coordinate descent from the current configuration: each parameter in turn
+is swept over its candidate values with the others fixed, keeping the
+fastest, until a full pass changes nothing.  Timing uses HIP events on the
+current stream; each candidate is warmed up once and timed over `reps`
+runs.  Configurations whose LDS use (stack_lds * block * 16 B) exceeds
+64 KiB are skipped.
+"""
+import itertools
+import json
+
+import torch
+
+from . import voxtrees as VT
+
+SPACE = {
+    "stack_lds": [2, 4, 8, 16],
+    "block": [64, 128, 256],
+    "waves_per_cu": [8, 12, 16, 20, 24, 32],
+    "refill": [1, 4, 8, 16, 32],
+    "chunk": [1, 2, 4, 8, 16, 32],
+}
+
+
+def _valid(cfg):
+    return cfg["stack_lds"] * cfg["block"] * 16 <= 64 * 1024
+
+
+def time_config(make_runner, cfg, reps=3):
+    """Milliseconds per run of the workload under `cfg`."""
+    VT.set_launch_config(**cfg)
+    run = make_runner(cfg)
+    run()
+    torch.cuda.synchronize()
+    e0 = torch.cuda.Event(enable_timing=True)
+    e1 = torch.cuda.Event(enable_timing=True)
+    e0.record()
+    for _ in range(reps):
+        run()
+    e1.record()
+    torch.cuda.synchronize()
+    return e0.elapsed_time(e1) / reps
+
+
+def autotune(make_runner, params=None, start=None, reps=3, max_passes=2,
+             log=None):
+    """make_runner(cfg) -> zero-argument callable running the workload once
+    (built after the configuration is set, for objects such as batches
+    whose layout depends on it).  Returns (best_cfg, best_ms, history)."""
+    params = list(params or SPACE.keys())
+    cfg = dict(start or VT.get_launch_config())
+    best = time_config(make_runner, cfg, reps)
+    history = [(dict(cfg), best)]
+    if log:
+        log("start %s %.3f ms" % (json.dumps(cfg), best))
+    seen = {json.dumps(cfg, sort_keys=True): best}
+    for _ in range(max_passes):
+        changed = False
+        for p in params:
+            for v in SPACE[p]:
+                cand = dict(cfg)
+                cand[p] = v
+                key = json.dumps(cand, sort_keys=True)
+                if key in seen or not _valid(cand):
+                    continue
+                ms = time_config(make_runner, cand, reps)
+                seen[key] = ms
+                history.append((dict(cand), ms))
+                if ms < best * 0.99:
+                    best, cfg, changed = ms, cand, True
+                    if log:
+                        log("  %s=%s -> %.3f ms" % (p, v, ms))
+        if not changed:
+            break
+    VT.set_launch_config(**cfg)
+    return cfg, best, history
+
+
+def grid(params):
+    """All combinations of the given parameter lists (small sweeps)."""
+    keys = list(params)
+    for vals in itertools.product(*[params[k] for k in keys]):
+        yield dict(zip(keys, vals))

@@ -352,6 +352,10 @@ struct RayWork {
         return false;
     }
     __device__ __forceinline__ void flush(const State &, Ctrl *) const {}
+    __device__ __forceinline__ const Tree &tree(const State &,
+                                                const Tree &t) const {
+        return t;
+    }
 };
 
 // A run of `chunk` consecutive rays per unit, traced one after the other
@@ -382,6 +386,60 @@ struct ChunkWork {
         return true;
     }
     __device__ __forceinline__ void flush(const State &, Ctrl *) const {}
+    __device__ __forceinline__ const Tree &tree(const State &,
+                                                const Tree &t) const {
+        return t;
+    }
+};
+
+// One item of a multi-tree batch: a tree and a run of rays of the batch.
+struct BatchItem {
+    Tree t;
+    uint32_t ray_begin, ray_count, unit_begin, pad;
+};
+
+// Many trees in one launch: units are ray chunks of all items; a lane
+// finds its unit's item by binary search over the items' first units and
+// traverses that item's tree.
+template <class Src, class Sink>
+struct MultiWork {
+    Src src;
+    Sink sink;
+    const BatchItem *items;
+    uint32_t n_items, chunk;
+    struct State {
+        uint32_t ray, end;
+        Tree t;
+    };
+    __device__ __forceinline__ void begin(uint32_t unit, State &st,
+                                          uint32_t &cur, f3 &org, f3 &d) const {
+        uint32_t lo = 0, hi = n_items - 1;
+        while (lo < hi) {
+            uint32_t mid = (lo + hi + 1) >> 1;
+            if (items[mid].unit_begin <= unit) lo = mid;
+            else hi = mid - 1;
+        }
+        const BatchItem &it = items[lo];
+        st.t = it.t;
+        st.ray = it.ray_begin + (unit - it.unit_begin) * chunk;
+        st.end = min(st.ray + chunk, it.ray_begin + it.ray_count);
+        src.get(st.ray, org, d);
+        cur = 0;
+    }
+    __device__ __forceinline__ bool finish(State &st, bool hit, uint32_t vid,
+                                           uint32_t node, f3 e, uint32_t &cur,
+                                           f3 &org, f3 &d) const {
+        sink.put(st.ray, hit, vid, node, e);
+        if (++st.ray >= st.end) return false;
+        src.get(st.ray, org, d);
+        cur = 0;
+        return true;
+    }
+    __device__ __forceinline__ void flush(const State &, Ctrl *) const {}
+    __device__ __forceinline__ const Tree &tree(const State &st,
+                                                const Tree &) const {
+        return st.t;
+    }
 };
 
 // renderer.c:23-25
@@ -544,6 +602,10 @@ struct BlockWork {
         st.i++;
         return next(st, cur, org, d);
     }
+    __device__ __forceinline__ const Tree &tree(const State &,
+                                                const Tree &t) const {
+        return t;
+    }
     __device__ __forceinline__ void flush(const State &st, Ctrl *ctrl) const {
         atomicAdd(&ctrl->fast_blocks, (unsigned long long)st.fast);
         atomicAdd(&ctrl->best_blocks, (unsigned long long)st.best);
@@ -564,7 +626,6 @@ rti_kernel(Tree t, Work work, uint32_t n, Ctrl *ctrl,
     const uint32_t gtid = blockIdx.x * bdim + tid;
     const uint32_t gthreads = gridDim.x * bdim;
     const uint32_t lane = lane_id();
-    const f3 V = f3{t.vx, t.vy, t.vz};
 
     bool active = false;
     bool exhausted = false;
@@ -606,27 +667,29 @@ rti_kernel(Tree t, Work work, uint32_t n, Ctrl *ctrl,
         if (!active) continue;
 
         // ---- process (cur, org) ----
+        const Tree &T = work.tree(st, t);
+        const f3 V = f3{T.vx, T.vy, T.vz};
         f3 e = org;
         bool hit = false, descend = false;
         uint32_t vid = kEnd;
         Node nd;
-        if (!t.empty) {
-            nd = load_node(t.nodes, cur);
+        if (!T.empty) {
+            nd = load_node(T.nodes, cur);
             if (COUNT) nv++;
         }
-        if (!t.empty && hit_box(nd.bmin, nd.bmax, org, d, e)) {
+        if (!T.empty && hit_box(nd.bmin, nd.bmax, org, d, e)) {
             uint32_t type = nd.info & NT_TYPE_MASK;
             if (type == NT_DENSE) {
                 // search.c:31-37
                 hit = true;
-                vid = dense_voxel(nd, e, t);
+                vid = dense_voxel(nd, e, T);
             } else if (type == NT_LEAF) {
                 // search.c:39-81
                 uint32_t cnt = nd.info >> 8;
                 float best = INFINITY;
                 f3 bres = e;
                 for (uint32_t k = 0; k < cnt; k++) {
-                    const float *p = t.dots + 3u * (nd.ref + k);
+                    const float *p = T.dots + 3u * (nd.ref + k);
                     f3 vmin = f3{p[0], p[1], p[2]};
                     f3 vmax = f3{vmin.x + V.x, vmin.y + V.y, vmin.z + V.z};
                     f3 far;
@@ -871,6 +934,13 @@ struct voxg_tree {
     uint32_t stack_bound = 0;
 };
 
+struct voxg_batch {
+    int device = 0;
+    dev::BatchItem *d_items = nullptr;
+    uint32_t n_items = 0, units = 0, chunk = 1, stack_bound = 1;
+    uint64_t rays_end = 0;  // one past the highest ray index used
+};
+
 namespace {
 
 std::mutex g_mu;
@@ -935,8 +1005,8 @@ dev::Tree dev_tree(const voxg_tree *t) {
 }
 
 template <int K, class Work, bool COUNT>
-int launch_k(const voxg_tree *t, const Work &work, size_t n,
-             hipStream_t stream, const voxg_launch_config &cfg,
+int launch_k(const dev::Tree &dt, uint32_t stack_bound, const Work &work,
+             size_t n, hipStream_t stream, const voxg_launch_config &cfg,
              voxg_counts *counts) {
     int devid = 0;
     HIPCHK(hipGetDevice(&devid));
@@ -950,7 +1020,7 @@ int launch_k(const voxg_tree *t, const Work &work, size_t n,
     if (need_blocks < 1) need_blocks = 1;
     if (grid > need_blocks) grid = (int)need_blocks;
     uint32_t gthreads = (uint32_t)grid * (uint32_t)block;
-    uint32_t slots = t->stack_bound > (uint32_t)K ? t->stack_bound - K : 0u;
+    uint32_t slots = stack_bound > (uint32_t)K ? stack_bound - K : 0u;
     Workspace *ws = nullptr;
     int rc = get_ws(stream, (size_t)std::max<uint32_t>(slots, 1u) * gthreads *
                                 sizeof(uint4),
@@ -967,9 +1037,9 @@ int launch_k(const voxg_tree *t, const Work &work, size_t n,
         HIPCHK(hipMemsetAsync(&ws->ctrl->visits, 0,
                               8 * sizeof(unsigned long long), stream));
     }
-    hipLaunchKernelGGL(kern, dim3(grid), dim3(block), lds, stream,
-                       dev_tree(t), work, (uint32_t)n, ws->ctrl, ws->spill,
-                       slots, (uint32_t)cfg.refill);
+    hipLaunchKernelGGL(kern, dim3(grid), dim3(block), lds, stream, dt, work,
+                       (uint32_t)n, ws->ctrl, ws->spill, slots,
+                       (uint32_t)cfg.refill);
     HIPCHK(hipGetLastError());
     if (COUNT) {
         dev::Ctrl h;
@@ -990,21 +1060,24 @@ int launch_k(const voxg_tree *t, const Work &work, size_t n,
 }
 
 template <class Work, bool COUNT>
-int launch(const voxg_tree *t, const Work &work, size_t n,
-           hipStream_t stream, voxg_counts *counts) {
-    voxg_launch_config cfg;
-    {
-        std::lock_guard<std::mutex> g(g_mu);
-        cfg = g_cfg;
-    }
+int launch_tree(const dev::Tree &t, uint32_t stack_bound, const Work &work,
+                size_t n, hipStream_t stream, voxg_counts *counts) {
+    voxg_launch_config cfg = current_config();
     switch (cfg.stack_lds) {
-    case 2: return launch_k<2, Work, COUNT>(t, work, n, stream, cfg, counts);
-    case 4: return launch_k<4, Work, COUNT>(t, work, n, stream, cfg, counts);
-    case 8: return launch_k<8, Work, COUNT>(t, work, n, stream, cfg, counts);
-    case 16: return launch_k<16, Work, COUNT>(t, work, n, stream, cfg, counts);
-    case 32: return launch_k<32, Work, COUNT>(t, work, n, stream, cfg, counts);
+    case 2: return launch_k<2, Work, COUNT>(t, stack_bound, work, n, stream, cfg, counts);
+    case 4: return launch_k<4, Work, COUNT>(t, stack_bound, work, n, stream, cfg, counts);
+    case 8: return launch_k<8, Work, COUNT>(t, stack_bound, work, n, stream, cfg, counts);
+    case 16: return launch_k<16, Work, COUNT>(t, stack_bound, work, n, stream, cfg, counts);
+    case 32: return launch_k<32, Work, COUNT>(t, stack_bound, work, n, stream, cfg, counts);
     default: return fail(VOXG_EINVAL, "stack_lds must be 2, 4, 8, 16 or 32");
     }
+}
+
+template <class Work, bool COUNT>
+int launch(const voxg_tree *t, const Work &work, size_t n,
+           hipStream_t stream, voxg_counts *counts) {
+    return launch_tree<Work, COUNT>(dev_tree(t), t->stack_bound, work, n,
+                                    stream, counts);
 }
 
 int check_tree(const voxg_tree *t) {
@@ -1441,6 +1514,91 @@ int voxg_render_count(const voxg_tree *t, size_t n_frames,
     dev::PixSink sink = make_sink(t, nullptr, nullptr, nullptr, nullptr, n);
     return render_launch<true>(t, n_frames, cams, W, H, params, sink, n, out,
                                stream);
+}
+
+
+int voxg_batch_create(size_t n_items, const voxg_batch_item *items,
+                      voxg_batch **out) {
+    if (!out) return fail(VOXG_EINVAL, "out is NULL");
+    *out = nullptr;
+    if (n_items == 0 || !items) return fail(VOXG_EINVAL, "no items");
+    if (n_items > (1u << 20)) return fail(VOXG_ERANGE, "too many items");
+    int devid = 0;
+    HIPCHK(hipGetDevice(&devid));
+    uint32_t chunk = (uint32_t)current_config().chunk;
+    std::vector<dev::BatchItem> h;
+    uint64_t units = 0, rays_end = 0;
+    uint32_t bound = 1;
+    for (size_t i = 0; i < n_items; i++) {
+        const voxg_batch_item &it = items[i];
+        int rc = check_tree(it.tree);
+        if (rc) return rc;
+        if (it.first + it.count >= 0x80000000ull)
+            return fail(VOXG_ERANGE, "ray indices must stay below 2^31");
+        if (it.count == 0) continue;
+        dev::BatchItem b;
+        std::memset(&b, 0, sizeof(b));
+        b.t = dev_tree(it.tree);
+        b.ray_begin = (uint32_t)it.first;
+        b.ray_count = (uint32_t)it.count;
+        b.unit_begin = (uint32_t)units;
+        units += (it.count + chunk - 1) / chunk;
+        rays_end = std::max<uint64_t>(rays_end, it.first + it.count);
+        bound = std::max(bound, it.tree->stack_bound);
+        h.push_back(b);
+    }
+    if (units >= 0x80000000ull) return fail(VOXG_ERANGE, "too many rays");
+    std::unique_ptr<voxg_batch> b(new (std::nothrow) voxg_batch());
+    if (!b) return fail(VOXG_ENOMEM, "host allocation failed");
+    b->device = devid;
+    b->n_items = (uint32_t)h.size();
+    b->units = (uint32_t)units;
+    b->chunk = chunk;
+    b->stack_bound = bound;
+    b->rays_end = rays_end;
+    if (!h.empty()) {
+        HIPCHK(hipMalloc(&b->d_items, h.size() * sizeof(dev::BatchItem)));
+        hipError_t e = hipMemcpy(b->d_items, h.data(),
+                                 h.size() * sizeof(dev::BatchItem),
+                                 hipMemcpyHostToDevice);
+        if (e != hipSuccess) {
+            (void)hipFree(b->d_items);
+            return fail(VOXG_EIO, hipGetErrorString(e));
+        }
+    }
+    *out = b.release();
+    return 0;
+}
+
+int voxg_batch_release(voxg_batch *b) {
+    if (!b) return 0;
+    if (b->d_items) HIPCHK(hipFree(b->d_items));
+    delete b;
+    return 0;
+}
+
+int voxg_batch_ray(const voxg_batch *b, const float *ox, const float *oy,
+                   const float *oz, const float *dx, const float *dy,
+                   const float *dz, uint32_t *voxel_out, uint32_t *leaf_out,
+                   float *hx, float *hy, float *hz, hipStream_t stream) {
+    if (!b) return fail(VOXG_EINVAL, "batch is NULL");
+    int devid = 0;
+    HIPCHK(hipGetDevice(&devid));
+    if (devid != b->device)
+        return fail(VOXG_EINVAL, "batch lives on another device");
+    if (b->units == 0) return 0;
+    if (!ox || !oy || !oz || !dx || !dy || !dz)
+        return fail(VOXG_EINVAL, "ray input pointer is NULL");
+    if ((hx || hy || hz) && !(hx && hy && hz))
+        return fail(VOXG_EINVAL, "hx, hy, hz must be all set or all NULL");
+    dev::MultiWork<dev::SoaSrc, dev::SoaSink> work{
+        dev::SoaSrc{ox, oy, oz, dx, dy, dz},
+        dev::SoaSink{voxel_out, leaf_out, hx, hy, hz}, b->d_items,
+        b->n_items, b->chunk};
+    dev::Tree none;
+    std::memset(&none, 0, sizeof(none));
+    return launch_tree<decltype(work), false>(none, b->stack_bound, work,
+                                              b->units, stream, nullptr);
 }
 
 }  // extern "C"

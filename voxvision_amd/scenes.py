@@ -180,3 +180,104 @@ def config_c2(n=256, seed=7, frames=64, W=1920, H=1080):
     cams = orbit_cameras(center, 1.5 * diag, frames, W, H)
     return dict(dots=dots, occupancy=float(occ.mean()), cameras=cams, W=W,
                 H=H, bbox=(bmin, bmax))
+
+
+# ---------------------------------------------------------------------------
+# uneven multi-scene batch (32 scenes of 32^3..512^3)
+# ---------------------------------------------------------------------------
+def menger_volume(edge, level=4):
+    """Level-`level` Menger sponge on a 3^level grid, resampled to edge^3:
+    a cell is empty when, at some level, two or more of its base-3 digits
+    are 1."""
+    base = 3 ** level
+    idx = (np.arange(edge) * base) // edge
+    holes = np.zeros((edge,) * 3, bool)
+    for lv in range(level):
+        dig = (idx // (3 ** lv)) % 3 == 1
+        ones = (dig[:, None, None].astype(np.int8) +
+                dig[None, :, None].astype(np.int8) +
+                dig[None, None, :].astype(np.int8))
+        holes |= ones >= 2
+    return ~holes
+
+
+def shell_volume(edge, occupancy):
+    """Hollow sphere: outer radius 0.48*edge, thickness chosen so the shell
+    fills `occupancy` of the cube (capped at a full ball)."""
+    c = (edge - 1) / 2.0
+    R = 0.48 * edge
+    t = min(R, occupancy * edge ** 3 / (4.0 * np.pi * R * R))
+    x = np.arange(edge, dtype=np.float32) - c
+    r2 = x[:, None, None] ** 2 + x[None, :, None] ** 2 + x[None, None, :] ** 2
+    return (r2 <= R * R) & (r2 >= (R - t) ** 2)
+
+
+def noise_volume(edge, occupancy, seed):
+    v = value_noise(edge, seed)
+    k = max(1, int(occupancy * v.size))
+    flat = v.reshape(-1)
+    thr = np.partition(flat, flat.size - k)[flat.size - k]
+    return v >= thr
+
+
+def camera_rays(cam, W, H):
+    """Host-side pinhole rays of a simple camera, (W*H, 3) origins and
+    directions in float32 (row-major pixels)."""
+    q = np.array(cam["rotation"], np.float64)
+    xs = np.arange(W, dtype=np.float64) * cam["mul"] - cam["xsub"]
+    ys = cam["ysub"] - np.arange(H, dtype=np.float64) * cam["mul"]
+    vx = np.broadcast_to(xs[None, :], (H, W)).reshape(-1)
+    vz = np.broadcast_to(ys[:, None], (H, W)).reshape(-1)
+    v = np.stack([vx, np.ones_like(vx), vz], 1)
+    w, u = q[0], q[1:]
+    t = 2.0 * np.cross(u, v)
+    d = v + w * t + np.cross(u, t)
+    o = np.broadcast_to(np.asarray(cam["position"], np.float32), d.shape)
+    return np.ascontiguousarray(o, np.float32), d.astype(np.float32)
+
+
+def config_c4(seed=23, n_scenes=32, edge_range=(32, 512),
+              ray_range=(1 << 16, 1 << 23), max_edge=None, ray_scale=1.0):
+    """32 scenes of uneven size: edges log-uniform in edge_range, a mix of
+    value-noise volumes (1-40% occupancy), hollow sphere shells (1-40%)
+    and level-4 Menger sponges; rays per scene log-uniform in ray_range,
+    half from a pinhole camera looking at the scene, half random."""
+    rng = np.random.default_rng(seed)
+    out = []
+    for s in range(n_scenes):
+        edge = int(round(np.exp(rng.uniform(np.log(edge_range[0]),
+                                            np.log(edge_range[1])))))
+        if max_edge:
+            edge = min(edge, max_edge)
+        kind = ("noise", "shell", "menger")[s % 3]
+        occ = float(rng.uniform(0.01, 0.40))
+        nrays = int(np.exp(rng.uniform(np.log(ray_range[0]),
+                                       np.log(ray_range[1]))) * ray_scale)
+        nrays = max(64, nrays)
+        if kind == "noise":
+            vol = noise_volume(edge, occ, seed * 1000 + s)
+        elif kind == "shell":
+            vol = shell_volume(edge, occ)
+        else:
+            vol = menger_volume(edge)
+        dots = volume_to_dots(vol)
+        occ_real = float(vol.mean())
+        del vol
+        bmin, bmax = bbox_of(dots) if len(dots) else (np.zeros(3),
+                                                      np.ones(3) * edge)
+        ncam = nrays // 2
+        W = max(16, int(np.sqrt(ncam * 16 / 9)))
+        H = max(1, ncam // W)
+        center = (bmin + bmax) / 2.0
+        a = rng.uniform(0, 2 * np.pi)
+        dist = 1.5 * np.sqrt(3.0) * edge
+        pos = center + dist * np.array([np.cos(a) * 0.94, np.sin(a) * 0.94,
+                                        0.34])
+        cam = _camera.make_camera(pos, center, np.tan(np.radians(30)), W, H)
+        oc, dc = camera_rays(cam, W, H)
+        orr, drr = random_rays(nrays - len(oc), bmin, bmax,
+                               seed=seed * 1000 + 500 + s)
+        out.append(dict(kind=kind, edge=edge, occupancy=occ_real, dots=dots,
+                        o=np.concatenate([oc, orr]),
+                        d=np.concatenate([dc, drr]), camera_pixels=len(oc)))
+    return out
