@@ -233,6 +233,69 @@ int voxg_batch_ray(const voxg_batch *batch, const float *ox, const float *oy,
                    float *hx, float *hy, float *hz, hipStream_t stream);
 int voxg_batch_release(voxg_batch *batch);
 
+/* ---- serving: a request loop over resident trees ----
+ * A server keeps trees resident, accepts render and ball-query requests
+ * from any thread, admits them in arrival order while request slots and
+ * its device-memory budget allow, and launches every request admitted
+ * together as one group (one multi-job render launch plus one multi-job
+ * ball launch) on one of several streams, so groups that arrive while
+ * others run overlap on the device.  Results are copied into the caller's
+ * host buffers (pass pinned memory, e.g. hipHostMalloc, for asynchronous
+ * copies) and equal those of voxg_render / voxg_ball_batch on the same
+ * inputs.  A request that can never fit the budget is rejected
+ * (VOXG_ENOMEM). */
+typedef struct voxg_server voxg_server;
+typedef struct {
+    size_t hbm_budget;      /* device bytes for request buffers;
+                               0 = half of the free HBM at creation */
+    int max_inflight;       /* requests admitted at once; 0 = 256 */
+    int streams;            /* launch streams; 0 = 4 */
+    const uint8_t *texture; /* host 32^3 colour texture; NULL = no colours */
+    uint32_t rshift, gshift, bshift, amask;
+} voxg_server_config;
+typedef struct {
+    int tree;               /* id from voxg_server_add_tree */
+    voxg_camera cam;        /* host copy */
+    int W, H;               /* W % 16 == 0, H % 4 == 0 */
+    unsigned quality;       /* VOXG_QUALITY_* */
+    uint32_t *voxel_out;    /* host, W*H in the square layout, or NULL */
+    float *hit_out;         /* host, 3 planes of W*H (unspecified where
+                               the pixel missed), or NULL */
+    uint32_t *color_out;    /* host, W*H (needs a texture; 0 where the
+                               pixel missed, as after vox_redraw), or NULL */
+} voxg_render_request;
+typedef struct {
+    int tree;
+    size_t n;
+    const float *cx, *cy, *cz, *r; /* host; copied at submission */
+    uint8_t *out;                  /* host, n */
+} voxg_ball_request;
+typedef struct {
+    int status;             /* 0 ok, < 0 error code */
+    double t_submit, t_admit, t_done; /* seconds on voxg_server_now's clock */
+    uint64_t group;         /* launch group the request ran in */
+    uint32_t group_size;    /* requests in that group */
+} voxg_request_info;
+typedef struct {
+    uint64_t submitted, completed, rejected, groups;
+    uint64_t peak_inflight, peak_groups_inflight;
+    uint64_t peak_bytes, budget;
+} voxg_server_stats;
+int voxg_server_create(const voxg_server_config *cfg, voxg_server **out);
+int voxg_server_add_tree(voxg_server *s, const voxg_tree *tree, int *id);
+int voxg_server_submit_render(voxg_server *s, const voxg_render_request *r,
+                              uint64_t *ticket);
+int voxg_server_submit_balls(voxg_server *s, const voxg_ball_request *r,
+                             uint64_t *ticket);
+/* Blocks until the request is done (or timeout_s passes: VOXG_EIO with
+ * info->status untouched); a finished ticket is forgotten after wait. */
+int voxg_server_wait(voxg_server *s, uint64_t ticket, double timeout_s,
+                     voxg_request_info *info);
+int voxg_server_get_stats(voxg_server *s, voxg_server_stats *st);
+double voxg_server_now(const voxg_server *s);
+/* Waits for the work in flight, then frees the server (not the trees). */
+int voxg_server_destroy(voxg_server *s);
+
 /* Launch configuration of the traversal kernels (autotuned by
  * voxvision_amd.autotune; defaults are sane).  stack_lds: stack entries per
  * lane kept in LDS (power of two 2..32; deeper stacks spill to HBM and stay

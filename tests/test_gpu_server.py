@@ -1,0 +1,137 @@
+"""Serving loop: results are the same alone and in a mix of concurrent
+requests (and equal to the plain entry points); admission respects the
+device-memory budget and the slot count; oversized requests are
+rejected."""
+import threading
+
+import numpy as np
+import pytest
+import torch
+
+pytestmark = pytest.mark.gpu
+MISS = 0xFFFFFFFF
+
+
+def _setup():
+    from voxvision_amd import scenes, voxtrees as VT
+    c1 = scenes.config_c1()
+    rng = np.random.default_rng(3)
+    dots2 = scenes.volume_to_dots(rng.random((40, 30, 36)) < 0.12)
+    return c1, VT.make_tree(c1["dots"]), VT.make_tree(dots2), dots2
+
+
+def _cam_for(dots, W, H, seed):
+    from voxvision_amd import camera, scenes
+    bmin, bmax = scenes.bbox_of(dots)
+    c = (bmin + bmax) / 2
+    rng = np.random.default_rng(seed)
+    a = rng.uniform(0, 6.28)
+    dist = 1.6 * float(np.linalg.norm(bmax - bmin))
+    pos = c + dist * np.array([np.cos(a), np.sin(a), 0.3])
+    return camera.make_camera(pos, c, np.tan(np.radians(30)), W, H)
+
+
+def test_alone_equals_in_mix(gpu, oracle):
+    from voxvision_amd import scenes, server, voxrnd
+    c1, t1, t2, dots2 = _setup()
+    tex = np.random.default_rng(1).integers(0, 256, 32 ** 3).astype(np.uint8)
+    S = server.Server(streams=4, texture=tex)
+    i1, i2 = S.add_tree(t1), S.add_tree(t2)
+    cam = c1["camera"]
+    # alone
+    tk, ref = S.submit_render(i1, cam, 640, 480, quality=2 | 8, colors=True)
+    info = S.wait(tk)
+    assert info["status"] == 0 and info["group_size"] == 1
+    bc, br = c1["balls_c"], c1["balls_r"]
+    tb, refb = S.submit_balls(i1, bc, br)
+    S.wait(tb)
+    # equal to the plain entry points
+    out = voxrnd.render(t1, [cam], 640, 480, quality=2 | 8, texture=tex)
+    torch.cuda.synchronize()
+    assert torch.equal(ref["voxel"], out["voxel"].cpu())
+    assert torch.equal(ref["colors"], out["squares"].cpu())
+    m = ref["voxel"] != -1
+    assert torch.equal(ref["hit"].view(3, -1)[:, m], out["hit"].cpu()[:, m])
+    T = oracle.Tree(c1["dots"])
+    assert np.array_equal(refb.numpy(), T.balls(bc, br))
+    # the same requests inside a mix submitted from three threads
+    rng = np.random.default_rng(7)
+    jobs = []
+    for k in range(24):
+        if k % 3 == 0:
+            W, H = [(320, 240), (640, 480), (1280, 720)][k % 3 == 0 and
+                                                        rng.integers(0, 3)]
+            jobs.append(("r", i2, _cam_for(dots2, W, H, k), W, H,
+                         int(rng.choice([0, 1, 2, 6, 10]))))
+        elif k % 3 == 1:
+            c, r = scenes.random_balls(int(rng.integers(1000, 200000)),
+                                       *scenes.bbox_of(dots2), seed=k)
+            jobs.append(("b", i2, c, r))
+        else:
+            jobs.append(("same",))
+    results = {}
+    lock = threading.Lock()
+
+    def worker(part):
+        for k in part:
+            j = jobs[k]
+            if j[0] == "r":
+                tk_, o = S.submit_render(j[1], j[2], j[3], j[4], j[5])
+            elif j[0] == "b":
+                tk_, o = S.submit_balls(j[1], j[2], j[3])
+            else:
+                tk_, o = S.submit_render(i1, cam, 640, 480, quality=2 | 8,
+                                         colors=True)
+                tb_, ob = S.submit_balls(i1, bc, br)
+                with lock:
+                    results[(k, "b")] = (tb_, ob)
+            with lock:
+                results[k] = (tk_, o)
+
+    th = [threading.Thread(target=worker, args=(list(range(p, 24, 3)),))
+          for p in range(3)]
+    for t in th:
+        t.start()
+    for t in th:
+        t.join()
+    sizes = []
+    for key, (tk_, o) in results.items():
+        info = S.wait(tk_, 60.0)
+        assert info["status"] == 0
+        sizes.append(info["group_size"])
+        if key in [k for k in range(24) if jobs[k][0] == "same"]:
+            assert torch.equal(o["voxel"], ref["voxel"])
+            assert torch.equal(o["colors"], ref["colors"])
+            assert torch.equal(o["hit"].view(3, -1)[:, m],
+                               ref["hit"].view(3, -1)[:, m])
+        if isinstance(key, tuple):
+            assert torch.equal(o, refb)
+    st = S.stats()
+    assert st["completed"] >= 25 and max(sizes) > 1  # requests were batched
+    S.close()
+
+
+def test_admission_by_memory_and_slots(gpu):
+    from voxvision_amd import server
+    c1, t1, _, _ = _setup()
+    cam = c1["camera"]
+    one = 640 * 480 * 16 + 256
+    S = server.Server(hbm_budget=((one + 4095) // 4096) * 4096 + 8192,
+                      streams=2)
+    i1 = S.add_tree(t1)
+    tks = [S.submit_render(i1, cam, 640, 480)[0] for _ in range(3)]
+    infos = [S.wait(t, 60.0) for t in tks]
+    for a, b in zip(infos, infos[1:]):
+        assert b["t_admit"] >= a["t_done"] - 1e-6  # one at a time by memory
+    assert all(i["group_size"] == 1 for i in infos)
+    with pytest.raises(Exception):
+        S.submit_render(i1, cam, 1280, 720)  # can never fit: rejected
+    assert S.stats()["rejected"] == 1
+    S.close()
+    S = server.Server(max_inflight=2, streams=2)
+    i1 = S.add_tree(t1)
+    tks = [S.submit_render(i1, cam, 320, 240)[0] for _ in range(9)]
+    sizes = [S.wait(t, 60.0)["group_size"] for t in tks]
+    assert max(sizes) <= 2
+    assert S.stats()["peak_inflight"] <= 2
+    S.close()

@@ -25,7 +25,10 @@ EXPORTS = [
     "voxg_camera_set_window_size", "voxg_render", "voxg_ray_batch_count",
     "voxg_ball_batch_count", "voxg_render_count", "voxg_set_launch_config",
     "voxg_get_launch_config", "voxg_last_error", "voxg_batch_create",
-    "voxg_batch_ray", "voxg_batch_release",
+    "voxg_batch_ray", "voxg_batch_release", "voxg_server_create",
+    "voxg_server_add_tree", "voxg_server_submit_render",
+    "voxg_server_submit_balls", "voxg_server_wait", "voxg_server_get_stats",
+    "voxg_server_now", "voxg_server_destroy",
 ]
 
 

@@ -28,10 +28,15 @@
 #include <cmath>
 #include <cstdio>
 #include <cstring>
+#include <chrono>
+#include <condition_variable>
+#include <deque>
 #include <map>
 #include <memory>
 #include <mutex>
 #include <string>
+#include <thread>
+#include <vector>
 
 #include "tree_build.h"
 #include "voxg_layout.h"
@@ -457,17 +462,29 @@ __device__ __forceinline__ float sqr_metric(f3 a, f3 b) {
     return (x + y) + z;
 }
 
-// One 4x4 block per unit in FAST or ADAPTIVE mode (renderer.c:214-318):
-// the leaf found for one pixel is tried first for the next one, so the
-// pixels of a block are traced in order by one lane.
-struct BlockWork {
+// A render job: one tree, a run of camera poses of one size and quality,
+// its output buffers.  Units are 4x4 blocks, `bpf` per frame.
+struct RenderItem {
+    Tree t;
     const CamDev *cams;
-    uint32_t W, ws, bpf;     // width, blocks per row, blocks per frame
+    uint32_t W, ws, bpf, unit_begin;
     int rnd_mode, merge_mode;
     PixSink sink;
+};
+
+// One 4x4 block per unit (renderer.c:214-318): in FAST / ADAPTIVE mode the
+// leaf found for one pixel is tried first for the next, so the pixels of a
+// block are traced in order by one lane; in BEST mode every pixel is a
+// root search.  MULTI: units of several render jobs (different trees,
+// cameras, sizes, modes) in one launch.
+template <bool MULTI>
+struct BlockWork {
+    RenderItem one;             // the job when !MULTI
+    const RenderItem *items;    // the jobs when MULTI, by unit_begin
+    uint32_t n_items;
     enum { CORNER1 = 0, CORNER2 = 1, LOOP = 2 };
     struct State {
-        uint32_t blk, base;  // global block, output index of its pixel 0
+        uint32_t item, blk, base;  // job, block in the job, output index
         int i, iend, stage, mode, merge, leaf_try, p;
         uint32_t leaf, vid_last;
         f3 inter_last, c1_inter, c1_dir;
@@ -475,12 +492,17 @@ struct BlockWork {
         uint32_t fast, best, merged;
     };
 
+    __device__ __forceinline__ const RenderItem &item(const State &st) const {
+        if constexpr (MULTI) return items[st.item];
+        else return one;
+    }
     __device__ __forceinline__ const CamDev &cam(const State &st) const {
-        return cams[st.blk / bpf];
+        const RenderItem &it = item(st);
+        return it.cams[st.blk / it.bpf];
     }
     __device__ __forceinline__ void emit(const State &st, int p, bool hit,
                                          uint32_t vid, f3 e) const {
-        sink.put(st.base + (uint32_t)p, hit, vid, 0, e);
+        item(st).sink.put(st.base + (uint32_t)p, hit, vid, 0, e);
     }
     __device__ __forceinline__ void trace_pixel(const State &st, int p,
                                                 uint32_t start, uint32_t &cur,
@@ -514,12 +536,24 @@ struct BlockWork {
     }
     __device__ __forceinline__ void begin(uint32_t unit, State &st,
                                           uint32_t &cur, f3 &org, f3 &d) const {
-        uint32_t cs = unit % bpf;
-        st.blk = unit;
-        st.base = (unit / bpf) * bpf * 16u + cs * 16u;
-        st.xs = (int)(cs % ws) * 4;
-        st.ys = (int)(cs / ws) * 4;
-        st.mode = rnd_mode;
+        st.item = 0;
+        if constexpr (MULTI) {
+            uint32_t lo = 0, hi = n_items - 1;
+            while (lo < hi) {
+                uint32_t mid = (lo + hi + 1) >> 1;
+                if (items[mid].unit_begin <= unit) lo = mid;
+                else hi = mid - 1;
+            }
+            st.item = lo;
+        }
+        const RenderItem &it = item(st);
+        uint32_t u = unit - it.unit_begin;
+        uint32_t cs = u % it.bpf;
+        st.blk = u;
+        st.base = (u / it.bpf) * it.bpf * 16u + cs * 16u;
+        st.xs = (int)(cs % it.ws) * 4;
+        st.ys = (int)(cs / it.ws) * 4;
+        st.mode = it.rnd_mode;
         st.merge = 0;
         st.leaf = kEnd;
         st.vid_last = kEnd;
@@ -528,7 +562,7 @@ struct BlockWork {
         st.iend = 16;
         if (st.mode == VOXG_QUALITY_ADAPTIVE) {
             // renderer.c:240-253
-            st.merge = (merge_mode == VOXG_QUALITY_RAY_MERGE);
+            st.merge = (it.merge_mode == VOXG_QUALITY_RAY_MERGE);
             st.i = 1;
             st.stage = CORNER1;
             st.leaf_try = 0;
@@ -575,7 +609,8 @@ struct BlockWork {
                 if ((double)(dist / criteria) > 1.69) st.mode = VOXG_QUALITY_BEST;
                 if (d1 > 22500.0f) {
                     st.merge = (st.mode == VOXG_QUALITY_FAST &&
-                                merge_mode == VOXG_QUALITY_RAY_MERGE_ACCURATE)
+                                item(st).merge_mode ==
+                                    VOXG_QUALITY_RAY_MERGE_ACCURATE)
                                    ? 1
                                    : st.merge;
                 } else {
@@ -602,9 +637,10 @@ struct BlockWork {
         st.i++;
         return next(st, cur, org, d);
     }
-    __device__ __forceinline__ const Tree &tree(const State &,
+    __device__ __forceinline__ const Tree &tree(const State &st,
                                                 const Tree &t) const {
-        return t;
+        if constexpr (MULTI) return items[st.item].t;
+        else return t;
     }
     __device__ __forceinline__ void flush(const State &st, Ctrl *ctrl) const {
         atomicAdd(&ctrl->fast_blocks, (unsigned long long)st.fast);
@@ -865,17 +901,35 @@ rti_kernel(Tree t, Work work, uint32_t n, Ctrl *ctrl,
 // ball kernel (search.c:155-183): any-hit depth-first walk over children
 // 0..7 with escape links; the answer does not depend on the order.
 // ---------------------------------------------------------------------------
-template <bool COUNT>
+struct BallItem {
+    Tree t;
+    const float *cx, *cy, *cz, *r;
+    uint8_t *out;
+    uint32_t begin, count;  // queries [begin, begin+count) of the launch
+};
+
+template <bool COUNT, bool MULTI>
 __global__ void __launch_bounds__(256)
-ball_kernel(Tree t, uint32_t n, const float *__restrict__ cx,
-            const float *__restrict__ cy, const float *__restrict__ cz,
-            const float *__restrict__ rad, uint8_t *__restrict__ out,
-            Ctrl *ctrl) {
+ball_kernel(BallItem one, const BallItem *__restrict__ items,
+            uint32_t n_items, uint32_t n, Ctrl *ctrl) {
     uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
     unsigned long long nv = 0, nt = 0;
     if (i < n) {
-        f3 c = f3{cx[i], cy[i], cz[i]};
-        float r = rad[i];
+        const BallItem *itp = &one;
+        if constexpr (MULTI) {
+            uint32_t lo = 0, hi = n_items - 1;
+            while (lo < hi) {
+                uint32_t mid = (lo + hi + 1) >> 1;
+                if (items[mid].begin <= i) lo = mid;
+                else hi = mid - 1;
+            }
+            itp = &items[lo];
+        }
+        const BallItem &it = *itp;
+        const Tree &t = it.t;
+        uint32_t k0 = i - it.begin;
+        f3 c = f3{it.cx[k0], it.cy[k0], it.cz[k0]};
+        float r = it.r[k0];
         float rr = r * r;
         const f3 V = f3{t.vx, t.vy, t.vz};
         uint8_t res = 0;
@@ -910,7 +964,7 @@ ball_kernel(Tree t, uint32_t n, const float *__restrict__ cx,
                 node = nd.escape;
             }
         }
-        if (out) out[i] = res;
+        if (it.out) it.out[k0] = res;
     }
     if (COUNT) {
         atomicAdd(&ctrl->visits, nv);
@@ -1377,12 +1431,22 @@ static int ball_launch(const voxg_tree *t, size_t n, const float *cx,
     if (rc) return rc;
     int block = 256;
     int grid = (int)((n + block - 1) / block);
+    dev::BallItem one;
+    std::memset(&one, 0, sizeof(one));
+    one.t = dev_tree(t);
+    one.cx = cx;
+    one.cy = cy;
+    one.cz = cz;
+    one.r = r;
+    one.out = out;
+    one.begin = 0;
+    one.count = (uint32_t)n;
     if (counts) {
         HIPCHK(hipMemsetAsync(&ws->ctrl->visits, 0,
                               8 * sizeof(unsigned long long), stream));
-        hipLaunchKernelGGL(dev::ball_kernel<true>, dim3(grid), dim3(block), 0,
-                           stream, dev_tree(t), (uint32_t)n, cx, cy, cz, r,
-                           out, ws->ctrl);
+        hipLaunchKernelGGL((dev::ball_kernel<true, false>), dim3(grid),
+                           dim3(block), 0, stream, one, nullptr, 0u,
+                           (uint32_t)n, ws->ctrl);
         HIPCHK(hipGetLastError());
         dev::Ctrl h;
         HIPCHK(hipMemcpyAsync(&h, ws->ctrl, sizeof(h), hipMemcpyDeviceToHost,
@@ -1393,8 +1457,8 @@ static int ball_launch(const voxg_tree *t, size_t n, const float *cx,
         counts->rays = h.rays;
         return 0;
     }
-    hipLaunchKernelGGL(dev::ball_kernel<false>, dim3(grid), dim3(block), 0,
-                       stream, dev_tree(t), (uint32_t)n, cx, cy, cz, r, out,
+    hipLaunchKernelGGL((dev::ball_kernel<false, false>), dim3(grid),
+                       dim3(block), 0, stream, one, nullptr, 0u, (uint32_t)n,
                        ws->ctrl);
     HIPCHK(hipGetLastError());
     return 0;
@@ -1474,16 +1538,19 @@ static int render_launch(const voxg_tree *t, size_t n_frames,
     // one 4x4 block per lane (renderer.c:214): in FAST / ADAPTIVE mode the
     // leaf of one pixel is tried first for the next; in BEST mode every
     // pixel is a root search, traced one after the other by the lane
-    dev::BlockWork work;
+    dev::BlockWork<false> work;
     std::memset(&work, 0, sizeof(work));
-    work.cams = cd;
-    work.W = (uint32_t)W;
-    work.ws = (uint32_t)W / 4u;
-    work.bpf = (uint32_t)(W / 4) * (uint32_t)(H / 4);
-    work.rnd_mode = (int)(q & 3);
-    work.merge_mode = (int)(q & 0xc);
-    work.sink = sink;
-    return launch<dev::BlockWork, COUNT>(t, work, npix / 16, stream, counts);
+    work.one.t = dev_tree(t);
+    work.one.cams = cd;
+    work.one.W = (uint32_t)W;
+    work.one.ws = (uint32_t)W / 4u;
+    work.one.bpf = (uint32_t)(W / 4) * (uint32_t)(H / 4);
+    work.one.unit_begin = 0;
+    work.one.rnd_mode = (int)(q & 3);
+    work.one.merge_mode = (int)(q & 0xc);
+    work.one.sink = sink;
+    return launch<dev::BlockWork<false>, COUNT>(t, work, npix / 16, stream,
+                                                counts);
 }
 
 }  // extern "C++"
@@ -1599,6 +1666,562 @@ int voxg_batch_ray(const voxg_batch *b, const float *ox, const float *oy,
     std::memset(&none, 0, sizeof(none));
     return launch_tree<decltype(work), false>(none, b->stack_bound, work,
                                               b->units, stream, nullptr);
+}
+
+}  // extern "C"
+
+// ---------------------------------------------------------------------------
+// serving loop
+// ---------------------------------------------------------------------------
+namespace {
+
+using Clock = std::chrono::steady_clock;
+
+// First-fit allocator over the server's device arena (offsets, 256-byte
+// granules).  Admission is by free arena bytes.
+class Arena {
+  public:
+    void init(size_t size) {
+        size_ = size;
+        free_.clear();
+        if (size) free_[0] = size;
+        used_ = 0;
+    }
+    bool alloc(size_t bytes, size_t *off) {
+        bytes = (bytes + 255) & ~(size_t)255;
+        for (auto it = free_.begin(); it != free_.end(); ++it) {
+            if (it->second >= bytes) {
+                *off = it->first;
+                size_t rest = it->second - bytes;
+                size_t start = it->first + bytes;
+                free_.erase(it);
+                if (rest) free_[start] = rest;
+                used_ += bytes;
+                return true;
+            }
+        }
+        return false;
+    }
+    void release(size_t off, size_t bytes) {
+        bytes = (bytes + 255) & ~(size_t)255;
+        used_ -= bytes;
+        auto it = free_.emplace(off, bytes).first;
+        auto nx = std::next(it);
+        if (nx != free_.end() && it->first + it->second == nx->first) {
+            it->second += nx->second;
+            free_.erase(nx);
+        }
+        if (it != free_.begin()) {
+            auto pv = std::prev(it);
+            if (pv->first + pv->second == it->first) {
+                pv->second += it->second;
+                free_.erase(it);
+            }
+        }
+    }
+    size_t used() const { return used_; }
+    size_t size() const { return size_; }
+
+  private:
+    size_t size_ = 0, used_ = 0;
+    std::map<size_t, size_t> free_;
+};
+
+struct SrvRequest {
+    uint64_t id = 0;
+    int kind = 0;  // 0 render, 1 balls
+    int tree = 0;
+    voxg_render_request rr;
+    size_t n = 0;
+    std::vector<float> balls;  // cx | cy | cz | r
+    uint8_t *ball_out = nullptr;
+    size_t bytes = 0, off = 0;
+    double t_submit = 0, t_admit = 0, t_done = 0;
+    uint64_t group = 0;
+    uint32_t group_size = 0;
+    int status = 1;  // 1 pending, 0 ok, < 0 error
+};
+
+struct SrvGroup {
+    uint64_t id = 0;
+    int stream = 0;
+    hipEvent_t done = nullptr;
+    std::vector<SrvRequest *> reqs;
+};
+
+struct SrvStream {
+    hipStream_t s = nullptr;
+    void *h_table = nullptr;  // pinned item tables of the last group
+    void *d_table = nullptr;
+    hipEvent_t table_free = nullptr;  // the last group's table upload is done
+    bool table_pending = false;
+};
+
+constexpr size_t kTableBytes = 1u << 20;  // item tables of one group
+
+}  // namespace
+
+struct voxg_server {
+    int device = 0;
+    voxg_server_config cfg;
+    Clock::time_point t0;
+    std::vector<dev::Tree> trees;
+    uint32_t stack_bound = 1;
+    void *d_arena = nullptr;
+    uint8_t *d_texture = nullptr;
+    Arena arena;
+    std::vector<SrvStream> streams;
+    int next_stream = 0;
+    std::mutex mu;
+    std::condition_variable cv;     // wakes the scheduler
+    std::condition_variable done_cv;  // wakes waiters
+    std::deque<SrvRequest *> pending;
+    std::map<uint64_t, SrvRequest *> reqs;
+    std::vector<SrvGroup> groups;
+    uint64_t next_id = 1, next_group = 1;
+    uint32_t inflight = 0;
+    bool stop = false;
+    std::thread thr;
+    voxg_server_stats st;
+    std::string thread_error;
+
+    double now() const {
+        return std::chrono::duration<double>(Clock::now() - t0).count();
+    }
+};
+
+namespace {
+
+// Device bytes of a render request: voxel ids, then hit points and
+// colours when they are asked for (fixed offsets 4*np and 16*np).
+size_t render_bytes(const voxg_render_request &r, bool color) {
+    size_t np = (size_t)r.W * (size_t)r.H;
+    if (color && r.color_out) return np * 20 + 256;
+    if (r.hit_out) return np * 16 + 256;
+    return np * 4 + 256;
+}
+
+// Launch one group on its stream (scheduler thread, no lock held).
+int srv_launch(voxg_server *s, SrvGroup &g) {
+    SrvStream &ss = s->streams[g.stream];
+    if (ss.table_pending) {
+        HIPCHK(hipEventSynchronize(ss.table_free));
+        ss.table_pending = false;
+    }
+    char *arena = (char *)s->d_arena;
+    std::vector<dev::RenderItem> ritems;
+    std::vector<dev::BallItem> bitems;
+    std::vector<dev::CamDev> cams;
+    uint32_t units = 0, balls = 0;
+    bool color = s->d_texture != nullptr;
+    for (SrvRequest *r : g.reqs) {
+        if (r->kind == 0) {
+            size_t np = (size_t)r->rr.W * r->rr.H;
+            char *base = arena + r->off;
+            dev::RenderItem it;
+            std::memset(&it, 0, sizeof(it));
+            it.t = s->trees[r->tree];
+            it.cams = nullptr;  // set after the table layout is known
+            it.W = (uint32_t)r->rr.W;
+            it.ws = (uint32_t)r->rr.W / 4u;
+            it.bpf = (uint32_t)(np / 16);
+            it.unit_begin = units;
+            it.rnd_mode = (int)(r->rr.quality & 3);
+            it.merge_mode = (int)(r->rr.quality & 0xc);
+            bool want_color = color && r->rr.color_out;
+            it.sink.voxel = (uint32_t *)base;
+            it.sink.hit = r->rr.hit_out ? (float *)(base + np * 4) : nullptr;
+            it.sink.plane = np;
+            it.sink.squares =
+                want_color ? (uint32_t *)(base + np * 16) : nullptr;
+            if (want_color)  // pixels that miss stay 0, as after vox_redraw
+                HIPCHK(hipMemsetAsync(base + np * 16, 0, np * 4, ss.s));
+            it.sink.sh.tex = s->d_texture;
+            it.sink.sh.rshift = s->cfg.rshift;
+            it.sink.sh.gshift = s->cfg.gshift;
+            it.sink.sh.bshift = s->cfg.bshift;
+            it.sink.sh.amask = s->cfg.amask;
+            it.sink.sh.vx = it.t.vx;
+            it.sink.sh.vy = it.t.vy;
+            it.sink.sh.vz = it.t.vz;
+            units += it.bpf;
+            ritems.push_back(it);
+            dev::CamDev c;
+            std::memcpy(&c, &r->rr.cam, sizeof(c));
+            cams.push_back(c);
+        } else {
+            char *base = arena + r->off;
+            size_t n = r->n;
+            dev::BallItem it;
+            std::memset(&it, 0, sizeof(it));
+            it.t = s->trees[r->tree];
+            it.cx = (const float *)base;
+            it.cy = it.cx + n;
+            it.cz = it.cy + n;
+            it.r = it.cz + n;
+            it.out = (uint8_t *)(base + 16 * n);
+            it.begin = balls;
+            it.count = (uint32_t)n;
+            balls += (uint32_t)n;
+            bitems.push_back(it);
+            HIPCHK(hipMemcpyAsync(base, r->balls.data(), 16 * n,
+                                  hipMemcpyHostToDevice, ss.s));
+        }
+    }
+    // table layout: [render items][ball items][cameras]
+    size_t rb = ritems.size() * sizeof(dev::RenderItem);
+    size_t bb = bitems.size() * sizeof(dev::BallItem);
+    size_t cb = cams.size() * sizeof(dev::CamDev);
+    size_t o_b = (rb + 255) & ~(size_t)255;
+    size_t o_c = (o_b + bb + 255) & ~(size_t)255;
+    if (o_c + cb > kTableBytes) return fail(VOXG_ERANGE, "group too large");
+    char *dt = (char *)ss.d_table;
+    for (size_t i = 0; i < ritems.size(); i++)
+        ritems[i].cams = (const dev::CamDev *)(dt + o_c) + i;
+    char *ht = (char *)ss.h_table;
+    if (rb) std::memcpy(ht, ritems.data(), rb);
+    if (bb) std::memcpy(ht + o_b, bitems.data(), bb);
+    if (cb) std::memcpy(ht + o_c, cams.data(), cb);
+    HIPCHK(hipMemcpyAsync(dt, ht, o_c + cb, hipMemcpyHostToDevice, ss.s));
+    HIPCHK(hipEventRecord(ss.table_free, ss.s));
+    ss.table_pending = true;
+    if (units) {
+        dev::BlockWork<true> work;
+        std::memset(&work, 0, sizeof(work));
+        work.items = (const dev::RenderItem *)dt;
+        work.n_items = (uint32_t)ritems.size();
+        dev::Tree none;
+        std::memset(&none, 0, sizeof(none));
+        int rc = launch_tree<dev::BlockWork<true>, false>(
+            none, s->stack_bound, work, units, ss.s, nullptr);
+        if (rc) return rc;
+    }
+    if (balls) {
+        dev::BallItem one;
+        std::memset(&one, 0, sizeof(one));
+        Workspace *ws = nullptr;
+        int rc = get_ws(ss.s, 16, &ws);
+        if (rc) return rc;
+        hipLaunchKernelGGL((dev::ball_kernel<false, true>),
+                           dim3((balls + 255) / 256), dim3(256), 0, ss.s, one,
+                           (const dev::BallItem *)(dt + o_b),
+                           (uint32_t)bitems.size(), balls, ws->ctrl);
+        HIPCHK(hipGetLastError());
+    }
+    // results back to the callers' buffers
+    for (SrvRequest *r : g.reqs) {
+        char *base = arena + r->off;
+        if (r->kind == 0) {
+            size_t np = (size_t)r->rr.W * r->rr.H;
+            if (r->rr.voxel_out)
+                HIPCHK(hipMemcpyAsync(r->rr.voxel_out, base, np * 4,
+                                      hipMemcpyDeviceToHost, ss.s));
+            if (r->rr.hit_out)
+                HIPCHK(hipMemcpyAsync(r->rr.hit_out, base + np * 4, np * 12,
+                                      hipMemcpyDeviceToHost, ss.s));
+            if (r->rr.color_out && color)
+                HIPCHK(hipMemcpyAsync(r->rr.color_out, base + np * 16, np * 4,
+                                      hipMemcpyDeviceToHost, ss.s));
+        } else {
+            HIPCHK(hipMemcpyAsync(r->ball_out, base + 16 * r->n, r->n,
+                                  hipMemcpyDeviceToHost, ss.s));
+        }
+    }
+    HIPCHK(hipEventRecord(g.done, ss.s));
+    return 0;
+}
+
+void srv_loop(voxg_server *s) {
+    (void)hipSetDevice(s->device);
+    std::unique_lock<std::mutex> lk(s->mu);
+    for (;;) {
+        // retire finished groups
+        for (size_t i = 0; i < s->groups.size();) {
+            SrvGroup &g = s->groups[i];
+            if (hipEventQuery(g.done) == hipSuccess) {
+                double t = s->now();
+                for (SrvRequest *r : g.reqs) {
+                    r->t_done = t;
+                    r->status = 0;
+                    s->arena.release(r->off, r->bytes);
+                    s->inflight--;
+                    s->st.completed++;
+                }
+                (void)hipEventDestroy(g.done);
+                s->groups.erase(s->groups.begin() + i);
+                s->done_cv.notify_all();
+            } else {
+                i++;
+            }
+        }
+        if (s->stop && s->pending.empty() && s->groups.empty()) break;
+        // admit in arrival order while slots and memory allow
+        SrvGroup g;
+        while (!s->pending.empty() &&
+               s->inflight < (uint32_t)s->cfg.max_inflight) {
+            SrvRequest *r = s->pending.front();
+            size_t off;
+            if (!s->arena.alloc(r->bytes, &off)) break;
+            s->pending.pop_front();
+            r->off = off;
+            r->t_admit = s->now();
+            s->inflight++;
+            g.reqs.push_back(r);
+        }
+        if (!g.reqs.empty()) {
+            g.id = s->next_group++;
+            g.stream = s->next_stream;
+            s->next_stream = (s->next_stream + 1) % (int)s->streams.size();
+            for (SrvRequest *r : g.reqs) {
+                r->group = g.id;
+                r->group_size = (uint32_t)g.reqs.size();
+            }
+            s->st.groups++;
+            s->st.peak_inflight =
+                std::max<uint64_t>(s->st.peak_inflight, s->inflight);
+            s->st.peak_bytes =
+                std::max<uint64_t>(s->st.peak_bytes, s->arena.used());
+            lk.unlock();
+            int rc = (hipEventCreateWithFlags(&g.done,
+                                              hipEventDisableTiming) ==
+                      hipSuccess)
+                         ? srv_launch(s, g)
+                         : VOXG_EIO;
+            lk.lock();
+            if (rc) {
+                s->thread_error = g_err;
+                for (SrvRequest *r : g.reqs) {
+                    r->status = rc;
+                    r->t_done = s->now();
+                    s->arena.release(r->off, r->bytes);
+                    s->inflight--;
+                }
+                if (g.done) (void)hipEventDestroy(g.done);
+                s->done_cv.notify_all();
+            } else {
+                s->groups.push_back(std::move(g));
+                s->st.peak_groups_inflight = std::max<uint64_t>(
+                    s->st.peak_groups_inflight, s->groups.size());
+            }
+            continue;
+        }
+        if (s->groups.empty()) {
+            s->cv.wait(lk, [s] { return s->stop || !s->pending.empty(); });
+        } else {
+            s->cv.wait_for(lk, std::chrono::microseconds(20));
+        }
+    }
+}
+
+}  // namespace
+
+extern "C" {
+
+double voxg_server_now(const voxg_server *s) { return s ? s->now() : 0.0; }
+
+int voxg_server_create(const voxg_server_config *cfg, voxg_server **out) {
+    if (!out) return fail(VOXG_EINVAL, "out is NULL");
+    *out = nullptr;
+    std::unique_ptr<voxg_server> s(new (std::nothrow) voxg_server());
+    if (!s) return fail(VOXG_ENOMEM, "host allocation failed");
+    std::memset(&s->cfg, 0, sizeof(s->cfg));
+    if (cfg) s->cfg = *cfg;
+    if (s->cfg.max_inflight <= 0) s->cfg.max_inflight = 256;
+    if (s->cfg.streams <= 0) s->cfg.streams = 4;
+    if (s->cfg.streams > 16) return fail(VOXG_EINVAL, "at most 16 streams");
+    std::memset(&s->st, 0, sizeof(s->st));
+    HIPCHK(hipGetDevice(&s->device));
+    size_t fr = 0, tot = 0;
+    HIPCHK(hipMemGetInfo(&fr, &tot));
+    size_t budget = s->cfg.hbm_budget ? s->cfg.hbm_budget : fr / 2;
+    if (budget > fr) return fail(VOXG_ENOMEM, "budget exceeds free HBM");
+    budget &= ~(size_t)255;
+    if (budget < 4096) return fail(VOXG_EINVAL, "budget too small");
+    HIPCHK(hipMalloc(&s->d_arena, budget));
+    s->arena.init(budget);
+    s->st.budget = budget;
+    if (s->cfg.texture) {
+        HIPCHK(hipMalloc(&s->d_texture, 32 * 32 * 32));
+        HIPCHK(hipMemcpy(s->d_texture, s->cfg.texture, 32 * 32 * 32,
+                         hipMemcpyHostToDevice));
+    }
+    s->streams.resize(s->cfg.streams);
+    for (auto &ss : s->streams) {
+        HIPCHK(hipStreamCreateWithFlags(&ss.s, hipStreamNonBlocking));
+        HIPCHK(hipHostMalloc(&ss.h_table, kTableBytes, hipHostMallocDefault));
+        HIPCHK(hipMalloc(&ss.d_table, kTableBytes));
+        HIPCHK(hipEventCreateWithFlags(&ss.table_free, hipEventDisableTiming));
+        Workspace *ws = nullptr;
+        int rc = get_ws(ss.s, 16, &ws);
+        if (rc) return rc;
+    }
+    s->t0 = Clock::now();
+    voxg_server *raw = s.release();
+    raw->thr = std::thread(srv_loop, raw);
+    *out = raw;
+    return 0;
+}
+
+int voxg_server_add_tree(voxg_server *s, const voxg_tree *t, int *id) {
+    if (!s || !id) return fail(VOXG_EINVAL, "NULL argument");
+    int rc = check_tree(t);
+    if (rc) return rc;
+    // size every stream's spill buffer for the deepest tree
+    uint32_t bound = std::max(s->stack_bound, t->stack_bound);
+    voxg_launch_config c = current_config();
+    uint32_t slots = bound > (uint32_t)c.stack_lds ? bound - c.stack_lds : 1;
+    int cus = cu_count(s->device);
+    size_t threads = (size_t)cus * std::max(1, c.waves_per_cu) * 64;
+    for (auto &ss : s->streams) {
+        Workspace *ws = nullptr;
+        rc = get_ws(ss.s, (size_t)slots * threads * sizeof(uint4), &ws);
+        if (rc) return rc;
+    }
+    std::lock_guard<std::mutex> g(s->mu);
+    s->stack_bound = bound;
+    s->trees.push_back(dev_tree(t));
+    *id = (int)s->trees.size() - 1;
+    return 0;
+}
+
+static int srv_submit(voxg_server *s, SrvRequest *r, uint64_t *ticket) {
+    std::lock_guard<std::mutex> g(s->mu);
+    if (r->bytes > s->arena.size()) {
+        s->st.rejected++;
+        delete r;
+        return fail(VOXG_ENOMEM, "request larger than the server budget");
+    }
+    r->id = s->next_id++;
+    r->t_submit = s->now();
+    s->reqs[r->id] = r;
+    s->pending.push_back(r);
+    s->st.submitted++;
+    *ticket = r->id;
+    s->cv.notify_one();
+    return 0;
+}
+
+int voxg_server_submit_render(voxg_server *s, const voxg_render_request *rq,
+                              uint64_t *ticket) {
+    if (!s || !rq || !ticket) return fail(VOXG_EINVAL, "NULL argument");
+    if (rq->tree < 0 || rq->tree >= (int)s->trees.size())
+        return fail(VOXG_EINVAL, "unknown tree id");
+    if (rq->W <= 0 || rq->H <= 0 || (rq->W & 0xf) || (rq->H & 0x3))
+        return fail(VOXG_EINVAL,
+                    "W must be a positive multiple of 16 and H of 4");
+    unsigned q = rq->quality;
+    if ((q & 0xc) && (q & 3) != VOXG_QUALITY_ADAPTIVE)
+        return fail(VOXG_EINVAL, "ray merging needs the adaptive mode");
+    if ((q & 3) > 2 || (q & 0xc) > 8 || (q & ~0xfu))
+        return fail(VOXG_EINVAL, "reserved quality bits set");
+    if (rq->color_out && !s->d_texture)
+        return fail(VOXG_EINVAL, "colour output needs a server texture");
+    if ((size_t)rq->W * rq->H >= 0x40000000ull)
+        return fail(VOXG_ERANGE, "frame too large");
+    SrvRequest *r = new (std::nothrow) SrvRequest();
+    if (!r) return fail(VOXG_ENOMEM, "host allocation failed");
+    r->kind = 0;
+    r->tree = rq->tree;
+    r->rr = *rq;
+    r->bytes = render_bytes(*rq, s->d_texture != nullptr);
+    return srv_submit(s, r, ticket);
+}
+
+int voxg_server_submit_balls(voxg_server *s, const voxg_ball_request *rq,
+                             uint64_t *ticket) {
+    if (!s || !rq || !ticket) return fail(VOXG_EINVAL, "NULL argument");
+    if (rq->tree < 0 || rq->tree >= (int)s->trees.size())
+        return fail(VOXG_EINVAL, "unknown tree id");
+    if (rq->n == 0 || !rq->cx || !rq->cy || !rq->cz || !rq->r || !rq->out)
+        return fail(VOXG_EINVAL, "empty or NULL ball request");
+    if (rq->n >= 0x10000000ull) return fail(VOXG_ERANGE, "too many balls");
+    SrvRequest *r = new (std::nothrow) SrvRequest();
+    if (!r) return fail(VOXG_ENOMEM, "host allocation failed");
+    r->kind = 1;
+    r->tree = rq->tree;
+    r->n = rq->n;
+    try {
+        r->balls.resize(4 * rq->n);
+    } catch (const std::bad_alloc &) {
+        delete r;
+        return fail(VOXG_ENOMEM, "host allocation failed");
+    }
+    std::memcpy(r->balls.data(), rq->cx, rq->n * 4);
+    std::memcpy(r->balls.data() + rq->n, rq->cy, rq->n * 4);
+    std::memcpy(r->balls.data() + 2 * rq->n, rq->cz, rq->n * 4);
+    std::memcpy(r->balls.data() + 3 * rq->n, rq->r, rq->n * 4);
+    r->ball_out = rq->out;
+    r->bytes = 17 * rq->n + 256;
+    return srv_submit(s, r, ticket);
+}
+
+int voxg_server_wait(voxg_server *s, uint64_t ticket, double timeout_s,
+                     voxg_request_info *info) {
+    if (!s) return fail(VOXG_EINVAL, "server is NULL");
+    std::unique_lock<std::mutex> lk(s->mu);
+    auto it = s->reqs.find(ticket);
+    if (it == s->reqs.end()) return fail(VOXG_EINVAL, "unknown ticket");
+    SrvRequest *r = it->second;
+    auto ready = [r] { return r->status != 1; };
+    if (timeout_s > 0) {
+        if (!s->done_cv.wait_for(lk, std::chrono::duration<double>(timeout_s),
+                                 ready))
+            return fail(VOXG_EIO, "timed out");
+    } else {
+        s->done_cv.wait(lk, ready);
+    }
+    if (info) {
+        info->status = r->status;
+        info->t_submit = r->t_submit;
+        info->t_admit = r->t_admit;
+        info->t_done = r->t_done;
+        info->group = r->group;
+        info->group_size = r->group_size;
+    }
+    int st = r->status;
+    s->reqs.erase(it);
+    delete r;
+    if (st) return fail(st, s->thread_error);
+    return 0;
+}
+
+int voxg_server_get_stats(voxg_server *s, voxg_server_stats *out) {
+    if (!s || !out) return fail(VOXG_EINVAL, "NULL argument");
+    std::lock_guard<std::mutex> g(s->mu);
+    *out = s->st;
+    return 0;
+}
+
+int voxg_server_destroy(voxg_server *s) {
+    if (!s) return 0;
+    {
+        std::lock_guard<std::mutex> g(s->mu);
+        s->stop = true;
+        s->cv.notify_all();
+    }
+    if (s->thr.joinable()) s->thr.join();
+    for (auto &kv : s->reqs) delete kv.second;
+    for (auto &ss : s->streams) {
+        if (ss.s) (void)hipStreamSynchronize(ss.s);
+        if (ss.h_table) (void)hipHostFree(ss.h_table);
+        if (ss.d_table) (void)hipFree(ss.d_table);
+        if (ss.table_free) (void)hipEventDestroy(ss.table_free);
+        if (ss.s) {
+            std::lock_guard<std::mutex> g(g_mu);
+            auto it = g_ws.find({s->device, ss.s});
+            if (it != g_ws.end()) {
+                if (it->second->ctrl) (void)hipFree(it->second->ctrl);
+                if (it->second->spill) (void)hipFree(it->second->spill);
+                g_ws.erase(it);
+            }
+        }
+    }
+    if (s->d_arena) (void)hipFree(s->d_arena);
+    if (s->d_texture) (void)hipFree(s->d_texture);
+    for (auto &ss : s->streams)
+        if (ss.s) (void)hipStreamDestroy(ss.s);
+    delete s;
+    return 0;
 }
 
 }  // extern "C"
