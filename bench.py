@@ -133,6 +133,121 @@ def batched_leg(ev0, ev1, stream, log):
             "plain_config": pcfg, "batched_config": bcfg}
 
 
+def serving_leg(log, seconds, rate, seed=31):
+    """Serving: 8 resident 256^3 trees (CT-like, different seeds); a
+    seeded Poisson stream of requests, 60% renders (320x240 .. 3840x2160,
+    random resident tree, seeded pose, BEST/FAST/ADAPTIVE) and 40%
+    collision batches (1K..1M balls, radii U[0.5, 8]); latency is server
+    time from submission to results in host memory."""
+    import queue
+    import threading
+    import numpy as np
+    import torch
+    from voxvision_amd import camera, scenes, server, voxtrees as VT
+    t0 = time.perf_counter()
+    trees, boxes = [], []
+    for k in range(8):
+        dots = scenes.volume_to_dots(scenes.ct_volume(256, seed=100 + k))
+        trees.append(VT.make_tree(dots))
+        boxes.append(scenes.bbox_of(dots))
+    S = server.Server(streams=4)
+    ids = [S.add_tree(t) for t in trees]
+    rng = np.random.default_rng(seed)
+    sizes = [(320, 240), (640, 480), (1280, 720), (1920, 1080), (3840, 2160)]
+    wts = [0.3, 0.25, 0.2, 0.15, 0.1]
+    pools = {sz: queue.Queue() for sz in sizes}
+    for sz, cnt in zip(sizes, (48, 48, 48, 32, 24)):
+        for _ in range(cnt):
+            pools[sz].put({"voxel": torch.empty(sz[0] * sz[1],
+                                                dtype=torch.int32,
+                                                pin_memory=True)})
+    bc, br = scenes.random_balls(1 << 20, *boxes[0], seed=seed)
+    plan, t = [], 0.0
+    while True:
+        t += rng.exponential(1.0 / rate)
+        if t >= seconds:
+            break
+        k = int(rng.integers(0, 8))
+        if rng.random() < 0.6:
+            sz = sizes[int(rng.choice(len(sizes), p=wts))]
+            bmin, bmax = boxes[k]
+            c = (bmin + bmax) / 2
+            a, el = rng.uniform(0, 2 * np.pi), rng.uniform(-0.4, 0.4)
+            dist = 1.5 * float(np.linalg.norm(bmax - bmin))
+            pos = c + dist * np.array([np.cos(a) * np.cos(el),
+                                       np.sin(a) * np.cos(el), np.sin(el)])
+            cam = camera.make_camera(pos, c, np.tan(np.radians(30)), *sz)
+            q = int(rng.choice([0, 1, 2, 6, 10]))
+            plan.append((t, "r", ids[k], cam, sz, q))
+        else:
+            n = int(np.exp(rng.uniform(np.log(1e3), np.log(1e6))))
+            off = int(rng.integers(0, (1 << 20) - n))
+            plan.append((t, "b", ids[k], n, off))
+    log("[bench] serving leg: %d requests planned over %.1f s, setup %.1f s"
+        % (len(plan), seconds, time.perf_counter() - t0))
+    tickets = queue.Queue()
+
+    def collect(out):
+        while True:
+            item = tickets.get()
+            if item is None:
+                return
+            tk, kind, px, buf, sz = item
+            info = S.wait(tk, 120.0)
+            if buf is not None:
+                pools[sz].put(buf)
+            out.append((kind, px, info))
+
+    done = []
+    th = threading.Thread(target=collect, args=(done,))
+    th.start()
+    start = time.perf_counter()
+    for item in plan:
+        delay = item[0] - (time.perf_counter() - start)
+        if delay > 0:
+            time.sleep(delay)
+        if item[1] == "r":
+            _, _, tid, cam, sz, q = item
+            buf = pools[sz].get()
+            tk, _ = S.submit_render(tid, cam, sz[0], sz[1], q, out=buf)
+            tickets.put((tk, "r", sz[0] * sz[1], buf, sz))
+        else:
+            _, _, tid, n, off = item
+            tk, _ = S.submit_balls(tid, bc[off:off + n], br[off:off + n])
+            tickets.put((tk, "b", n, None, None))
+    tickets.put(None)
+    th.join()
+    st = S.stats()
+    S.close()
+    lat = np.array([i["t_done"] - i["t_submit"] for _, _, i in done]) * 1e3
+    lr = np.array([i["t_done"] - i["t_submit"] for k, _, i in done
+                   if k == "r"]) * 1e3
+    lb = np.array([i["t_done"] - i["t_submit"] for k, _, i in done
+                   if k == "b"]) * 1e3
+    first = min(i["t_submit"] for _, _, i in done)
+    last = max(i["t_done"] for _, _, i in done)
+    span = last - first
+    px = sum(p for k, p, _ in done if k == "r")
+    nb = sum(p for k, p, _ in done if k == "b")
+    sizes_g = [i["group_size"] for _, _, i in done]
+    res = {"requests": len(done), "offered_rate_per_s": rate,
+           "span_s": round(span, 3),
+           "p50_ms": round(float(np.percentile(lat, 50)), 3),
+           "p99_ms": round(float(np.percentile(lat, 99)), 3),
+           "render_p50_ms": round(float(np.percentile(lr, 50)), 3),
+           "render_p99_ms": round(float(np.percentile(lr, 99)), 3),
+           "balls_p50_ms": round(float(np.percentile(lb, 50)), 3),
+           "balls_p99_ms": round(float(np.percentile(lb, 99)), 3),
+           "sustained_mrays_s": round(px / span / 1e6, 1),
+           "sustained_mballs_s": round(nb / span / 1e6, 1),
+           "groups": st["groups"],
+           "mean_group_size": round(float(np.mean(sizes_g)), 2),
+           "peak_groups_inflight": st["peak_groups_inflight"],
+           "peak_inflight": st["peak_inflight"]}
+    log("[bench] serving leg: %s" % json.dumps(res))
+    return res
+
+
 def load_traffic(name):
     p = os.path.join(ROOT, "profiles", name)
     if not os.path.exists(p):
@@ -154,6 +269,11 @@ def main():
                     help="skip the incoherent and ball extras")
     ap.add_argument("--no-batched", action="store_true",
                     help="skip the 32-scene batched-vs-plain leg")
+    ap.add_argument("--no-serving", action="store_true",
+                    help="skip the serving (request stream) leg")
+    ap.add_argument("--serve-seconds", type=float, default=3.0)
+    ap.add_argument("--serve-rate", type=float, default=1000.0,
+                    help="Poisson arrival rate of the serving leg (req/s)")
     ap.add_argument("--no-tune", action="store_true",
                     help="use the library's default launch configuration")
     ap.add_argument("--copy-calib-mib", type=int, default=0,
@@ -289,6 +409,9 @@ def main():
 
     if not args.no_batched:
         extra["batched"] = batched_leg(ev0, ev1, stream, log)
+    if not args.no_serving:
+        extra["serving"] = serving_leg(log, args.serve_seconds,
+                                       args.serve_rate)
 
     cpu = None
     if not args.no_cpu:
