@@ -303,13 +303,16 @@ int voxg_server_destroy(voxg_server *s);
  * waves_per_cu: persistent waves launched per CU; refill: minimum idle
  * lanes before a wave fetches new work (1..64); chunk: consecutive rays a
  * lane traces before it needs new work in voxg_ray_batch (1..4096; a
- * render lane always takes one 4x4 block). */
+ * render lane always takes one 4x4 block); render_cull: 1 = voxg_render
+ * first finishes, in a coherent pre-pass, the 4x4 blocks whose rays all
+ * miss the scene box, 0 = every block goes through the traversal. */
 typedef struct {
     int stack_lds;
     int block;
     int waves_per_cu;
     int refill;
     int chunk;
+    int render_cull;
 } voxg_launch_config;
 int voxg_set_launch_config(const voxg_launch_config *cfg);
 int voxg_get_launch_config(voxg_launch_config *cfg);

@@ -313,3 +313,38 @@ def test_render_adaptive_ct_frames(gpu, oracle):
         m = vox_o != MISS
         assert np.array_equal(hg[:, i, m].view(np.uint32),
                               hit_o[:, m].view(np.uint32))
+
+
+@pytest.mark.parametrize("quality", [0, 1, 2, 6, 10])
+def test_render_with_and_without_block_culling(gpu, oracle, quality):
+    """The pre-pass that finishes blocks missing the scene box changes no
+    output and no counter, in every quality mode."""
+    from voxvision_amd import voxrnd, voxtrees as VT
+    c1 = _c1()
+    G = VT.make_tree(c1["dots"])
+    tex = np.random.default_rng(4).integers(0, 256, 32 ** 3).astype(np.uint8)
+    W, H = c1["W"], c1["H"]
+    old = VT.get_launch_config()
+    res = []
+    try:
+        for cull in (0, 1):
+            VT.set_launch_config(render_cull=cull)
+            out = voxrnd.render(G, [c1["camera"]], W, H, quality=quality,
+                                texture=tex)
+            cnt = voxrnd.render_counts(G, [c1["camera"]], W, H,
+                                       quality=quality)
+            torch.cuda.synchronize()
+            res.append((out, cnt))
+    finally:
+        VT.set_launch_config(**old)
+    (a, ca), (b, cb) = res
+    assert torch.equal(a["voxel"], b["voxel"])
+    assert torch.equal(a["squares"], b["squares"])
+    m = a["voxel"] != -1
+    assert torch.equal(a["hit"][:, m], b["hit"][:, m])
+    assert ca == cb
+    T = oracle.Tree(c1["dots"])
+    _, vox_o, _, st = T.render(c1["camera"], W, H, quality, texture=tex)
+    assert np.array_equal(_np_u32(b["voxel"]), vox_o)
+    assert cb["rays"] == st["root_traversals"]
+    assert cb["node_visits"] == st["node_visits"]

@@ -24,6 +24,7 @@ SPACE = {
     "waves_per_cu": [8, 12, 16, 20, 24, 32],
     "refill": [1, 4, 8, 16, 32],
     "chunk": [1, 2, 4, 8, 16, 32],
+    "render_cull": [0, 1],
 }
 
 

@@ -356,6 +356,8 @@ int flatten(const View &view, typename View::N root, const float V[3],
                 dense_dims(r.bmin, r.bmax, V, dim);
                 r.ref = (uint32_t)dense_next;
                 dense_next += (uint64_t)dim[0] * dim[1] * dim[2];
+                // the device reads the dimensions from the centre slot
+                std::memcpy(r.c, dim, sizeof(dim));
             } else {
                 uint32_t nk = (uint32_t)__builtin_popcount((r.info >> 8) & 0xff);
                 for (int k = (int)nk - 1; k >= 0; k--) st.push_back(r.ref + k);

@@ -28,7 +28,8 @@ struct alignas(16) NodeRec {
                     // dense: first voxel id
     uint32_t info;  // [1:0] type, [2] overflow; inner: [15:8] child mask;
                     // leaf: [31:8] dot count
-    float c[3];     // inner: centre of subdivision (tree.c:31-55)
+    float c[3];     // inner: centre of subdivision (tree.c:31-55);
+                    // dense: its dimensions in voxels (3 x uint32)
     uint32_t escape;  // next record in depth-first order after this subtree
 };
 static_assert(sizeof(NodeRec) == 48, "NodeRec must be 48 bytes");

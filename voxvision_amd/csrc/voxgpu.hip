@@ -166,7 +166,15 @@ struct Tree {
     const float *__restrict__ dots;
     float vx, vy, vz;
     uint32_t empty;
+    float ivx, ivy, ivz;  // 1/V when V is a power of two, else 0
+    uint32_t pad;
 };
+
+// x / v, as a multiplication when v is a power of two (iv = 1/v exactly):
+// both round the exact quotient once, so the bits are the same.
+__device__ __forceinline__ float div_v(float x, float v, float iv) {
+    return iv != 0.0f ? x * iv : x / v;
+}
 
 struct Node {
     f3 bmin, bmax, c;
@@ -189,17 +197,16 @@ __device__ __forceinline__ Node load_node(const NodeRec *__restrict__ nodes,
 
 // Voxel id of a dense-leaf hit: floor((res - min)/V) clamped to the leaf's
 // dimensions (get_dimensions, geom.c:192-196).
+// The dimensions are precomputed at flattening and stored in the record's
+// centre slot, which dense leaves do not use.
 __device__ __forceinline__ uint32_t dense_voxel(const Node &n, f3 r,
                                                 const Tree &t) {
-    float dxf = (n.bmax.x - n.bmin.x) / t.vx;
-    float dyf = (n.bmax.y - n.bmin.y) / t.vy;
-    float dzf = (n.bmax.z - n.bmin.z) / t.vz;
-    int dx = dxf > 0.0f ? (int)(uint32_t)dxf : 0;
-    int dy = dyf > 0.0f ? (int)(uint32_t)dyf : 0;
-    int dz = dzf > 0.0f ? (int)(uint32_t)dzf : 0;
-    int ix = (int)floorf((r.x - n.bmin.x) / t.vx);
-    int iy = (int)floorf((r.y - n.bmin.y) / t.vy);
-    int iz = (int)floorf((r.z - n.bmin.z) / t.vz);
+    int dx = (int)__float_as_uint(n.c.x);
+    int dy = (int)__float_as_uint(n.c.y);
+    int dz = (int)__float_as_uint(n.c.z);
+    int ix = (int)floorf(div_v(r.x - n.bmin.x, t.vx, t.ivx));
+    int iy = (int)floorf(div_v(r.y - n.bmin.y, t.vy, t.ivy));
+    int iz = (int)floorf(div_v(r.z - n.bmin.z, t.vz, t.ivz));
     ix = ix < 0 ? 0 : (ix > dx - 1 ? dx - 1 : ix);
     iy = iy < 0 ? 0 : (iy > dy - 1 ? dy - 1 : iy);
     iz = iz < 0 ? 0 : (iz > dz - 1 ? dz - 1 : iz);
@@ -290,11 +297,12 @@ struct Shade {
     const uint8_t *tex;
     uint32_t rshift, gshift, bshift, amask;
     float vx, vy, vz;
+    float ivx, ivy, ivz;
     // renderer.c:27-52 without a light manager
     __device__ __forceinline__ uint32_t color(f3 p) const {
-        int x = abs((int)(p.x / vx)) & 31;
-        int y = abs((int)(p.y / vy)) & 31;
-        int z = abs((int)(p.z / vz)) & 31;
+        int x = abs((int)div_v(p.x, vx, ivx)) & 31;
+        int y = abs((int)div_v(p.y, vy, ivy)) & 31;
+        int z = abs((int)div_v(p.z, vz, ivz)) & 31;
         uint32_t c = tex[x * 1024 + y * 32 + z];
         return (c << rshift) | (c << gshift) | (c << bshift) | amask;
     }
@@ -323,7 +331,7 @@ struct Ctrl {
     uint32_t next;     // next work unit to hand out
     uint32_t left;     // waves that have finished
     uint32_t overflow; // set when a stack bound was exceeded (never expected)
-    uint32_t pad;
+    uint32_t qcount;   // length of the block queue built by block_cull
     unsigned long long visits, tests, rays, leaf_trav;
     unsigned long long fast_blocks, best_blocks, merge_blocks, pad2;
 };
@@ -482,6 +490,7 @@ struct BlockWork {
     RenderItem one;             // the job when !MULTI
     const RenderItem *items;    // the jobs when MULTI, by unit_begin
     uint32_t n_items;
+    const uint32_t *queue;      // !MULTI: units are queue entries if set
     enum { CORNER1 = 0, CORNER2 = 1, LOOP = 2 };
     struct State {
         uint32_t item, blk, base;  // job, block in the job, output index
@@ -537,6 +546,9 @@ struct BlockWork {
     __device__ __forceinline__ void begin(uint32_t unit, State &st,
                                           uint32_t &cur, f3 &org, f3 &d) const {
         st.item = 0;
+        if constexpr (!MULTI) {
+            if (queue) unit = queue[unit];
+        }
         if constexpr (MULTI) {
             uint32_t lo = 0, hi = n_items - 1;
             while (lo < hi) {
@@ -654,9 +666,10 @@ struct BlockWork {
 // ---------------------------------------------------------------------------
 template <int K, class Work, bool COUNT>
 __global__ void __launch_bounds__(256)
-rti_kernel(Tree t, Work work, uint32_t n, Ctrl *ctrl,
+rti_kernel(Tree t, Work work, uint32_t n, const uint32_t *n_dev, Ctrl *ctrl,
            uint4 *__restrict__ spill, uint32_t spill_slots, uint32_t refill) {
     extern __shared__ uint4 ring[];  // [K][blockDim.x]
+    if (n_dev) n = *n_dev;  // unit count produced on the device
     const uint32_t tid = threadIdx.x;
     const uint32_t bdim = blockDim.x;
     const uint32_t gtid = blockIdx.x * bdim + tid;
@@ -892,8 +905,86 @@ rti_kernel(Tree t, Work work, uint32_t n, Ctrl *ctrl,
         uint32_t done = atomicAdd(&ctrl->left, 1u);
         if (done == total - 1) {
             atomicExch(&ctrl->next, 0u);
+            atomicExch(&ctrl->qcount, 0u);
             atomicExch(&ctrl->left, 0u);
         }
+    }
+}
+
+// Pre-pass of a render launch: one thread per pixel tests its camera ray
+// against the root box.  A block none of whose 16 rays enters the box is
+// finished here -- every pixel is a miss in every quality mode -- and the
+// others go, in block order, to the queue the traversal kernel takes its
+// blocks from.  A workgroup covers kCullTile pixels (256 blocks): it marks
+// the entering blocks in LDS, compacts them with a workgroup scan and
+// reserves its queue range with a single atomic.  In counting mode it adds
+// what the loop would have counted for each finished block: one root
+// search per traced pixel (ADAPTIVE with ray merging traces 8 of the 16
+// pixels of a block whose first corner misses), one node visit per search,
+// and the block's final mode.
+constexpr uint32_t kCullTile = 256 * 16;
+
+template <bool COUNT>
+__global__ void __launch_bounds__(256)
+block_cull_kernel(RenderItem it, uint32_t npix, Ctrl *ctrl,
+                  uint32_t *__restrict__ queue) {
+    __shared__ uint8_t enter[kCullTile / 16];
+    __shared__ uint32_t wave_sum[4];
+    __shared__ uint32_t gbase;
+    __shared__ unsigned long long missed;
+    const uint32_t tid = threadIdx.x, lane = lane_id(), w = tid >> 6;
+    if (tid == 0) missed = 0;
+    const uint32_t tile0 = blockIdx.x * kCullTile;
+    uint32_t my_missed = 0;
+    // the host launches this kernel only for a tree with a root record
+    const float4 *q = reinterpret_cast<const float4 *>(it.t.nodes);
+    const float4 a = q[0], b = q[1];
+    const f3 lo = f3{a.x, a.y, a.z}, hi = f3{a.w, b.x, b.y};
+    for (uint32_t k = 0; k < 16; ++k) {
+        const uint32_t r = tile0 + k * 256 + tid;
+        const bool valid = r < npix;
+        const uint32_t blk = r >> 4, p = r & 15u;
+        bool enters = false;
+        if (valid && !it.t.empty) {
+            const uint32_t cs = blk % it.bpf;
+            const CamDev &c = it.cams[blk / it.bpf];
+            const int x = (int)(cs % it.ws) * 4 + (int)(p & 3u);
+            const int y = (int)(cs / it.ws) * 4 + (int)(p >> 2);
+            const f3 d = screen2world(c, x, y);
+            f3 e;
+            enters = hit_box(lo, hi, f3{c.px, c.py, c.pz}, d, e);
+        }
+        const uint64_t m = __ballot(enters);
+        const bool block_enters = ((m >> (16 * (lane >> 4))) & 0xffffull) != 0;
+        if (valid && !block_enters) it.sink.put(r, false, kEnd, 0, f3{0, 0, 0});
+        if (p == 0) enter[k * 16 + (tid >> 4)] = (valid && block_enters) ? 1 : 0;
+        if (COUNT && valid && p == 0 && !block_enters) my_missed++;
+    }
+    __syncthreads();
+    // ordered compaction of the 256 block flags: thread t owns block t
+    const bool mine = enter[tid] != 0;
+    const uint64_t wm = __ballot(mine);
+    if (lane == 0) wave_sum[w] = (uint32_t)__popcll(wm);
+    if (COUNT && my_missed) atomicAdd(&missed, (unsigned long long)my_missed);
+    __syncthreads();
+    uint32_t off = 0, total = 0;
+    for (uint32_t i = 0; i < 4; ++i) {
+        off += (i < w) ? wave_sum[i] : 0u;
+        total += wave_sum[i];
+    }
+    if (tid == 0) gbase = total ? atomicAdd(&ctrl->qcount, total) : 0u;
+    __syncthreads();
+    if (mine) queue[gbase + off + lanes_below(wm)] = tile0 / 16 + tid;
+    if (COUNT && tid == 0 && missed) {
+        const unsigned long long nb = missed;
+        const bool halved = it.rnd_mode == VOXG_QUALITY_ADAPTIVE &&
+                            it.merge_mode == VOXG_QUALITY_RAY_MERGE;
+        const unsigned long long traced = nb * (halved ? 8ull : 16ull);
+        atomicAdd(&ctrl->visits, traced);
+        atomicAdd(&ctrl->rays, traced);
+        if (it.rnd_mode == VOXG_QUALITY_BEST) atomicAdd(&ctrl->best_blocks, nb);
+        else atomicAdd(&ctrl->fast_blocks, nb);
+        if (halved) atomicAdd(&ctrl->merge_blocks, nb);
     }
 }
 
@@ -998,7 +1089,7 @@ struct voxg_batch {
 namespace {
 
 std::mutex g_mu;
-voxg_launch_config g_cfg = {8, 256, 20, 8, 4};
+voxg_launch_config g_cfg = {8, 256, 20, 8, 4, 1};
 
 voxg_launch_config current_config() {
     std::lock_guard<std::mutex> g(g_mu);
@@ -1010,6 +1101,8 @@ struct Workspace {
     dev::Ctrl *ctrl = nullptr;
     uint4 *spill = nullptr;
     size_t spill_bytes = 0;
+    uint32_t *queue = nullptr;  // block queue of the render pre-pass
+    size_t queue_bytes = 0;
 };
 std::map<std::pair<int, hipStream_t>, std::unique_ptr<Workspace>> g_ws;
 
@@ -1047,13 +1140,37 @@ int get_ws(hipStream_t s, size_t spill_bytes, Workspace **out) {
     return 0;
 }
 
+int get_queue(Workspace *ws, size_t bytes) {
+    std::lock_guard<std::mutex> g(g_mu);
+    if (bytes > ws->queue_bytes) {
+        if (ws->queue) HIPCHK(hipFree(ws->queue));
+        ws->queue = nullptr;
+        ws->queue_bytes = 0;
+        HIPCHK(hipMalloc(&ws->queue, bytes));
+        ws->queue_bytes = bytes;
+    }
+    return 0;
+}
+
+// 1/v when v is a power of two (then x * (1/v) == x / v exactly), else 0.
+float pow2_inverse(float v) {
+    int e = 0;
+    float m = std::frexp(v, &e);
+    if (!(v > 0.0f) || m != 0.5f || e < -125 || e > 126) return 0.0f;
+    return std::ldexp(1.0f, 1 - e);
+}
+
 dev::Tree dev_tree(const voxg_tree *t) {
     dev::Tree d;
+    std::memset(&d, 0, sizeof(d));
     d.nodes = t->d_nodes;
     d.dots = t->d_dots;
     d.vx = t->host.voxel[0];
     d.vy = t->host.voxel[1];
     d.vz = t->host.voxel[2];
+    d.ivx = pow2_inverse(d.vx);
+    d.ivy = pow2_inverse(d.vy);
+    d.ivz = pow2_inverse(d.vz);
     d.empty = t->host.nodes.empty() ? 1u : 0u;
     return d;
 }
@@ -1061,7 +1178,7 @@ dev::Tree dev_tree(const voxg_tree *t) {
 template <int K, class Work, bool COUNT>
 int launch_k(const dev::Tree &dt, uint32_t stack_bound, const Work &work,
              size_t n, hipStream_t stream, const voxg_launch_config &cfg,
-             voxg_counts *counts) {
+             voxg_counts *counts, const uint32_t *n_dev, bool reset_counts) {
     int devid = 0;
     HIPCHK(hipGetDevice(&devid));
     int cus = cu_count(devid);
@@ -1069,7 +1186,8 @@ int launch_k(const dev::Tree &dt, uint32_t stack_bound, const Work &work,
     int waves = std::max(1, cfg.waves_per_cu);
     long long threads = (long long)cus * waves * 64;
     int grid = (int)std::max<long long>(1, threads / block);
-    // never launch more lanes than rays need (rounded up to whole blocks)
+    // never launch more lanes than units need (rounded up to whole blocks;
+    // n is only an upper bound when the count comes from the device)
     long long need_blocks = ((long long)n + block - 1) / block;
     if (need_blocks < 1) need_blocks = 1;
     if (grid > need_blocks) grid = (int)need_blocks;
@@ -1087,12 +1205,12 @@ int launch_k(const dev::Tree &dt, uint32_t stack_bound, const Work &work,
                                    hipFuncAttributeMaxDynamicSharedMemorySize,
                                    (int)lds));
     }
-    if (COUNT) {
+    if (COUNT && reset_counts) {
         HIPCHK(hipMemsetAsync(&ws->ctrl->visits, 0,
                               8 * sizeof(unsigned long long), stream));
     }
     hipLaunchKernelGGL(kern, dim3(grid), dim3(block), lds, stream, dt, work,
-                       (uint32_t)n, ws->ctrl, ws->spill, slots,
+                       (uint32_t)n, n_dev, ws->ctrl, ws->spill, slots,
                        (uint32_t)cfg.refill);
     HIPCHK(hipGetLastError());
     if (COUNT) {
@@ -1115,14 +1233,15 @@ int launch_k(const dev::Tree &dt, uint32_t stack_bound, const Work &work,
 
 template <class Work, bool COUNT>
 int launch_tree(const dev::Tree &t, uint32_t stack_bound, const Work &work,
-                size_t n, hipStream_t stream, voxg_counts *counts) {
+                size_t n, hipStream_t stream, voxg_counts *counts,
+                const uint32_t *n_dev = nullptr, bool reset_counts = true) {
     voxg_launch_config cfg = current_config();
     switch (cfg.stack_lds) {
-    case 2: return launch_k<2, Work, COUNT>(t, stack_bound, work, n, stream, cfg, counts);
-    case 4: return launch_k<4, Work, COUNT>(t, stack_bound, work, n, stream, cfg, counts);
-    case 8: return launch_k<8, Work, COUNT>(t, stack_bound, work, n, stream, cfg, counts);
-    case 16: return launch_k<16, Work, COUNT>(t, stack_bound, work, n, stream, cfg, counts);
-    case 32: return launch_k<32, Work, COUNT>(t, stack_bound, work, n, stream, cfg, counts);
+    case 2: return launch_k<2, Work, COUNT>(t, stack_bound, work, n, stream, cfg, counts, n_dev, reset_counts);
+    case 4: return launch_k<4, Work, COUNT>(t, stack_bound, work, n, stream, cfg, counts, n_dev, reset_counts);
+    case 8: return launch_k<8, Work, COUNT>(t, stack_bound, work, n, stream, cfg, counts, n_dev, reset_counts);
+    case 16: return launch_k<16, Work, COUNT>(t, stack_bound, work, n, stream, cfg, counts, n_dev, reset_counts);
+    case 32: return launch_k<32, Work, COUNT>(t, stack_bound, work, n, stream, cfg, counts, n_dev, reset_counts);
     default: return fail(VOXG_EINVAL, "stack_lds must be 2, 4, 8, 16 or 32");
     }
 }
@@ -1523,6 +1642,9 @@ static dev::PixSink make_sink(const voxg_tree *t,
     sink.sh.vx = t->host.voxel[0];
     sink.sh.vy = t->host.voxel[1];
     sink.sh.vz = t->host.voxel[2];
+    sink.sh.ivx = pow2_inverse(sink.sh.vx);
+    sink.sh.ivy = pow2_inverse(sink.sh.vy);
+    sink.sh.ivz = pow2_inverse(sink.sh.vz);
     return sink;
 }
 
@@ -1549,8 +1671,29 @@ static int render_launch(const voxg_tree *t, size_t n_frames,
     work.one.rnd_mode = (int)(q & 3);
     work.one.merge_mode = (int)(q & 0xc);
     work.one.sink = sink;
-    return launch<dev::BlockWork<false>, COUNT>(t, work, npix / 16, stream,
-                                                counts);
+    size_t nblk = npix / 16;
+    if (!current_config().render_cull || t->host.nodes.empty())
+        return launch<dev::BlockWork<false>, COUNT>(t, work, nblk, stream,
+                                                    counts);
+    // pre-pass: finish the blocks whose rays all miss the root box and
+    // queue the others for the traversal kernel
+    Workspace *ws = nullptr;
+    int rc = get_ws(stream, 16, &ws);
+    if (rc) return rc;
+    rc = get_queue(ws, nblk * sizeof(uint32_t));
+    if (rc) return rc;
+    if (COUNT)
+        HIPCHK(hipMemsetAsync(&ws->ctrl->visits, 0,
+                              8 * sizeof(unsigned long long), stream));
+    hipLaunchKernelGGL((dev::block_cull_kernel<COUNT>),
+                       dim3((npix + dev::kCullTile - 1) / dev::kCullTile),
+                       dim3(256), 0, stream,
+                       work.one, (uint32_t)npix, ws->ctrl, ws->queue);
+    HIPCHK(hipGetLastError());
+    work.queue = ws->queue;
+    return launch_tree<dev::BlockWork<false>, COUNT>(
+        dev_tree(t), t->stack_bound, work, nblk, stream, counts,
+        &ws->ctrl->qcount, false);
 }
 
 }  // extern "C++"
@@ -1844,6 +1987,9 @@ int srv_launch(voxg_server *s, SrvGroup &g) {
             it.sink.sh.vx = it.t.vx;
             it.sink.sh.vy = it.t.vy;
             it.sink.sh.vz = it.t.vz;
+            it.sink.sh.ivx = it.t.ivx;
+            it.sink.sh.ivy = it.t.ivy;
+            it.sink.sh.ivz = it.t.ivz;
             units += it.bpf;
             ritems.push_back(it);
             dev::CamDev c;

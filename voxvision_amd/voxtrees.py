@@ -209,7 +209,7 @@ def ball_counts(tree, c, r, stream=None):
 
 
 def set_launch_config(stack_lds=None, block=None, waves_per_cu=None,
-                      refill=None, chunk=None):
+                      refill=None, chunk=None, render_cull=None):
     L = _lib.load()
     cfg = _lib.LaunchConfig()
     check(L.voxg_get_launch_config(C.byref(cfg)))
@@ -223,6 +223,8 @@ def set_launch_config(stack_lds=None, block=None, waves_per_cu=None,
         cfg.refill = refill
     if chunk is not None:
         cfg.chunk = chunk
+    if render_cull is not None:
+        cfg.render_cull = render_cull
     check(L.voxg_set_launch_config(C.byref(cfg)), "voxg_set_launch_config")
     return get_launch_config()
 
@@ -232,4 +234,4 @@ def get_launch_config():
     check(_lib.load().voxg_get_launch_config(C.byref(cfg)))
     return dict(stack_lds=cfg.stack_lds, block=cfg.block,
                 waves_per_cu=cfg.waves_per_cu, refill=cfg.refill,
-                chunk=cfg.chunk)
+                chunk=cfg.chunk, render_cull=cfg.render_cull)
