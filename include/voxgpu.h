@@ -297,15 +297,26 @@ double voxg_server_now(const voxg_server *s);
 int voxg_server_destroy(voxg_server *s);
 
 /* Launch configuration of the traversal kernels (autotuned by
- * voxvision_amd.autotune; defaults are sane).  stack_lds: stack entries per
- * lane kept in LDS (power of two 2..32; deeper stacks spill to HBM and stay
- * exact); block: threads per workgroup (64..1024, multiple of 64);
- * waves_per_cu: persistent waves launched per CU; refill: minimum idle
- * lanes before a wave fetches new work (1..64); chunk: consecutive rays a
- * lane traces before it needs new work in voxg_ray_batch (1..4096; a
- * render lane always takes one 4x4 block); render_cull: 1 = voxg_render
- * first finishes, in a coherent pre-pass, the 4x4 blocks whose rays all
- * miss the scene box, 0 = every block goes through the traversal. */
+ * voxvision_amd.autotune; defaults are sane).
+ *   stack_lds     stack entries per lane kept in LDS (power of two 2..32;
+ *                 deeper stacks spill to HBM and stay exact)
+ *   block         threads per workgroup (64, 128, 192 or 256)
+ *   waves_per_cu  persistent waves launched per CU (1..64)
+ *   refill        minimum idle lanes before a wave fetches new work (1..64)
+ *   chunk         consecutive rays a lane traces before it needs new work
+ *                 in voxg_ray_batch and the batched form (1..4096; lowered
+ *                 for a launch too small to give every persistent lane a
+ *                 unit; a render lane always takes one 4x4 block)
+ *   render_cull   1: voxg_render first finishes, in a coherent pre-pass,
+ *                 the 4x4 blocks whose rays all miss the scene box; 0: every
+ *                 block goes through the traversal
+ *   leaf_batch    lanes of a wave that wait at a leaf before the wave
+ *                 searches its voxels (1..64; fewer when no lane of the wave
+ *                 can go on traversing)
+ *   finish_batch  lanes of a wave whose traversal ended before the wave
+ *                 stores their results and starts their next rays (1..64;
+ *                 fewer when no other lane is left)
+ * None of them changes any result. */
 typedef struct {
     int stack_lds;
     int block;
@@ -313,6 +324,8 @@ typedef struct {
     int refill;
     int chunk;
     int render_cull;
+    int leaf_batch;
+    int finish_batch;
 } voxg_launch_config;
 int voxg_set_launch_config(const voxg_launch_config *cfg);
 int voxg_get_launch_config(voxg_launch_config *cfg);

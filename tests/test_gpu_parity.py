@@ -348,3 +348,34 @@ def test_render_with_and_without_block_culling(gpu, oracle, quality):
     assert np.array_equal(_np_u32(b["voxel"]), vox_o)
     assert cb["rays"] == st["root_traversals"]
     assert cb["node_visits"] == st["node_visits"]
+
+
+@pytest.mark.parametrize("leaf_batch,finish_batch", [(1, 1), (64, 64),
+                                                     (8, 32), (32, 4)])
+def test_lane_batching_changes_nothing(gpu, oracle, leaf_batch, finish_batch):
+    """Waves gather lanes at leaves and at ended traversals before they
+    serve them; any thresholds give the oracle's rays, pixels and counts."""
+    from voxvision_amd import scenes, voxrnd, voxtrees as VT
+    c1 = _c1()
+    T = oracle.Tree(c1["dots"])
+    G = VT.make_tree(c1["dots"])
+    o, d = scenes.random_rays(20000, *c1["bbox"], seed=23)
+    vox_o, hit_o = T.rays(o, d)
+    W, H = c1["W"], c1["H"]
+    q = 2 | 8
+    _, pvox_o, phit_o, st = T.render(c1["camera"], W, H, q)
+    old = VT.get_launch_config()
+    try:
+        VT.set_launch_config(leaf_batch=leaf_batch, finish_batch=finish_batch)
+        vox_g, hit_g = VT.ray_tree_intersection(G, o, d)
+        out = voxrnd.render(G, [c1["camera"]], W, H, quality=q,
+                            want_colors=False)
+        cnt = voxrnd.render_counts(G, [c1["camera"]], W, H, quality=q)
+        torch.cuda.synchronize()
+    finally:
+        VT.set_launch_config(**old)
+    _assert_rays_equal(vox_g, hit_g, vox_o, hit_o)
+    _assert_rays_equal(out["voxel"], out["hit"], pvox_o, phit_o)
+    assert cnt["node_visits"] == st["node_visits"]
+    assert cnt["voxel_tests"] == st["voxel_tests"]
+    assert cnt["fast_blocks"] == st["fast_blocks"]

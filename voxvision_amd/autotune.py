@@ -3,7 +3,9 @@
 The tuned parameters are those of voxg_launch_config: entries of the
 per-lane LDS stack ring (`stack_lds`), threads per workgroup (`block`),
 persistent waves per CU (`waves_per_cu`), idle lanes that trigger a refill
-(`refill`) and rays a lane takes per fetch (`chunk`).  The search is
+(`refill`), rays a lane takes per fetch (`chunk`), the render pre-pass
+(`render_cull`) and the lanes a wave gathers before it searches leaves
+(`leaf_batch`) or finishes traversals (`finish_batch`).  The search is
 coordinate descent from the current configuration: each parameter in turn
 is swept over its candidate values with the others fixed, keeping the
 fastest, until a full pass changes nothing.  Timing uses HIP events on the
@@ -25,6 +27,8 @@ SPACE = {
     "refill": [1, 4, 8, 16, 32],
     "chunk": [1, 2, 4, 8, 16, 32],
     "render_cull": [0, 1],
+    "leaf_batch": [1, 8, 16, 24, 32],
+    "finish_batch": [1, 8, 16, 24, 32],
 }
 
 

@@ -86,8 +86,37 @@ struct f3 {
     float x, y, z;
 };
 
+// Reciprocals of a ray direction's components in double precision,
+// RN(1/(double)d) per axis, computed once per ray.
+struct rcp3 {
+    double x, y, z;
+};
+
+__device__ __forceinline__ rcp3 make_rcp(f3 d) {
+    return rcp3{1.0 / (double)d.x, 1.0 / (double)d.y, 1.0 / (double)d.z};
+}
+
+// The IEEE single-precision quotient a / b from rb = RN(1/(double)b):
+// q = RN_single(RN_double(a * rb)).  For a quotient in the normal single
+// range this is the correctly rounded a / b: the exact quotient of two
+// floats lies at least 2^-49 (relative) away from every midpoint between
+// adjacent floats (a - m*b is a non-zero multiple of 2^(exp(m)+exp(b)),
+// since the odd part of m*b has 25 or more bits and that of a at most 24),
+// while a * rb rounded to double is within 2^-52 of it.  Quotients below
+// 2^-126 can fall exactly on a midpoint of the subnormal grid; those, and
+// only those, are divided again in single precision.  Zeros, infinities
+// and NaNs come out as the single-precision division gives them.
+__device__ __forceinline__ float qdiv(float a, double rb, float b) {
+    float q = (float)((double)a * rb);
+    if (__builtin_expect(__builtin_fabsf(q) < 0x1p-126f && a != 0.0f, 0))
+        q = a / b;
+    return q;
+}
+
 // geom-sse.c:84-116 (fit_into_box :34-40)
-__device__ __forceinline__ bool hit_box(f3 bmin, f3 bmax, f3 o, f3 d, f3 &r) {
+template <bool RCP>
+__device__ __forceinline__ bool hit_box_t(f3 bmin, f3 bmax, f3 o, f3 d,
+                                          const rcp3 &rd, f3 &r) {
     float fx = smin(smax(o.x, bmin.x), bmax.x);
     float fy = smin(smax(o.y, bmin.y), bmax.y);
     float fz = smin(smax(o.z, bmin.z), bmax.z);
@@ -96,9 +125,16 @@ __device__ __forceinline__ bool hit_box(f3 bmin, f3 bmax, f3 o, f3 d, f3 &r) {
         r = o;
         return true;
     }
-    float tx = (d.x == 0.0f) ? 0.0f : sx / d.x;
-    float ty = (d.y == 0.0f) ? 0.0f : sy / d.y;
-    float tz = (d.z == 0.0f) ? 0.0f : sz / d.z;
+    float tx, ty, tz;
+    if constexpr (RCP) {
+        tx = (d.x == 0.0f) ? 0.0f : qdiv(sx, rd.x, d.x);
+        ty = (d.y == 0.0f) ? 0.0f : qdiv(sy, rd.y, d.y);
+        tz = (d.z == 0.0f) ? 0.0f : qdiv(sz, rd.z, d.z);
+    } else {
+        tx = (d.x == 0.0f) ? 0.0f : sx / d.x;
+        ty = (d.y == 0.0f) ? 0.0f : sy / d.y;
+        tz = (d.z == 0.0f) ? 0.0f : sz / d.z;
+    }
     // lane-wise result of the shuffle/max network (:101-105)
     float m0 = smax(smax(tx, tz), smax(tx, ty));
     if (m0 <= 0.0f) return false;
@@ -119,12 +155,19 @@ __device__ __forceinline__ float comp(f3 v, int i) {
 }
 
 // geom-sse.c:118-142
-__device__ __forceinline__ bool hit_plane(f3 o, f3 d, f3 c, int p, f3 bmin,
-                                          f3 bmax, f3 &r) {
+__device__ __forceinline__ bool hit_box(f3 bmin, f3 bmax, f3 o, f3 d, f3 &r) {
+    return hit_box_t<false>(bmin, bmax, o, d, rcp3{0, 0, 0}, r);
+}
+__device__ __forceinline__ bool hit_box(f3 bmin, f3 bmax, f3 o, f3 d,
+                                        const rcp3 &rd, f3 &r) {
+    return hit_box_t<true>(bmin, bmax, o, d, rd, r);
+}
+__device__ __forceinline__ bool hit_plane(f3 o, f3 d, const rcp3 &rd, f3 c,
+                                          int p, f3 bmin, f3 bmax, f3 &r) {
     float cp = comp(c, p), op = comp(o, p), dp = comp(d, p);
     float k = cp - op;
     if (k == 0.0f || ((dp < 0.0f) != (k < 0.0f))) return false;
-    k = k / dp;
+    k = qdiv(k, p == 0 ? rd.x : (p == 1 ? rd.y : rd.z), dp);
     float rx = __builtin_fmaf(k, d.x, o.x);
     float ry = __builtin_fmaf(k, d.y, o.y);
     float rz = __builtin_fmaf(k, d.z, o.z);
@@ -538,9 +581,9 @@ struct BlockWork {
             trace_pixel(st, p, st.leaf_try ? st.leaf : 0u, cur, org, d);
             return true;
         }
-        if (st.mode == VOXG_QUALITY_FAST) st.fast++;
-        else st.best++;
-        if (st.merge) st.merged++;
+        st.fast += (st.mode == VOXG_QUALITY_FAST) ? 1u : 0u;
+        st.best += (st.mode == VOXG_QUALITY_FAST) ? 0u : 1u;
+        st.merged += st.merge ? 1u : 0u;
         return false;
     }
     __device__ __forceinline__ void begin(uint32_t unit, State &st,
@@ -667,7 +710,8 @@ struct BlockWork {
 template <int K, class Work, bool COUNT>
 __global__ void __launch_bounds__(256)
 rti_kernel(Tree t, Work work, uint32_t n, const uint32_t *n_dev, Ctrl *ctrl,
-           uint4 *__restrict__ spill, uint32_t spill_slots, uint32_t refill) {
+           uint4 *__restrict__ spill, uint32_t spill_slots, uint32_t refill,
+           uint32_t fin_batch, uint32_t leaf_batch) {
     extern __shared__ uint4 ring[];  // [K][blockDim.x]
     if (n_dev) n = *n_dev;  // unit count produced on the device
     const uint32_t tid = threadIdx.x;
@@ -680,6 +724,14 @@ rti_kernel(Tree t, Work work, uint32_t n, const uint32_t *n_dev, Ctrl *ctrl,
     bool exhausted = false;
     uint32_t cur = 0;
     f3 d{0, 0, 0}, org{0, 0, 0};
+    rcp3 rd{0, 0, 0};
+    // lane phase: traversing, waiting at a leaf whose box it entered (entry
+    // point in org), or ended with a result
+    // in (p_hit, p_vid, cur, org) that waits to be handed to the work unit
+    enum { PH_TRAV = 0, PH_LEAF = 1, PH_DONE = 2 };
+    int ph = PH_TRAV;
+    bool p_hit = false;
+    uint32_t p_vid = kEnd;
     typename Work::State st;
     std::memset(&st, 0, sizeof(st));
     int sp = 0, lo = 0;
@@ -700,7 +752,9 @@ rti_kernel(Tree t, Work work, uint32_t n, const uint32_t *n_dev, Ctrl *ctrl,
                 uint32_t my = base + lanes_below(idle);
                 if (my < n) {
                     work.begin(my, st, cur, org, d);
+                    rd = make_rcp(d);
                     sp = lo = 0;
+                    ph = PH_TRAV;
                     active = true;
                     if (COUNT) {
                         if (cur == 0) nr++;
@@ -715,58 +769,32 @@ rti_kernel(Tree t, Work work, uint32_t n, const uint32_t *n_dev, Ctrl *ctrl,
         }
         if (!active) continue;
 
-        // ---- process (cur, org) ----
         const Tree &T = work.tree(st, t);
-        const f3 V = f3{T.vx, T.vy, T.vz};
-        f3 e = org;
-        bool hit = false, descend = false;
-        uint32_t vid = kEnd;
-        Node nd;
-        if (!T.empty) {
-            nd = load_node(T.nodes, cur);
-            if (COUNT) nv++;
-        }
-        if (!T.empty && hit_box(nd.bmin, nd.bmax, org, d, e)) {
-            uint32_t type = nd.info & NT_TYPE_MASK;
-            if (type == NT_DENSE) {
-                // search.c:31-37
-                hit = true;
-                vid = dense_voxel(nd, e, T);
-            } else if (type == NT_LEAF) {
-                // search.c:39-81
-                uint32_t cnt = nd.info >> 8;
-                float best = INFINITY;
-                f3 bres = e;
-                for (uint32_t k = 0; k < cnt; k++) {
-                    const float *p = T.dots + 3u * (nd.ref + k);
-                    f3 vmin = f3{p[0], p[1], p[2]};
-                    f3 vmax = f3{vmin.x + V.x, vmin.y + V.y, vmin.z + V.z};
-                    f3 far;
-                    if (COUNT) nt++;
-                    if (hit_box(vmin, vmax, e, d, far)) {
-                        float m = abs_metric(e, far);
-                        if (m == 0.0f) {
-                            bres = far;
-                            vid = nd.ref + k;
-                            hit = true;
-                            break;
-                        }
-                        if (m < best) {
-                            best = m;
-                            bres = far;
-                            vid = nd.ref + k;
-                            hit = true;
-                        }
-                    }
-                }
-                e = bres;
+        bool popme = false;
+
+        // phase 1: one node of every traversing lane (search.c:8-29, 84-148)
+        if (ph == PH_TRAV) {
+            f3 e = org;
+            Node nd;
+            bool in = false;
+            if (!T.empty) {
+                nd = load_node(T.nodes, cur);
+                if (COUNT) nv++;
+                in = hit_box(nd.bmin, nd.bmax, org, d, rd, e);
+            }
+            if (!in) {
+                popme = true;
+            } else if ((nd.info & NT_TYPE_MASK) != NT_INNER) {
+                // a leaf: wait at it with the entry point in org
+                ph = PH_LEAF;
+                org = e;
             } else {
                 // search.c:84-148
                 int s = corrected_subspace(nd.c, e, d);
                 f3 P0, P1, P2;
-                bool h0 = hit_plane(e, d, nd.c, 0, nd.bmin, nd.bmax, P0);
-                bool h1 = hit_plane(e, d, nd.c, 1, nd.bmin, nd.bmax, P1);
-                bool h2 = hit_plane(e, d, nd.c, 2, nd.bmin, nd.bmax, P2);
+                bool h0 = hit_plane(e, d, rd, nd.c, 0, nd.bmin, nd.bmax, P0);
+                bool h1 = hit_plane(e, d, rd, nd.c, 1, nd.bmin, nd.bmax, P1);
+                bool h2 = hit_plane(e, d, rd, nd.c, 2, nd.bmin, nd.bmax, P2);
                 // compact the hits in plane order (:108-113)
                 int cntp = 0, i0 = 0, i1 = 0, i2 = 0;
                 f3 Q0 = P0, Q1 = P1, Q2 = P2;
@@ -804,12 +832,12 @@ rti_kernel(Tree t, Work work, uint32_t n, const uint32_t *n_dev, Ctrl *ctrl,
                 if (cntp >= 2 && m1 < m0) {
                     f3 tq = Q0; Q0 = Q1; Q1 = tq;
                     int ti = i0; i0 = i1; i1 = ti;
-                    float tm = m0; m0 = m1; m1 = tm;
+                    float tv = m0; m0 = m1; m1 = tv;
                 }
                 if (cntp >= 3 && m2 < m0) {
                     f3 tq = Q0; Q0 = Q2; Q2 = tq;
                     int ti = i0; i0 = i2; i2 = ti;
-                    float tm = m0; m0 = m2; m2 = tm;
+                    float tv = m0; m0 = m2; m2 = tv;
                 }
                 if (cntp >= 3 && m2 < m1) {
                     f3 tq = Q1; Q1 = Q2; Q2 = tq;
@@ -824,21 +852,21 @@ rti_kernel(Tree t, Work work, uint32_t n, const uint32_t *n_dev, Ctrl *ctrl,
                 uint32_t cs2 = cntp >= 2 ? child_of(nd.ref, mask, s2) : kEnd;
                 uint32_t cs3 = cntp >= 3 ? child_of(nd.ref, mask, s3) : kEnd;
                 // push in reverse visiting order (:137-147)
-#define VOXG_PUSH(CI, Q)                                                      \
-    if ((CI) != kEnd) {                                                      \
-        if (sp - lo == K) {                                                  \
-            if ((uint32_t)lo >= spill_slots) {                               \
-                atomicOr(&ctrl->overflow, 1u);                               \
-            } else {                                                         \
-                spill[(size_t)lo * gthreads + gtid] =                        \
-                    ring[(lo & (K - 1)) * bdim + tid];                       \
-            }                                                                \
-            lo++;                                                            \
-        }                                                                    \
-        ring[(sp & (K - 1)) * bdim + tid] =                                  \
-            make_uint4((CI), __float_as_uint((Q).x), __float_as_uint((Q).y), \
-                       __float_as_uint((Q).z));                              \
-        sp++;                                                                \
+#define VOXG_PUSH(CI, Q)                                                  \
+    if ((CI) != kEnd) {                                                  \
+        if (sp - lo == K) {                                              \
+            if ((uint32_t)lo >= spill_slots) {                           \
+                atomicOr(&ctrl->overflow, 1u);                           \
+            } else {                                                     \
+                spill[(size_t)lo * gthreads + gtid] =                    \
+                    ring[(lo & (K - 1)) * bdim + tid];                   \
+            }                                                            \
+            lo++;                                                        \
+        }                                                                \
+        ring[(sp & (K - 1)) * bdim + tid] = make_uint4(                  \
+            (CI), __float_as_uint((Q).x), __float_as_uint((Q).y),        \
+            __float_as_uint((Q).z));                                     \
+        sp++;                                                            \
     }
                 VOXG_PUSH(cs3, Q2)
                 VOXG_PUSH(cs2, Q1)
@@ -847,16 +875,67 @@ rti_kernel(Tree t, Work work, uint32_t n, const uint32_t *n_dev, Ctrl *ctrl,
                 if (cs0 != kEnd) {
                     cur = cs0;
                     org = e;
-                    descend = true;
+                } else {
+                    popme = true;
                 }
             }
         }
-        bool done = false;
-        if (hit) {
-            done = true;
-        } else if (!descend) {
+        const uint64_t tm = __ballot(ph == PH_TRAV);
+        const uint64_t lm = __ballot(ph == PH_LEAF);
+
+        // phase 2: leaves (search.c:31-81), once enough lanes wait at one
+        // or no lane is traversing
+        if (ph == PH_LEAF && ((uint32_t)__popcll(lm) >= leaf_batch || tm == 0)) {
+            const Node nd = load_node(T.nodes, cur);
+            const f3 e = org;
+            bool hit = false;
+            uint32_t vid = kEnd;
+            f3 bres = e;
+            if ((nd.info & NT_TYPE_MASK) == NT_DENSE) {
+                hit = true;
+                vid = dense_voxel(nd, e, T);
+            } else {
+                const f3 V = f3{T.vx, T.vy, T.vz};
+                const uint32_t cnt = nd.info >> 8;
+                float best = INFINITY;
+                for (uint32_t k = 0; k < cnt; k++) {
+                    const float *p = T.dots + 3u * (nd.ref + k);
+                    f3 vmin = f3{p[0], p[1], p[2]};
+                    f3 vmax = f3{vmin.x + V.x, vmin.y + V.y, vmin.z + V.z};
+                    f3 far;
+                    if (COUNT) nt++;
+                    if (hit_box(vmin, vmax, e, d, rd, far)) {
+                        float m = abs_metric(e, far);
+                        if (m == 0.0f) {
+                            bres = far;
+                            vid = nd.ref + k;
+                            hit = true;
+                            break;
+                        }
+                        if (m < best) {
+                            best = m;
+                            bres = far;
+                            vid = nd.ref + k;
+                            hit = true;
+                        }
+                    }
+                }
+            }
+            if (hit) {
+                ph = PH_DONE;
+                p_hit = true;
+                p_vid = vid;
+                org = bres;
+            } else {
+                ph = PH_TRAV;
+                popme = true;
+            }
+        }
+
+        // phase 3: continue with the next pending (node, origin) pair
+        if (popme) {
             if (sp == 0) {
-                done = true;
+                cur = kEnd;
             } else {
                 uint4 v;
                 if (sp == lo) {
@@ -869,18 +948,25 @@ rti_kernel(Tree t, Work work, uint32_t n, const uint32_t *n_dev, Ctrl *ctrl,
                     sp--;
                     v = ring[(sp & (K - 1)) * bdim + tid];
                 }
-                cur = v.x;
+                cur = v.x;  // kEnd only after a reported overflow
                 org = f3{__uint_as_float(v.y), __uint_as_float(v.z),
                          __uint_as_float(v.w)};
-                if (cur == kEnd) done = true;  // only after a reported overflow
+            }
+            if (cur == kEnd) {
+                ph = PH_DONE;
+                p_hit = false;
+                p_vid = kEnd;
             }
         }
-        if (done) {
-            // the traversal ended: hand the result to the work unit
-            uint32_t node = cur;
+
+        // phase 4: hand ended traversals to their work units, in batches
+        const uint64_t dm = __ballot(ph == PH_DONE);
+        const uint64_t am = __ballot(true);
+        if (ph == PH_DONE && ((uint32_t)__popcll(dm) >= fin_batch || dm == am)) {
+            ph = PH_TRAV;
             sp = lo = 0;
-            if (work.finish(st, hit, hit ? vid : kEnd, hit ? node : kEnd, e,
-                            cur, org, d)) {
+            if (work.finish(st, p_hit, p_vid, cur, org, cur, org, d)) {
+                rd = make_rcp(d);
                 if (COUNT) {
                     if (cur == 0) nr++;
                     else nl++;
@@ -1089,7 +1175,7 @@ struct voxg_batch {
 namespace {
 
 std::mutex g_mu;
-voxg_launch_config g_cfg = {8, 256, 20, 8, 4, 1};
+voxg_launch_config g_cfg = {8, 256, 20, 8, 4, 1, 16, 16};
 
 voxg_launch_config current_config() {
     std::lock_guard<std::mutex> g(g_mu);
@@ -1117,6 +1203,19 @@ int cu_count(int device) {
         n = 256;
     cache[device] = n;
     return n;
+}
+
+// Rays per work unit for a launch of n rays: the configured chunk, cut so
+// that a small launch still has a unit for every persistent lane.
+uint32_t effective_chunk(size_t n) {
+    voxg_launch_config cfg = current_config();
+    int devid = 0;
+    if (hipGetDevice(&devid) != hipSuccess) devid = 0;
+    uint64_t lanes = (uint64_t)cu_count(devid) *
+                     (uint64_t)std::max(1, cfg.waves_per_cu) * 64u;
+    uint64_t c = (uint64_t)std::max(1, cfg.chunk);
+    uint64_t fit = std::max<uint64_t>(1, (uint64_t)n / lanes);
+    return (uint32_t)std::min(c, fit);
 }
 
 int get_ws(hipStream_t s, size_t spill_bytes, Workspace **out) {
@@ -1211,7 +1310,8 @@ int launch_k(const dev::Tree &dt, uint32_t stack_bound, const Work &work,
     }
     hipLaunchKernelGGL(kern, dim3(grid), dim3(block), lds, stream, dt, work,
                        (uint32_t)n, n_dev, ws->ctrl, ws->spill, slots,
-                       (uint32_t)cfg.refill);
+                       (uint32_t)cfg.refill, (uint32_t)cfg.finish_batch,
+                       (uint32_t)cfg.leaf_batch);
     HIPCHK(hipGetLastError());
     if (COUNT) {
         dev::Ctrl h;
@@ -1476,6 +1576,12 @@ int voxg_set_launch_config(const voxg_launch_config *cfg) {
         return fail(VOXG_EINVAL, "refill must be in 1..64");
     if (cfg->chunk < 1 || cfg->chunk > 4096)
         return fail(VOXG_EINVAL, "chunk must be in 1..4096");
+    if (cfg->render_cull != 0 && cfg->render_cull != 1)
+        return fail(VOXG_EINVAL, "render_cull must be 0 or 1");
+    if (cfg->leaf_batch < 1 || cfg->leaf_batch > 64)
+        return fail(VOXG_EINVAL, "leaf_batch must be in 1..64");
+    if (cfg->finish_batch < 1 || cfg->finish_batch > 64)
+        return fail(VOXG_EINVAL, "finish_batch must be in 1..64");
     std::lock_guard<std::mutex> g(g_mu);
     g_cfg = *cfg;
     return 0;
@@ -1509,7 +1615,7 @@ int voxg_ray_batch(const voxg_tree *t, size_t n, const float *ox,
     if ((hx || hy || hz) && !(hx && hy && hz))
         return fail(VOXG_EINVAL, "hx, hy, hz must be all set or all NULL");
     if (n == 0) return 0;
-    uint32_t c = (uint32_t)current_config().chunk;
+    uint32_t c = effective_chunk(n);
     dev::ChunkWork<dev::SoaSrc, dev::SoaSink> work{
         dev::SoaSrc{ox, oy, oz, dx, dy, dz},
         dev::SoaSink{voxel_out, leaf_out, hx, hy, hz}, (uint32_t)n, c};
@@ -1526,7 +1632,7 @@ int voxg_ray_batch_count(const voxg_tree *t, size_t n, const float *ox,
     if (!out) return fail(VOXG_EINVAL, "out is NULL");
     std::memset(out, 0, sizeof(*out));
     if (n == 0) return 0;
-    uint32_t c = (uint32_t)current_config().chunk;
+    uint32_t c = effective_chunk(n);
     dev::ChunkWork<dev::SoaSrc, dev::SoaSink> work{
         dev::SoaSrc{ox, oy, oz, dx, dy, dz},
         dev::SoaSink{nullptr, nullptr, nullptr, nullptr, nullptr},
@@ -1735,7 +1841,9 @@ int voxg_batch_create(size_t n_items, const voxg_batch_item *items,
     if (n_items > (1u << 20)) return fail(VOXG_ERANGE, "too many items");
     int devid = 0;
     HIPCHK(hipGetDevice(&devid));
-    uint32_t chunk = (uint32_t)current_config().chunk;
+    uint64_t total = 0;
+    for (size_t i = 0; i < n_items; i++) total += items[i].count;
+    uint32_t chunk = effective_chunk((size_t)total);
     std::vector<dev::BatchItem> h;
     uint64_t units = 0, rays_end = 0;
     uint32_t bound = 1;

@@ -208,23 +208,19 @@ def ball_counts(tree, c, r, stream=None):
                 rays=cnt.rays)
 
 
-def set_launch_config(stack_lds=None, block=None, waves_per_cu=None,
-                      refill=None, chunk=None, render_cull=None):
+def set_launch_config(**fields):
+    """Change fields of the traversal launch configuration
+    (voxg_launch_config: stack_lds, block, waves_per_cu, refill, chunk,
+    render_cull, leaf_batch, finish_batch); returns the new configuration."""
     L = _lib.load()
     cfg = _lib.LaunchConfig()
     check(L.voxg_get_launch_config(C.byref(cfg)))
-    if stack_lds is not None:
-        cfg.stack_lds = stack_lds
-    if block is not None:
-        cfg.block = block
-    if waves_per_cu is not None:
-        cfg.waves_per_cu = waves_per_cu
-    if refill is not None:
-        cfg.refill = refill
-    if chunk is not None:
-        cfg.chunk = chunk
-    if render_cull is not None:
-        cfg.render_cull = render_cull
+    names = [k for k, _ in cfg._fields_]
+    for k, v in fields.items():
+        if k not in names:
+            raise TypeError("unknown launch configuration field %r" % k)
+        if v is not None:
+            setattr(cfg, k, int(v))
     check(L.voxg_set_launch_config(C.byref(cfg)), "voxg_set_launch_config")
     return get_launch_config()
 
@@ -232,6 +228,4 @@ def set_launch_config(stack_lds=None, block=None, waves_per_cu=None,
 def get_launch_config():
     cfg = _lib.LaunchConfig()
     check(_lib.load().voxg_get_launch_config(C.byref(cfg)))
-    return dict(stack_lds=cfg.stack_lds, block=cfg.block,
-                waves_per_cu=cfg.waves_per_cu, refill=cfg.refill,
-                chunk=cfg.chunk, render_cull=cfg.render_cull)
+    return {k: getattr(cfg, k) for k, _ in cfg._fields_}
