@@ -86,37 +86,8 @@ struct f3 {
     float x, y, z;
 };
 
-// Reciprocals of a ray direction's components in double precision,
-// RN(1/(double)d) per axis, computed once per ray.
-struct rcp3 {
-    double x, y, z;
-};
-
-__device__ __forceinline__ rcp3 make_rcp(f3 d) {
-    return rcp3{1.0 / (double)d.x, 1.0 / (double)d.y, 1.0 / (double)d.z};
-}
-
-// The IEEE single-precision quotient a / b from rb = RN(1/(double)b):
-// q = RN_single(RN_double(a * rb)).  For a quotient in the normal single
-// range this is the correctly rounded a / b: the exact quotient of two
-// floats lies at least 2^-49 (relative) away from every midpoint between
-// adjacent floats (a - m*b is a non-zero multiple of 2^(exp(m)+exp(b)),
-// since the odd part of m*b has 25 or more bits and that of a at most 24),
-// while a * rb rounded to double is within 2^-52 of it.  Quotients below
-// 2^-126 can fall exactly on a midpoint of the subnormal grid; those, and
-// only those, are divided again in single precision.  Zeros, infinities
-// and NaNs come out as the single-precision division gives them.
-__device__ __forceinline__ float qdiv(float a, double rb, float b) {
-    float q = (float)((double)a * rb);
-    if (__builtin_expect(__builtin_fabsf(q) < 0x1p-126f && a != 0.0f, 0))
-        q = a / b;
-    return q;
-}
-
 // geom-sse.c:84-116 (fit_into_box :34-40)
-template <bool RCP>
-__device__ __forceinline__ bool hit_box_t(f3 bmin, f3 bmax, f3 o, f3 d,
-                                          const rcp3 &rd, f3 &r) {
+__device__ __forceinline__ bool hit_box(f3 bmin, f3 bmax, f3 o, f3 d, f3 &r) {
     float fx = smin(smax(o.x, bmin.x), bmax.x);
     float fy = smin(smax(o.y, bmin.y), bmax.y);
     float fz = smin(smax(o.z, bmin.z), bmax.z);
@@ -125,16 +96,9 @@ __device__ __forceinline__ bool hit_box_t(f3 bmin, f3 bmax, f3 o, f3 d,
         r = o;
         return true;
     }
-    float tx, ty, tz;
-    if constexpr (RCP) {
-        tx = (d.x == 0.0f) ? 0.0f : qdiv(sx, rd.x, d.x);
-        ty = (d.y == 0.0f) ? 0.0f : qdiv(sy, rd.y, d.y);
-        tz = (d.z == 0.0f) ? 0.0f : qdiv(sz, rd.z, d.z);
-    } else {
-        tx = (d.x == 0.0f) ? 0.0f : sx / d.x;
-        ty = (d.y == 0.0f) ? 0.0f : sy / d.y;
-        tz = (d.z == 0.0f) ? 0.0f : sz / d.z;
-    }
+    float tx = (d.x == 0.0f) ? 0.0f : sx / d.x;
+    float ty = (d.y == 0.0f) ? 0.0f : sy / d.y;
+    float tz = (d.z == 0.0f) ? 0.0f : sz / d.z;
     // lane-wise result of the shuffle/max network (:101-105)
     float m0 = smax(smax(tx, tz), smax(tx, ty));
     if (m0 <= 0.0f) return false;
@@ -155,19 +119,12 @@ __device__ __forceinline__ float comp(f3 v, int i) {
 }
 
 // geom-sse.c:118-142
-__device__ __forceinline__ bool hit_box(f3 bmin, f3 bmax, f3 o, f3 d, f3 &r) {
-    return hit_box_t<false>(bmin, bmax, o, d, rcp3{0, 0, 0}, r);
-}
-__device__ __forceinline__ bool hit_box(f3 bmin, f3 bmax, f3 o, f3 d,
-                                        const rcp3 &rd, f3 &r) {
-    return hit_box_t<true>(bmin, bmax, o, d, rd, r);
-}
-__device__ __forceinline__ bool hit_plane(f3 o, f3 d, const rcp3 &rd, f3 c,
-                                          int p, f3 bmin, f3 bmax, f3 &r) {
+__device__ __forceinline__ bool hit_plane(f3 o, f3 d, f3 c, int p, f3 bmin,
+                                          f3 bmax, f3 &r) {
     float cp = comp(c, p), op = comp(o, p), dp = comp(d, p);
     float k = cp - op;
     if (k == 0.0f || ((dp < 0.0f) != (k < 0.0f))) return false;
-    k = qdiv(k, p == 0 ? rd.x : (p == 1 ? rd.y : rd.z), dp);
+    k = k / dp;
     float rx = __builtin_fmaf(k, d.x, o.x);
     float ry = __builtin_fmaf(k, d.y, o.y);
     float rz = __builtin_fmaf(k, d.z, o.z);
@@ -317,25 +274,6 @@ __device__ __forceinline__ f3 screen2world(const CamDev &c, int sx, int sy) {
               __builtin_fmaf(c.qw, t3, vz) + c3};
 }
 
-// Camera rays in the square layout: ray r of a frame is pixel index
-// cs*16 + p, i.e. 16x4-pixel tiles of four 4x4 blocks per 64 rays.
-struct CamSrc {
-    const CamDev *cams;
-    uint32_t W, per_frame;
-    __device__ __forceinline__ void get(uint32_t r, f3 &o, f3 &d) const {
-        uint32_t f = r / per_frame, q = r - f * per_frame;
-        uint32_t row = 4u * W;
-        uint32_t ty = q / row, rem = q - ty * row;
-        uint32_t tx = rem >> 6, l = rem & 63u;
-        uint32_t b = l >> 4, p = l & 15u;
-        int sx = (int)((tx * 4u + b) * 4u + (p & 3u));
-        int sy = (int)(ty * 4u + (p >> 2));
-        const CamDev &c = cams[f];
-        o = f3{c.px, c.py, c.pz};
-        d = screen2world(c, sx, sy);
-    }
-};
-
 struct Shade {
     const uint8_t *tex;
     uint32_t rshift, gshift, bshift, amask;
@@ -386,33 +324,6 @@ struct Ctrl {
 //   finish(st, hit, vid, node, e, cur, org, d) -> consumes a traversal's
 //       result; returns true when it started another traversal
 // ---------------------------------------------------------------------------
-
-// One ray per unit (vox_ray_tree_intersection, or BEST-mode pixels).
-template <class Src, class Sink>
-struct RayWork {
-    Src src;
-    Sink sink;
-    struct State {
-        uint32_t ray;
-    };
-    __device__ __forceinline__ void begin(uint32_t unit, State &st,
-                                          uint32_t &cur, f3 &org, f3 &d) const {
-        st.ray = unit;
-        src.get(unit, org, d);
-        cur = 0;
-    }
-    __device__ __forceinline__ bool finish(State &st, bool hit, uint32_t vid,
-                                           uint32_t node, f3 e, uint32_t &,
-                                           f3 &, f3 &) const {
-        sink.put(st.ray, hit, vid, node, e);
-        return false;
-    }
-    __device__ __forceinline__ void flush(const State &, Ctrl *) const {}
-    __device__ __forceinline__ const Tree &tree(const State &,
-                                                const Tree &t) const {
-        return t;
-    }
-};
 
 // A run of `chunk` consecutive rays per unit, traced one after the other
 // by the same lane: a lane whose ray ends moves on at once instead of
@@ -724,7 +635,6 @@ rti_kernel(Tree t, Work work, uint32_t n, const uint32_t *n_dev, Ctrl *ctrl,
     bool exhausted = false;
     uint32_t cur = 0;
     f3 d{0, 0, 0}, org{0, 0, 0};
-    rcp3 rd{0, 0, 0};
     // lane phase: traversing, waiting at a leaf whose box it entered (entry
     // point in org), or ended with a result
     // in (p_hit, p_vid, cur, org) that waits to be handed to the work unit
@@ -752,7 +662,6 @@ rti_kernel(Tree t, Work work, uint32_t n, const uint32_t *n_dev, Ctrl *ctrl,
                 uint32_t my = base + lanes_below(idle);
                 if (my < n) {
                     work.begin(my, st, cur, org, d);
-                    rd = make_rcp(d);
                     sp = lo = 0;
                     ph = PH_TRAV;
                     active = true;
@@ -780,7 +689,7 @@ rti_kernel(Tree t, Work work, uint32_t n, const uint32_t *n_dev, Ctrl *ctrl,
             if (!T.empty) {
                 nd = load_node(T.nodes, cur);
                 if (COUNT) nv++;
-                in = hit_box(nd.bmin, nd.bmax, org, d, rd, e);
+                in = hit_box(nd.bmin, nd.bmax, org, d, e);
             }
             if (!in) {
                 popme = true;
@@ -792,9 +701,9 @@ rti_kernel(Tree t, Work work, uint32_t n, const uint32_t *n_dev, Ctrl *ctrl,
                 // search.c:84-148
                 int s = corrected_subspace(nd.c, e, d);
                 f3 P0, P1, P2;
-                bool h0 = hit_plane(e, d, rd, nd.c, 0, nd.bmin, nd.bmax, P0);
-                bool h1 = hit_plane(e, d, rd, nd.c, 1, nd.bmin, nd.bmax, P1);
-                bool h2 = hit_plane(e, d, rd, nd.c, 2, nd.bmin, nd.bmax, P2);
+                bool h0 = hit_plane(e, d, nd.c, 0, nd.bmin, nd.bmax, P0);
+                bool h1 = hit_plane(e, d, nd.c, 1, nd.bmin, nd.bmax, P1);
+                bool h2 = hit_plane(e, d, nd.c, 2, nd.bmin, nd.bmax, P2);
                 // compact the hits in plane order (:108-113)
                 int cntp = 0, i0 = 0, i1 = 0, i2 = 0;
                 f3 Q0 = P0, Q1 = P1, Q2 = P2;
@@ -904,7 +813,7 @@ rti_kernel(Tree t, Work work, uint32_t n, const uint32_t *n_dev, Ctrl *ctrl,
                     f3 vmax = f3{vmin.x + V.x, vmin.y + V.y, vmin.z + V.z};
                     f3 far;
                     if (COUNT) nt++;
-                    if (hit_box(vmin, vmax, e, d, rd, far)) {
+                    if (hit_box(vmin, vmax, e, d, far)) {
                         float m = abs_metric(e, far);
                         if (m == 0.0f) {
                             bres = far;
@@ -966,7 +875,6 @@ rti_kernel(Tree t, Work work, uint32_t n, const uint32_t *n_dev, Ctrl *ctrl,
             ph = PH_TRAV;
             sp = lo = 0;
             if (work.finish(st, p_hit, p_vid, cur, org, cur, org, d)) {
-                rd = make_rcp(d);
                 if (COUNT) {
                     if (cur == 0) nr++;
                     else nl++;
