@@ -102,7 +102,8 @@ def batched_leg(ev0, ev1, stream, log):
         B = batch.Batch(list(zip(trees, first, counts)))
         return lambda: B.ray(so, sd, vox, None, hit)
 
-    space = ["chunk", "waves_per_cu", "stack_lds", "refill"]
+    space = ["chunk", "waves_per_cu", "stack_lds", "refill", "leaf_batch",
+             "finish_batch"]
     pcfg, pms, _ = autotune.autotune(plain_runner, params=space, reps=2)
     bcfg, bms, _ = autotune.autotune(batch_runner, params=space, reps=2)
     VT.set_launch_config(**bcfg)
@@ -292,6 +293,7 @@ def main():
 
     L = _lib.load()
     _lib.check(L.voxg_init(0), "voxg_init")
+    default_cfg = VT.get_launch_config()
     torch.cuda.set_device(0)
     stream = torch.cuda.current_stream()
 
@@ -314,7 +316,8 @@ def main():
         from voxvision_amd import autotune
         cfg, best_ms, hist = autotune.autotune(
             lambda cfg: step, params=["stack_lds", "waves_per_cu", "refill",
-                                      "block"], reps=2)
+                                      "block", "leaf_batch", "finish_batch"],
+            reps=2)
         tuned = {"config": cfg, "ms": round(best_ms, 3),
                  "candidates": len(hist)}
         log("[bench] autotuned render config %s (%.3f ms)" % (cfg, best_ms))
@@ -352,11 +355,14 @@ def main():
                  BYTES_PER_TEST * cnt["voxel_tests"])
     achieved = alg_bytes / (ms / 1e3) / 1e9
     tr = load_traffic("traffic_ct256_render.json")
+    # one render call = the block pre-pass (block_cull_kernel) + the
+    # traversal (rti_kernel); both are timed together by the events above
     roofline = {"bound": "hbm", "achieved": round(achieved, 2),
                 "peak": HBM_PEAK_GBS, "unit": "GB/s",
                 "frac": round(achieved / HBM_PEAK_GBS, 5),
                 "traffic": tr.get("hbm_bytes_per_launch") if tr else None,
                 "algorithmic_bytes_per_launch": alg_bytes,
+                "timed": "render call: block_cull_kernel + rti_kernel",
                 "node_visits_per_ray": round(cnt["node_visits"] / n, 4),
                 "voxel_tests_per_ray": round(cnt["voxel_tests"] / n, 4)}
 
@@ -377,6 +383,13 @@ def main():
                 G.handle, 1 << 24, *[t.data_ptr() for t in so + sd],
                 vo.data_ptr(), None, *[t.data_ptr() for t in ho],
                 C.c_void_p(stream.cuda_stream)), "voxg_ray_batch")
+        render_cfg = VT.get_launch_config()
+        if not args.no_tune:
+            from voxvision_amd import autotune
+            icfg, _, _ = autotune.autotune(
+                lambda cfg: inc, params=["leaf_batch", "finish_batch", "chunk",
+                                         "refill"], reps=2)
+            extra["incoherent_launch"] = icfg
         inc()
         torch.cuda.synchronize()
         ev0.record(stream)
@@ -386,6 +399,7 @@ def main():
         torch.cuda.synchronize()
         ims = ev0.elapsed_time(ev1) / 5
         extra["incoherent_16M_mrays_s"] = round((1 << 24) / ims / 1e3, 1)
+        VT.set_launch_config(**render_cfg)
         cb, rb = scenes.random_balls(1 << 22, bmin, bmax, seed=13)
         cbt = [torch.from_numpy(np.ascontiguousarray(cb[:, k])).cuda()
                for k in range(3)]
@@ -410,6 +424,7 @@ def main():
     if not args.no_batched:
         extra["batched"] = batched_leg(ev0, ev1, stream, log)
     if not args.no_serving:
+        VT.set_launch_config(**default_cfg)
         extra["serving"] = serving_leg(log, args.serve_seconds,
                                        args.serve_rate)
 
