@@ -18,6 +18,10 @@
  *   voxg_render             vox_render per-pixel loop src/voxrnd/renderer.h:208,
  *                           with screen2world of the simple camera
  *                           src/voxrnd/simple-camera.c:21-31
+ *   voxg_redraw             vox_redraw                src/voxrnd/renderer.h:169
+ *   voxg_cd_collide_batch   vox_cd_collide            src/voxrnd/cd.h:47
+ *   voxg_read_raw_data      vox_read_raw_data         src/voxtrees/datareader.h:35
+ *   voxg_raw_select         its sample loop           src/voxtrees/datareader.c:49-75
  */
 #ifndef VOXGPU_H
 #define VOXGPU_H
@@ -145,6 +149,20 @@ int voxg_ray_batch(const voxg_tree *tree, size_t n, const float *ox,
 int voxg_ball_batch(const voxg_tree *tree, size_t n, const float *cx,
                     const float *cy, const float *cz, const float *r,
                     uint8_t *out, hipStream_t stream);
+
+/* vox_cd_collide (src/voxrnd/cd.c:38-55) for n bodies against one tree.
+ * Body i is a ball of radius[i] at (px,py,pz)[i] that was last at
+ * (lx,ly,lz)[i].  If the two positions differ (vox_dot_equalp,
+ * voxvision.h.in:52: x, y, z all compare equal means "not moved") and the
+ * ball collides with the tree (vox_tree_ball_collidep), the body goes back
+ * to its last position; otherwise its last position becomes its position.
+ * Positions are updated in place; collided (may be NULL) gets 1 for a body
+ * moved back, else 0.  All pointers are device pointers.  The reference's
+ * gravity (vox_cd_gravity, cd.c:33-36) is a stub and has no counterpart. */
+int voxg_cd_collide_batch(const voxg_tree *tree, size_t n, float *px,
+                          float *py, float *pz, float *lx, float *ly,
+                          float *lz, const float *radius, uint8_t *collided,
+                          hipStream_t stream);
 
 /* Simple-camera descriptor (fields of struct vox_simple_camera,
  * simple-camera.c:10-17).  mul, xsub, ysub as set by set_window_size
@@ -295,6 +313,41 @@ int voxg_server_get_stats(voxg_server *s, voxg_server_stats *st);
 double voxg_server_now(const voxg_server *s);
 /* Waits for the work in flight, then frees the server (not the trees). */
 int voxg_server_destroy(voxg_server *s);
+
+/* vox_redraw (src/voxrnd/renderer.c:321-327): copy_squares
+ * (src/voxrnd/copy-helper.c:5-40) of n_frames frames from the square
+ * layout of voxg_render to row-major pixels,
+ *   pixels[f*W*H + y*W + x] = squares[f*W*H + ((y/4)*(W/4) + x/4)*16
+ *                                     + (y%4)*4 + x%4],
+ * then, when clear != 0, zeroes the squares as vox_redraw does, so the
+ * next frame starts black.  W % 16 == 0, H % 4 == 0; both buffers are
+ * 16-byte aligned device memory.  The SDL surface update is the caller's. */
+int voxg_redraw(uint32_t *squares, uint32_t *pixels, size_t n_frames, int W,
+                int H, int clear, hipStream_t stream);
+
+/* The sample loop of vox_read_raw_data (src/voxtrees/datareader.c:49-75)
+ * on the device: samples holds dim[0] x dim[1] x dim[2] little-endian
+ * samples of samplesize (1..4) bytes, i slowest and k fastest; the sample
+ * at (i, j, k) is selected when lo <= value < hi (the reference's
+ * callback, as used by its callers: `sample > t` is lo = t + 1, hi = 2^32;
+ * the Lua ranged reader is lo = min, hi = max).  Selected voxels' min
+ * corners (i*V0, j*V1, k*V2) go to xyz_out (packed xyz, device) in the
+ * reader's order, at most `capacity` of them; *count_out (device) gets the
+ * number selected.  xyz_out may be NULL to count only.  Fewer than 2^32
+ * samples.  voxel = vox_voxel (NULL = {1,1,1}). */
+int voxg_raw_select(const void *samples, const unsigned dim[3],
+                    unsigned samplesize, uint64_t lo, uint64_t hi,
+                    const float voxel[3], float *xyz_out, size_t capacity,
+                    unsigned long long *count_out, hipStream_t stream);
+
+/* vox_read_raw_data (datareader.c:11-85) with the test lo <= value < hi:
+ * reads the file (its size must be dim[0]*dim[1]*dim[2]*samplesize,
+ * "Wrong size of dataset" otherwise), selects the voxels on the device
+ * (voxg_raw_select), builds the tree on the host (voxg_tree_build) and
+ * uploads it.  Synchronous. */
+int voxg_read_raw_data(const char *filename, const unsigned dim[3],
+                       unsigned samplesize, uint64_t lo, uint64_t hi,
+                       const float voxel[3], voxg_tree **out);
 
 /* Launch configuration of the traversal kernels (autotuned by
  * voxvision_amd.autotune; defaults are sane).

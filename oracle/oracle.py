@@ -112,6 +112,14 @@ def lib():
             F] * 4 + [C.POINTER(C.c_uint8), C.c_int]
         L.orc_cpu_count.restype = C.c_int
         L.orc_check_tree.argtypes = [P]
+        U32 = C.POINTER(C.c_uint32)
+        L.orc_copy_squares.argtypes = [U32, U32, C.c_uint, C.c_uint]
+        L.orc_cd_collide.argtypes = [P, C.c_size_t, F, F, F,
+                                     C.POINTER(C.c_uint8)]
+        L.orc_read_raw.argtypes = [C.POINTER(C.c_uint8),
+                                   C.POINTER(C.c_uint), C.c_uint,
+                                   C.c_uint64, C.c_uint64, F]
+        L.orc_read_raw.restype = C.c_size_t
         _lib = L
     return _lib
 
@@ -192,6 +200,19 @@ class Tree:
             return vox, hit, (cnt.node_visits, cnt.voxel_tests)
         return vox, hit
 
+    def cd_collide(self, pos, last, radius):
+        """vox_cd_collide for n bodies (cd.c:38-55): returns (new
+        positions, new last positions, collided flags)."""
+        set_voxel(self.voxel)
+        pos = _f32(pos).reshape(-1, 3).copy()
+        last = _f32(last).reshape(-1, 3).copy()
+        radius = _f32(radius).reshape(-1)
+        out = np.zeros(len(pos), np.uint8)
+        lib().orc_cd_collide(self.ptr, len(pos), _fp(pos), _fp(last),
+                             _fp(radius),
+                             out.ctypes.data_as(C.POINTER(C.c_uint8)))
+        return pos, last, out
+
     def balls(self, c, r, nthreads=0, with_counts=False):
         set_voxel(self.voxel)
         c = _f32(c).reshape(-1, 3)
@@ -249,6 +270,31 @@ def camera_struct(cam):
     c.xsub = float(cam["xsub"])
     c.ysub = float(cam["ysub"])
     return c
+
+
+def copy_squares(squares, W, H):
+    """copy_squares (copy-helper.c:5-40) of one W x H frame."""
+    sq = np.ascontiguousarray(squares, np.uint32).reshape(-1)
+    px = np.zeros(W * H, np.uint32)
+    U32 = C.POINTER(C.c_uint32)
+    lib().orc_copy_squares(sq.ctypes.data_as(U32), px.ctypes.data_as(U32),
+                           W // 4, H // 4)
+    return px.reshape(H, W)
+
+
+def read_raw(data, dim, samplesize, lo, hi, voxel=(1.0, 1.0, 1.0)):
+    """Sample loop of vox_read_raw_data (datareader.c:49-75) with the
+    test lo <= sample < hi; returns the selected voxel corners (m, 3)."""
+    set_voxel(voxel)
+    data = np.ascontiguousarray(np.frombuffer(bytes(data), np.uint8)
+                                if not isinstance(data, np.ndarray)
+                                else data.view(np.uint8).reshape(-1))
+    d = (C.c_uint * 3)(*[int(x) for x in dim])
+    n = int(dim[0]) * int(dim[1]) * int(dim[2])
+    out = np.zeros((max(n, 1), 3), np.float32)
+    m = lib().orc_read_raw(data.ctypes.data_as(C.POINTER(C.c_uint8)), d,
+                           int(samplesize), int(lo), int(hi), _fp(out))
+    return out[:m].copy()
 
 
 def set_window_size(fov, w, h):

@@ -1138,3 +1138,67 @@ int orc_check_tree(const struct orc_node *tree)
         return orc_voxels_in_tree(tree) == sum ? 0 : 5;
     }
 }
+
+/* ------------------------------------------------------------------ */
+/* src/voxrnd/copy-helper.c:5-40 -- square layout to pixel rows        */
+/* ------------------------------------------------------------------ */
+void orc_copy_squares(const uint32_t *squares, uint32_t *pixels, unsigned ws,
+                      unsigned hs)
+{
+    /* square (sx, sy) holds pixel rows 4 sy .. 4 sy + 3 and columns
+     * 4 sx .. 4 sx + 3, four pixels per row in order */
+    unsigned w = ws * 4, sy, sx, line;
+    for (sy = 0; sy < hs; sy++)
+        for (sx = 0; sx < ws; sx++)
+            for (line = 0; line < 4; line++)
+                memcpy(pixels + (size_t)(4 * sy + line) * w + 4 * sx,
+                       squares + ((size_t)sy * ws + sx) * 16 + 4 * line,
+                       4 * sizeof(uint32_t));
+}
+
+/* ------------------------------------------------------------------ */
+/* src/voxrnd/cd.c:38-55 -- collision of moving bodies with the scene  */
+/* ------------------------------------------------------------------ */
+void orc_cd_collide(const struct orc_node *tree, size_t n, float *pos,
+                    float *last, const float *radius, uint8_t *collided)
+{
+    size_t i;
+    for (i = 0; i < n; i++) {
+        float *p = pos + 3 * i, *l = last + 3 * i;
+        orc_dot c = {p[0], p[1], p[2], 0.0f};
+        collided[i] = 0;
+        /* vox_dot_equalp: all of x, y, z compare equal */
+        if (p[0] == l[0] && p[1] == l[1] && p[2] == l[2]) continue;
+        if (orc_tree_ball_collidep(tree, c, radius[i])) {
+            memcpy(p, l, 3 * sizeof(float));
+            collided[i] = 1;
+        } else {
+            memcpy(l, p, 3 * sizeof(float));
+        }
+    }
+}
+
+/* ------------------------------------------------------------------ */
+/* src/voxtrees/datareader.c:49-75 -- samples to voxel corners          */
+/* ------------------------------------------------------------------ */
+size_t orc_read_raw(const uint8_t *data, const unsigned dim[3],
+                    unsigned samplesize, uint64_t lo, uint64_t hi,
+                    float *xyz_out)
+{
+    unsigned i, j, k, l;
+    size_t pos = 0, counter = 0;
+    for (i = 0; i < dim[0]; i++)
+        for (j = 0; j < dim[1]; j++)
+            for (k = 0; k < dim[2]; k++) {
+                uint32_t value = 0;
+                for (l = 0; l < samplesize; l++)
+                    value |= (uint32_t)data[pos++] << (8 * l);
+                if (value >= lo && value < hi) {
+                    xyz_out[3 * counter + 0] = i * orc_voxel[0];
+                    xyz_out[3 * counter + 1] = j * orc_voxel[1];
+                    xyz_out[3 * counter + 2] = k * orc_voxel[2];
+                    counter++;
+                }
+            }
+    return counter;
+}

@@ -196,6 +196,28 @@ int orc_check_tree(const struct orc_node *tree);
 /* Number of online CPUs usable by this process (sched_getaffinity). */
 int orc_cpu_count(void);
 
+/* ---- around the render loop and the tree ---- */
+/* copy_squares (copy-helper.c:5-40) for one frame: 4x4 squares (16
+ * uint32 each, ws x hs of them) to a row-major (4 ws) x (4 hs) image. */
+void orc_copy_squares(const uint32_t *squares, uint32_t *pixels, unsigned ws,
+                      unsigned hs);
+/* vox_cd_collide (cd.c:38-55) for n bodies, one after the other: a body
+ * whose position differs from its last position (vox_dot_equalp,
+ * voxvision.h.in:52) is moved back when a ball of its radius collides with
+ * the tree, else its last position becomes the position.  pos and last
+ * are n x 3, updated in place; collided[i] = 1 when body i was moved
+ * back. */
+void orc_cd_collide(const struct orc_node *tree, size_t n, float *pos,
+                    float *last, const float *radius, uint8_t *collided);
+/* The sample loop of vox_read_raw_data (datareader.c:49-75) over an
+ * in-memory volume of dim[0] x dim[1] x dim[2] little-endian samples of
+ * samplesize bytes, with the test lo <= sample < hi.  Writes the selected
+ * voxels' min corners (i*V0, j*V1, k*V2) in the reader's order into
+ * xyz_out (n x 3, room for every sample) and returns their number. */
+size_t orc_read_raw(const uint8_t *data, const unsigned dim[3],
+                    unsigned samplesize, uint64_t lo, uint64_t hi,
+                    float *xyz_out);
+
 #ifdef __cplusplus
 }
 #endif

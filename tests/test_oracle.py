@@ -383,3 +383,75 @@ def test_render_adaptive_modes_consistent(oracle):
         assert np.array_equal(va != oracle.MISS, vb != oracle.MISS) or \
             ((va != oracle.MISS) & (vb == oracle.MISS)).sum() == 0
         assert st["root_traversals"] + st["leaf_traversals"] > 0
+
+
+def test_copy_squares_matches_square_layout(oracle):
+    """copy_squares (copy-helper.c:5-40) against the square layout of
+    renderer.h:14 written as a reshape: square (sx, sy) holds rows 4sy..4sy+3
+    and columns 4sx..4sx+3, four pixels per row."""
+    rng = np.random.default_rng(0)
+    for W, H in ((16, 4), (32, 8), (48, 12)):
+        sq = rng.integers(0, 2 ** 32, W * H, dtype=np.uint64).astype(np.uint32)
+        px = oracle.copy_squares(sq, W, H)
+        ref = sq.reshape(H // 4, W // 4, 4, 4).transpose(0, 2, 1, 3)
+        assert np.array_equal(px, ref.reshape(H, W))
+    # pixel (x, y) = (5, 6) of a 16x8 frame lives in square 1*4 + 1, slot 2*4+1
+    sq = np.arange(16 * 8, dtype=np.uint32)
+    assert oracle.copy_squares(sq, 16, 8)[6, 5] == (1 * 4 + 1) * 16 + 2 * 4 + 1
+
+
+def test_read_raw_matches_numpy_selection(oracle):
+    """The sample loop of vox_read_raw_data (datareader.c:49-75): little
+    endian samples, i slowest, corners i*V0, j*V1, k*V2 in raster order."""
+    rng = np.random.default_rng(1)
+    dim = (5, 7, 9)
+    n = dim[0] * dim[1] * dim[2]
+    V = (1.0, 0.5, 2.5)
+    for ss in (1, 2, 3, 4):
+        raw = rng.integers(0, 256, n * ss).astype(np.uint8)
+        vals = np.zeros(n, np.uint64)
+        for l in range(ss):
+            vals |= raw[l::ss].astype(np.uint64) << np.uint64(8 * l)
+        top = 1 << (8 * ss)
+        for lo, hi in ((top // 2, top), (0, top), (top // 3, top // 3 + 5),
+                       (top, top)):
+            got = oracle.read_raw(raw.tobytes(), dim, ss, lo, hi, voxel=V)
+            sel = np.nonzero((vals >= lo) & (vals < hi))[0]
+            i, j, k = np.unravel_index(sel, dim)
+            want = np.stack([i.astype(np.float32) * np.float32(V[0]),
+                             j.astype(np.float32) * np.float32(V[1]),
+                             k.astype(np.float32) * np.float32(V[2])], 1)
+            assert got.shape == want.shape
+            assert np.array_equal(got, want)
+    # a 2-byte sample 0x0180 = 384: selected by [384, 385), not by [0, 384)
+    raw = np.array([0x80, 0x01], np.uint8).tobytes()
+    assert len(oracle.read_raw(raw, (1, 1, 1), 2, 384, 385)) == 1
+    assert len(oracle.read_raw(raw, (1, 1, 1), 2, 0, 384)) == 0
+
+
+def test_cd_collide_rules(oracle):
+    """vox_cd_collide (cd.c:38-55): an unmoved body (equal positions, -0 ==
+    +0) is left alone even inside the scene; a moved body whose ball hits a
+    voxel goes back; a free moved body's last position follows it.  The
+    ball answer is checked against the fp32 brute force."""
+    dots = np.array([[0, 0, 0], [1, 0, 0], [5, 5, 5]], np.float32)
+    T = oracle.Tree(dots)
+    pos = np.array([[0.5, 0.5, 0.5],     # unmoved, inside a voxel
+                    [-0.0, 3.0, 3.0],    # unmoved: -0 == +0
+                    [2.5, 0.5, 0.5],     # moved, 0.5 from voxel (1,0,0)
+                    [10.0, 10.0, 10.0],  # moved, free
+                    [3.0, 3.0, 3.0]], np.float32)
+    last = np.array([[0.5, 0.5, 0.5], [0.0, 3.0, 3.0], [9.0, 9.0, 9.0],
+                     [12.0, 12.0, 12.0], [3.0, 3.0, 2.0]], np.float32)
+    r = np.array([1.0, 1.0, 0.75, 1.0, 3.5], np.float32)
+    p2, l2, hit = T.cd_collide(pos, last, r)
+    assert hit.tolist() == [0, 0, 1, 0, 1]
+    assert np.array_equal(p2[0], pos[0]) and np.array_equal(l2[0], last[0])
+    assert np.array_equal(p2[2], last[2]) and np.array_equal(l2[2], last[2])
+    assert np.array_equal(p2[3], pos[3]) and np.array_equal(l2[3], pos[3])
+    rng = np.random.default_rng(2)
+    c = rng.uniform(-2, 8, (500, 3)).astype(np.float32)
+    rr = rng.uniform(0.1, 2.0, 500).astype(np.float32)
+    far = c + 100.0
+    _, _, hit = T.cd_collide(c, far, rr)
+    assert np.array_equal(hit, oracle.brute_balls(dots, c, rr))

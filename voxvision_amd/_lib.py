@@ -28,7 +28,8 @@ EXPORTS = [
     "voxg_batch_ray", "voxg_batch_release", "voxg_server_create",
     "voxg_server_add_tree", "voxg_server_submit_render",
     "voxg_server_submit_balls", "voxg_server_wait", "voxg_server_get_stats",
-    "voxg_server_now", "voxg_server_destroy",
+    "voxg_server_now", "voxg_server_destroy", "voxg_cd_collide_batch",
+    "voxg_redraw", "voxg_raw_select", "voxg_read_raw_data",
 ]
 
 
@@ -125,6 +126,12 @@ def load():
     L.voxg_batch_create.argtypes = [S, C.POINTER(BatchItem), C.POINTER(P)]
     L.voxg_batch_ray.argtypes = [P] + [P] * 6 + [P] * 5 + [P]
     L.voxg_batch_release.argtypes = [P]
+    L.voxg_cd_collide_batch.argtypes = [P, S] + [P] * 8 + [P]
+    L.voxg_redraw.argtypes = [P, P, S, I, I, I, P]
+    U64 = C.c_uint64
+    L.voxg_raw_select.argtypes = [P, P, C.c_uint, U64, U64, P, P, S, P, P]
+    L.voxg_read_raw_data.argtypes = [C.c_char_p, P, C.c_uint, U64, U64, P,
+                                     C.POINTER(P)]
     _lib = L
     return L
 

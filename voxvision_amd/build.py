@@ -18,6 +18,7 @@ OUT = os.path.join(PKG, "libvoxgpu.so")
 BUILD = os.path.join(PKG, "_build")
 
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
+HIP_SOURCES = ["voxgpu.hip", "volume.hip"]
 ARCH = "gfx950"
 
 HIP_FLAGS = [
@@ -49,11 +50,14 @@ def build(verbose=False):
     log = _run([cxx] + CXX_FLAGS + ["-c", os.path.join(CSRC, "tree_build.cpp"),
                                      "-o", obj])
     tmp = OUT + ".tmp"
-    hobj = os.path.join(BUILD, "voxgpu.o")
-    log += _run([HIPCC] + HIP_FLAGS + [
-        "-c", os.path.join(CSRC, "voxgpu.hip"), "-o", hobj])
+    hobjs = []
+    for name in HIP_SOURCES:
+        hobj = os.path.join(BUILD, name.replace(".hip", ".o"))
+        log += _run([HIPCC] + HIP_FLAGS + [
+            "-c", os.path.join(CSRC, name), "-o", hobj])
+        hobjs.append(hobj)
     log += _run([HIPCC, "--offload-arch=" + ARCH, "-shared", "-o", tmp,
-                 "-x", "none", hobj, obj, "-lpthread"])
+                 "-x", "none"] + hobjs + [obj, "-lpthread"])
     os.replace(tmp, OUT)
     if verbose and log:
         print(log)

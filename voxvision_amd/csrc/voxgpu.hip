@@ -39,6 +39,7 @@
 #include <vector>
 
 #include "tree_build.h"
+#include "voxg_internal.h"
 #include "voxg_layout.h"
 
 using namespace voxg;
@@ -986,6 +987,41 @@ block_cull_kernel(RenderItem it, uint32_t npix, Ctrl *ctrl,
 // ball kernel (search.c:155-183): any-hit depth-first walk over children
 // 0..7 with escape links; the answer does not depend on the order.
 // ---------------------------------------------------------------------------
+// search.c:155-183: is there a voxel whose box comes closer than r to c
+// (rr = r * r)?  Any-hit depth-first walk over children 0..7 through the
+// escape links; the answer does not depend on the order.
+template <bool COUNT>
+__device__ __forceinline__ uint8_t ball_collides(const Tree &t, f3 c, float rr,
+                                                 unsigned long long &nv,
+                                                 unsigned long long &nt) {
+    const f3 V = f3{t.vx, t.vy, t.vz};
+    uint32_t node = t.empty ? kEnd : 0u;
+    while (node != kEnd) {
+        Node nd = load_node(t.nodes, node);
+        if (COUNT) nv++;
+        if (box_ball(nd.bmin, nd.bmax, c, rr)) {
+            uint32_t type = nd.info & NT_TYPE_MASK;
+            if (type == NT_DENSE) return 1;
+            if (type == NT_LEAF) {
+                uint32_t cnt = nd.info >> 8;
+                for (uint32_t k = 0; k < cnt; k++) {
+                    const float *p = t.dots + 3u * (nd.ref + k);
+                    f3 vmin = f3{p[0], p[1], p[2]};
+                    f3 vmax = f3{vmin.x + V.x, vmin.y + V.y, vmin.z + V.z};
+                    if (COUNT) nt++;
+                    if (box_ball(vmin, vmax, c, rr)) return 1;
+                }
+                node = nd.escape;
+            } else {
+                node = (nd.ref != kEnd) ? nd.ref : nd.escape;
+            }
+        } else {
+            node = nd.escape;
+        }
+    }
+    return 0;
+}
+
 struct BallItem {
     Tree t;
     const float *cx, *cy, *cz, *r;
@@ -1011,44 +1047,10 @@ ball_kernel(BallItem one, const BallItem *__restrict__ items,
             itp = &items[lo];
         }
         const BallItem &it = *itp;
-        const Tree &t = it.t;
         uint32_t k0 = i - it.begin;
         f3 c = f3{it.cx[k0], it.cy[k0], it.cz[k0]};
         float r = it.r[k0];
-        float rr = r * r;
-        const f3 V = f3{t.vx, t.vy, t.vz};
-        uint8_t res = 0;
-        uint32_t node = t.empty ? kEnd : 0u;
-        while (node != kEnd) {
-            Node nd = load_node(t.nodes, node);
-            if (COUNT) nv++;
-            if (box_ball(nd.bmin, nd.bmax, c, rr)) {
-                uint32_t type = nd.info & NT_TYPE_MASK;
-                if (type == NT_DENSE) {
-                    res = 1;
-                    break;
-                }
-                if (type == NT_LEAF) {
-                    uint32_t cnt = nd.info >> 8;
-                    for (uint32_t k = 0; k < cnt; k++) {
-                        const float *p = t.dots + 3u * (nd.ref + k);
-                        f3 vmin = f3{p[0], p[1], p[2]};
-                        f3 vmax = f3{vmin.x + V.x, vmin.y + V.y, vmin.z + V.z};
-                        if (COUNT) nt++;
-                        if (box_ball(vmin, vmax, c, rr)) {
-                            res = 1;
-                            break;
-                        }
-                    }
-                    if (res) break;
-                    node = nd.escape;
-                } else {
-                    node = (nd.ref != kEnd) ? nd.ref : nd.escape;
-                }
-            } else {
-                node = nd.escape;
-            }
-        }
+        uint8_t res = ball_collides<COUNT>(it.t, c, r * r, nv, nt);
         if (it.out) it.out[k0] = res;
     }
     if (COUNT) {
@@ -1056,6 +1058,38 @@ ball_kernel(BallItem one, const BallItem *__restrict__ items,
         atomicAdd(&ctrl->tests, nt);
         if (i < n) atomicAdd(&ctrl->rays, 1ull);
     }
+}
+
+// cd.c:38-55 for n bodies: a body that moved (vox_dot_equalp,
+// voxvision.h.in:52: x, y and z all compare equal) and whose ball collides
+// with the scene goes back to its last position; otherwise its last
+// position becomes its position.
+__global__ void __launch_bounds__(256)
+collide_kernel(Tree t, uint32_t n, float *__restrict__ px,
+               float *__restrict__ py, float *__restrict__ pz,
+               float *__restrict__ lx, float *__restrict__ ly,
+               float *__restrict__ lz, const float *__restrict__ radius,
+               uint8_t *__restrict__ collided) {
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const f3 p = f3{px[i], py[i], pz[i]};
+    const f3 l = f3{lx[i], ly[i], lz[i]};
+    uint8_t moved_back = 0;
+    if (!(p.x == l.x && p.y == l.y && p.z == l.z)) {
+        const float r = radius[i];
+        unsigned long long nv = 0, nt = 0;
+        if (ball_collides<false>(t, p, r * r, nv, nt)) {
+            px[i] = l.x;
+            py[i] = l.y;
+            pz[i] = l.z;
+            moved_back = 1;
+        } else {
+            lx[i] = p.x;
+            ly[i] = p.y;
+            lz[i] = p.z;
+        }
+    }
+    if (collided) collided[i] = moved_back;
 }
 
 }  // namespace dev
@@ -1097,6 +1131,8 @@ struct Workspace {
     size_t spill_bytes = 0;
     uint32_t *queue = nullptr;  // block queue of the render pre-pass
     size_t queue_bytes = 0;
+    void *aux = nullptr;        // scratch of the other translation units
+    size_t aux_bytes = 0;
 };
 std::map<std::pair<int, hipStream_t>, std::unique_ptr<Workspace>> g_ws;
 
@@ -1158,6 +1194,30 @@ int get_queue(Workspace *ws, size_t bytes) {
     }
     return 0;
 }
+
+}  // namespace
+
+namespace voxg_internal {
+int fail(int code, const std::string &msg) { return ::fail(code, msg); }
+
+int scratch(hipStream_t stream, size_t bytes, void **out) {
+    Workspace *ws = nullptr;
+    int rc = get_ws(stream, 16, &ws);
+    if (rc) return rc;
+    std::lock_guard<std::mutex> g(g_mu);
+    if (bytes > ws->aux_bytes) {
+        if (ws->aux) HIPCHK(hipFree(ws->aux));
+        ws->aux = nullptr;
+        ws->aux_bytes = 0;
+        HIPCHK(hipMalloc(&ws->aux, bytes));
+        ws->aux_bytes = bytes;
+    }
+    *out = ws->aux;
+    return 0;
+}
+}  // namespace voxg_internal
+
+namespace {
 
 // 1/v when v is a power of two (then x * (1/v) == x / v exactly), else 0.
 float pow2_inverse(float v) {
@@ -1602,6 +1662,25 @@ int voxg_ball_batch(const voxg_tree *t, size_t n, const float *cx,
                     uint8_t *out, hipStream_t stream) {
     if (n && !out) return fail(VOXG_EINVAL, "out is NULL");
     return ball_launch(t, n, cx, cy, cz, r, out, nullptr, stream);
+}
+
+int voxg_cd_collide_batch(const voxg_tree *t, size_t n, float *px, float *py,
+                          float *pz, float *lx, float *ly, float *lz,
+                          const float *radius, uint8_t *collided,
+                          hipStream_t stream) {
+    int rc = check_tree(t);
+    if (rc) return rc;
+    if (n >= 0x80000000ull) return fail(VOXG_ERANGE, "n must be < 2^31");
+    if (n == 0) return 0;
+    if (!px || !py || !pz || !lx || !ly || !lz || !radius)
+        return fail(VOXG_EINVAL, "body pointer is NULL");
+    const uint32_t block = 256;
+    const uint32_t grid = (uint32_t)((n + block - 1) / block);
+    hipLaunchKernelGGL(dev::collide_kernel, dim3(grid), dim3(block), 0, stream,
+                       dev_tree(t), (uint32_t)n, px, py, pz, lx, ly, lz,
+                       radius, collided);
+    HIPCHK(hipGetLastError());
+    return 0;
 }
 
 int voxg_ball_batch_count(const voxg_tree *t, size_t n, const float *cx,

@@ -79,6 +79,37 @@ def render_counts(tree, cams, W, H, quality=QUALITY_BEST, stream=None):
     return {k: getattr(c, k) for k, _ in c._fields_}
 
 
+def redraw(squares, W, H, pixels=None, clear=True, stream=None):
+    """vox_redraw (renderer.c:321-327): copy the square-layout frames of
+    `squares` (device uint32/int32, n_frames*W*H) to row-major pixels
+    (returned as an (n_frames, H, W) device tensor) and, when clear, zero
+    the squares for the next frame."""
+    L = _lib.load()
+    n = squares.numel()
+    nf = n // (W * H)
+    if pixels is None:
+        pixels = torch.empty((nf, H, W), dtype=squares.dtype, device="cuda")
+    check(L.voxg_redraw(squares.data_ptr(), pixels.data_ptr(), nf, W, H,
+                        1 if clear else 0, _stream(stream)), "voxg_redraw")
+    return pixels
+
+
+def cd_collide(tree, pos, last, radius, collided=None, stream=None):
+    """vox_cd_collide (cd.c:38-55) for many bodies: pos and last are lists
+    of three device float tensors (x, y, z), updated in place; radius a
+    device float tensor.  Returns the device uint8 tensor of bodies moved
+    back to their last position."""
+    L = _lib.load()
+    n = radius.numel()
+    if collided is None:
+        collided = torch.empty(n, dtype=torch.uint8, device="cuda")
+    check(L.voxg_cd_collide_batch(
+        tree.handle, n, *[t.data_ptr() for t in list(pos) + list(last)],
+        radius.data_ptr(), collided.data_ptr(), _stream(stream)),
+        "voxg_cd_collide_batch")
+    return collided
+
+
 def squares_to_image(squares, W, H):
     """Host helper: square layout -> (H, W) image (copy_squares,
     src/voxrnd/copy-helper.c:5-40)."""

@@ -208,6 +208,53 @@ def ball_counts(tree, c, r, stream=None):
                 rays=cnt.rays)
 
 
+def _raw_range(samplesize, lo, hi, threshold):
+    top = 1 << (8 * samplesize)
+    if threshold is not None:  # the demo's test: sample > threshold
+        return int(threshold) + 1, top
+    # the Lua ranged reader's defaults (voxengine/modules/voxtrees.c:313-316)
+    lo = (1 << (8 * samplesize - 1)) if lo is None else int(lo)
+    hi = top if hi is None else int(hi)
+    return lo, hi
+
+
+def raw_select(samples, dim, samplesize, lo=None, hi=None, threshold=None,
+               voxel=(1.0, 1.0, 1.0), stream=None):
+    """Sample loop of vox_read_raw_data (datareader.c:49-75) on the device:
+    samples is a device uint8 tensor of dim[0]*dim[1]*dim[2]*samplesize
+    bytes; the test is lo <= sample < hi (or sample > threshold).  Returns
+    the selected voxel corners as a device (m, 3) float tensor, in the
+    reader's order."""
+    L = _lib.load()
+    lo, hi = _raw_range(samplesize, lo, hi, threshold)
+    d = (C.c_uint * 3)(*[int(x) for x in dim])
+    v = (C.c_float * 3)(*voxel)
+    cnt = torch.zeros(1, dtype=torch.int64, device="cuda")
+    st = _stream(stream)
+    check(L.voxg_raw_select(samples.data_ptr(), d, samplesize, lo, hi, v,
+                            None, 0, cnt.data_ptr(), st), "voxg_raw_select")
+    m = int(cnt.item())
+    xyz = torch.empty((m, 3), dtype=torch.float32, device="cuda")
+    if m:
+        check(L.voxg_raw_select(samples.data_ptr(), d, samplesize, lo, hi, v,
+                                xyz.data_ptr(), m, cnt.data_ptr(), st),
+              "voxg_raw_select")
+    return xyz
+
+
+def read_raw_data(filename, dim, samplesize, lo=None, hi=None, threshold=None,
+                  voxel=(1.0, 1.0, 1.0)):
+    """vox_read_raw_data (datareader.c:11-85): file -> device tree."""
+    L = _lib.load()
+    lo, hi = _raw_range(samplesize, lo, hi, threshold)
+    d = (C.c_uint * 3)(*[int(x) for x in dim])
+    v = (C.c_float * 3)(*voxel)
+    h = C.c_void_p()
+    check(L.voxg_read_raw_data(str(filename).encode(), d, samplesize, lo, hi,
+                               v, C.byref(h)), "voxg_read_raw_data")
+    return Tree(h, voxel)
+
+
 def set_launch_config(**fields):
     """Change fields of the traversal launch configuration
     (voxg_launch_config: stack_lds, block, waves_per_cu, refill, chunk,
