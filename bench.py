@@ -134,6 +134,80 @@ def batched_leg(ev0, ev1, stream, log):
             "plain_config": pcfg, "batched_config": bcfg}
 
 
+def aux_leg(G, out, W, H, nf, ev0, ev1, stream, log):
+    """The calls on either side of the path: vox_redraw's square-to-row
+    copy of the headline frames (with the clear), vox_cd_collide for 4 M
+    bodies, and vox_read_raw_data's selection over a 512^3 one-byte
+    volume (8% selected).  HBM bytes: redraw 12 B per pixel (read 4, write
+    4 + clear 4); selection 2 x 1 B per sample read + 12 B per selected
+    voxel written."""
+    import numpy as np
+    import torch
+    from voxvision_amd import voxrnd, voxtrees as VT
+
+    def timed(fn, reps=5):
+        fn()
+        torch.cuda.synchronize()
+        ev0.record(stream)
+        for _ in range(reps):
+            fn()
+        ev1.record(stream)
+        torch.cuda.synchronize()
+        return ev0.elapsed_time(ev1) / reps
+
+    res = {}
+    npx = nf * W * H
+    sq = torch.randint(0, 1 << 30, (npx,), dtype=torch.int32, device="cuda")
+    px = torch.empty((nf, H, W), dtype=torch.int32, device="cuda")
+    ms = timed(lambda: voxrnd.redraw(sq, W, H, pixels=px, clear=True))
+    res["redraw_64x1080p_ms"] = round(ms, 4)
+    res["redraw_gb_s"] = round(12.0 * npx / ms / 1e6, 1)
+    del sq, px
+    bmin, bmax = np.array(G.info["bbox_min"]), np.array(G.info["bbox_max"])
+    nb = 1 << 22
+    g = torch.Generator(device="cuda").manual_seed(17)
+    lo = torch.tensor(bmin, device="cuda", dtype=torch.float32)
+    ext = torch.tensor(bmax - bmin, device="cuda", dtype=torch.float32)
+    pos0 = [lo[k] + ext[k] * torch.rand(nb, device="cuda", generator=g)
+            for k in range(3)]
+    last0 = [p + 1.0 for p in pos0]
+    rad = 0.5 + 7.5 * torch.rand(nb, device="cuda", generator=g)
+    P = [p.clone() for p in pos0]
+    Lst = [p.clone() for p in last0]
+    hit = torch.empty(nb, dtype=torch.uint8, device="cuda")
+
+    def reset():  # every body moved again, so every body is tested
+        for a, b in zip(P, pos0):
+            a.copy_(b)
+        for a, b in zip(Lst, last0):
+            a.copy_(b)
+
+    def collide():
+        reset()
+        voxrnd.cd_collide(G, P, Lst, rad, collided=hit)
+    ms = timed(collide) - timed(reset)
+    res["cd_collide_4M_ms"] = round(ms, 4)
+    res["cd_collide_mbodies_s"] = round(nb / ms / 1e3, 1)
+    res["cd_collide_moved_back"] = round(float(hit.float().mean()), 4)
+    dim = (512, 512, 512)
+    vol = torch.randint(0, 256, dim, dtype=torch.uint8, device="cuda",
+                        generator=g)
+    flat = vol.reshape(-1)
+    xyz = [None]
+
+    def select():
+        xyz[0] = VT.raw_select(flat, dim, 1, lo=235, hi=256)
+    ms = timed(select, reps=3)
+    m = xyz[0].shape[0]
+    res["raw_select_512cube_ms"] = round(ms, 4)
+    res["raw_select_gsamples_s"] = round(flat.numel() / ms / 1e6, 1)
+    res["raw_select_selected"] = int(m)
+    res["raw_select_gb_s"] = round((2.0 * flat.numel() + 12.0 * m) / ms / 1e6,
+                                   1)
+    log("[bench] around the path: %s" % json.dumps(res))
+    return res
+
+
 def serving_leg(log, seconds, rate, seed=31):
     """Serving: 8 resident 256^3 trees (CT-like, different seeds); a
     seeded Poisson stream of requests, 60% renders (320x240 .. 3840x2160,
@@ -421,6 +495,9 @@ def main():
         bms = ev0.elapsed_time(ev1) / 5
         extra["balls_4M_mqueries_s"] = round((1 << 22) / bms / 1e3, 1)
 
+    if not args.quick:
+        extra["around_the_path"] = aux_leg(G, out, W, H, nf, ev0, ev1,
+                                           stream, log)
     if not args.no_batched:
         extra["batched"] = batched_leg(ev0, ev1, stream, log)
     if not args.no_serving:
