@@ -120,6 +120,15 @@ def lib():
                                    C.POINTER(C.c_uint), C.c_uint,
                                    C.c_uint64, C.c_uint64, F]
         L.orc_read_raw.restype = C.c_size_t
+        L.orc_insert_voxel.argtypes = [C.POINTER(P), F]
+        L.orc_delete_voxel.argtypes = [C.POINTER(P), F]
+        L.orc_rebuild_tree.argtypes = [P]
+        L.orc_rebuild_tree.restype = P
+        L.orc_make_dense_leaf_box.argtypes = [F, F]
+        L.orc_make_dense_leaf_box.restype = P
+        L.orc_tree_flags.argtypes = [P]
+        L.orc_voxels_in_tree.argtypes = [P]
+        L.orc_voxels_in_tree.restype = C.c_size_t
         _lib = L
     return _lib
 
@@ -152,9 +161,56 @@ class Tree:
         buf[:, :3] = d
         self.n = len(d)
         self.ptr = L.orc_make_tree_from(_fp(buf), len(d)) if len(d) else None
+        self._restat()
+
+    def _restat(self):
         st = _Stats()
-        L.orc_tree_stats_get(self.ptr, C.byref(st))
+        lib().orc_tree_stats_get(self.ptr, C.byref(st))
         self.stats = {k: getattr(st, k) for k, _ in _Stats._fields_}
+
+    @classmethod
+    def empty(cls, voxel=(1.0, 1.0, 1.0)):
+        return cls(np.zeros((0, 3), np.float32), voxel)
+
+    @classmethod
+    def dense_box(cls, mn, mx, voxel=(1.0, 1.0, 1.0)):
+        """A tree that is one dense leaf (vox_make_dense_leaf)."""
+        t = cls.empty(voxel)
+        t.ptr = lib().orc_make_dense_leaf_box(_fp(_f32(mn)), _fp(_f32(mx)))
+        t._restat()
+        return t
+
+    def _edit(self, fn, xyz):
+        set_voxel(self.voxel)
+        p = C.c_void_p(self.ptr)
+        r = fn(C.byref(p), _fp(_f32(xyz).reshape(3)))
+        self.ptr = p.value
+        self._restat()
+        return int(r)
+
+    def insert(self, xyz):
+        """vox_insert_voxel (tree.c:608-616): 1 when added."""
+        return self._edit(lib().orc_insert_voxel, xyz)
+
+    def delete(self, xyz):
+        """vox_delete_voxel (tree.c:820-828): 1 when removed."""
+        return self._edit(lib().orc_delete_voxel, xyz)
+
+    def rebuild(self):
+        """vox_rebuild_tree (tree.c:422-432) as a new Tree."""
+        set_voxel(self.voxel)
+        t = Tree.empty(self.voxel)
+        t.ptr = lib().orc_rebuild_tree(self.ptr)
+        t._restat()
+        return t
+
+    @property
+    def flags(self):
+        return int(lib().orc_tree_flags(self.ptr))
+
+    @property
+    def voxels(self):
+        return int(lib().orc_voxels_in_tree(self.ptr)) if self.ptr else 0
 
     def close(self):
         if self.ptr:

@@ -1202,3 +1202,436 @@ size_t orc_read_raw(const uint8_t *data, const unsigned dim[3],
             }
     return counter;
 }
+
+/* ------------------------------------------------------------------ */
+/* src/voxtrees/tree.c:358-835 -- flattening and edits                 */
+/* ------------------------------------------------------------------ */
+
+/* geom.c:192-196 */
+static void get_dims(const struct orc_box *b, size_t dim[3])
+{
+    int l;
+    for (l = 0; l < 3; l++)
+        dim[l] = (size_t)((b->max[l] - b->min[l]) / orc_voxel[l]);
+}
+
+/* geom.c:198-209: a box one voxel thick along two axes; *which = the
+ * other axis */
+static int stripep(const struct orc_box *b, int *which)
+{
+    size_t dim[3];
+    int l, count = 0;
+    get_dims(b, dim);
+    for (l = 0; l < 3; l++) {
+        if (dim[l] == 1) count++;
+        else *which = l;
+    }
+    return count == 2;
+}
+
+/* geom.c:169-181 */
+static int divide_box(const struct orc_box *box, const orc_dot center,
+                      struct orc_box *res, int idx)
+{
+    int l;
+    memcpy(res, box, sizeof(*res));
+    for (l = 0; l < 3; l++) {
+        if (idx & (1 << l)) res->max[l] = center[l];
+        else res->min[l] = center[l];
+        if (res->min[l] == res->max[l]) return 0;
+    }
+    return 1;
+}
+
+/* geom-sse.c:42-49: inside [min, max) on x, y, z */
+static int voxel_in_box(const struct orc_box *b, const orc_dot d)
+{
+    int l;
+    for (l = 0; l < 3; l++)
+        if (d[l] < b->min[l] || d[l] >= b->max[l]) return 0;
+    return 1;
+}
+
+/* geom-sse.c:51-62 */
+static void closest_vertex(const struct orc_box *b, const orc_dot d,
+                           orc_dot res)
+{
+    int l;
+    for (l = 0; l < 3; l++) {
+        float d1 = fabsf(b->min[l] - d[l]), d2 = fabsf(b->max[l] - d[l]);
+        res[l] = (d1 < d2) ? b->min[l] : b->max[l];
+    }
+    res[3] = (fabsf(b->min[3] - d[3]) < fabsf(b->max[3] - d[3])) ? b->min[3]
+                                                                 : b->max[3];
+}
+
+/* voxvision.h.in:52 */
+static int dot_equalp(const orc_dot a, const orc_dot b)
+{
+    return a[0] == b[0] && a[1] == b[1] && a[2] == b[2];
+}
+
+/* tree.c:137-143 (SSE variant) */
+static void update_bounding_box(struct orc_box *b, const orc_dot d)
+{
+    int l;
+    for (l = 0; l < 3; l++) {
+        float dmax = d[l] + orc_voxel[l];
+        b->min[l] = smin(d[l], b->min[l]);
+        b->max[l] = smax(dmax, b->max[l]);
+    }
+}
+
+/* tree.c:128-134 (SSE variant) */
+static void vox_align(orc_dot d)
+{
+    int l;
+    for (l = 0; l < 3; l++) d[l] = floorf(d[l] / orc_voxel[l]) * orc_voxel[l];
+}
+
+/* tree.c:229-240 */
+static struct orc_node *make_dense_leaf(const struct orc_box *box)
+{
+    struct orc_node *res = node_alloc(ORC_DENSE_LEAF);
+    size_t dim[3];
+    memcpy(&res->bounding_box, box, sizeof(*box));
+    res->flags = ORC_DENSE_LEAF;
+    get_dims(box, dim);
+    res->dots_num = (unsigned int)(dim[0] * dim[1] * dim[2]);
+    return res;
+}
+
+/* tree.c:364-420: dots of a subtree in DFS order; a dense leaf's voxels
+ * by repeated addition of the voxel size from the box minimum */
+static size_t flatten_tree(const struct orc_node *tree, orc_dot *set)
+{
+    size_t count = 0;
+    if (!tree) return 0;
+    if (tree->flags & ORC_LEAF) {
+        memcpy(set, tree->data.dots, tree->dots_num * sizeof(orc_dot));
+        count = tree->dots_num;
+    } else if (tree->flags & ORC_DENSE_LEAF) {
+        size_t dim[3], i, j, k;
+        orc_dot cur = {0.0f, 0.0f, 0.0f, 0.0f};
+        get_dims(&tree->bounding_box, dim);
+        cur[0] = tree->bounding_box.min[0];
+        for (i = 0; i < dim[0]; i++) {
+            cur[1] = tree->bounding_box.min[1];
+            for (j = 0; j < dim[1]; j++) {
+                cur[2] = tree->bounding_box.min[2];
+                for (k = 0; k < dim[2]; k++) {
+                    memcpy(set[count], cur, sizeof(orc_dot));
+                    cur[2] += orc_voxel[2];
+                    count++;
+                }
+                cur[1] += orc_voxel[1];
+            }
+            cur[0] += orc_voxel[0];
+        }
+    } else {
+        int i;
+        for (i = 0; i < 8; i++) {
+            size_t sub = flatten_tree(tree->data.inner.children[i], set);
+            set += sub;
+            count += sub;
+        }
+    }
+    return count;
+}
+
+struct orc_node *orc_rebuild_tree(const struct orc_node *tree)
+{
+    struct orc_node *t = NULL;
+    if (tree) {
+        orc_dot *dots = aligned_alloc(16, tree->dots_num * sizeof(orc_dot) + 16);
+        flatten_tree(tree, dots);
+        t = orc_make_tree(dots, tree->dots_num);
+        free(dots);
+        if (t) {
+            orc_tree_stats st;
+            orc_tree_stats_get(t, &st);
+        }
+    }
+    return t;
+}
+
+/* tree.c:437-458; the descent compares dot < center (geom-sse.c:64-68) */
+static int voxel_in_tree(const struct orc_node *tree, const orc_dot v)
+{
+    unsigned i;
+    if (tree && voxel_in_box(&tree->bounding_box, v)) {
+        if (tree->flags & ORC_DENSE_LEAF) return 1;
+        if (tree->flags & ORC_LEAF) {
+            for (i = 0; i < tree->dots_num; i++)
+                if (dot_equalp(v, tree->data.dots[i])) return 1;
+        } else {
+            return voxel_in_tree(
+                tree->data.inner.children[orc_get_subspace_idx(
+                    tree->data.inner.center, v)],
+                v);
+        }
+    }
+    return 0;
+}
+
+static void insert_(struct orc_node **tree_ptr, const orc_dot voxel);
+
+/* tree.c:461-501 */
+static struct orc_node *insert_in_big_dense(struct orc_node *tree,
+                                            const orc_dot voxel)
+{
+    struct orc_node *node = node_alloc(0);
+    orc_dot mid;
+    int l, idx1, idx2;
+    memcpy(&node->bounding_box, &tree->bounding_box, sizeof(struct orc_box));
+    update_bounding_box(&node->bounding_box, voxel);
+    node->flags = 0;
+    node->dots_num = tree->dots_num + 1;
+    closest_vertex(&tree->bounding_box, voxel, node->data.inner.center);
+    for (l = 0; l < 4; l++)
+        mid[l] = (tree->bounding_box.min[l] + tree->bounding_box.max[l]) / 2.0f;
+    idx1 = subspace_simd(node->data.inner.center, mid);
+    idx2 = subspace_simd(node->data.inner.center, voxel);
+    node->data.inner.children[idx1] = tree;
+    insert_(&node->data.inner.children[idx2], voxel);
+    return node;
+}
+
+/* a leaf (or dense leaf) rebuilt with one more voxel */
+static struct orc_node *remake_with(const struct orc_node *tree,
+                                    const orc_dot voxel)
+{
+    orc_dot *dots = aligned_alloc(16, (tree->dots_num + 1) * sizeof(orc_dot) + 16);
+    struct orc_node *node;
+    flatten_tree(tree, dots);
+    memcpy(dots[tree->dots_num], voxel, sizeof(orc_dot));
+    node = orc_make_tree(dots, tree->dots_num + 1);
+    free(dots);
+    return node;
+}
+
+/* tree.c:504-605 */
+static void insert_(struct orc_node **tree_ptr, const orc_dot voxel)
+{
+    struct orc_node *tree, *node;
+    struct orc_box bb;
+again:
+    tree = *tree_ptr;
+    node = tree;
+    if (tree) {
+        memcpy(&bb, &tree->bounding_box, sizeof(bb));
+        update_bounding_box(&tree->bounding_box, voxel);
+        if (orc_dense_set_p(&tree->bounding_box, tree->dots_num + 1)) {
+            if (tree->flags & ORC_DENSE_LEAF) {
+                tree->dots_num++;
+            } else {
+                node = make_dense_leaf(&tree->bounding_box);
+                orc_destroy_tree(tree);
+            }
+            goto done;
+        }
+        if (tree->flags & ORC_DENSE_LEAF)
+            memcpy(&tree->bounding_box, &bb, sizeof(bb));
+    } else {
+        orc_dot one[1];
+        memcpy(one[0], voxel, sizeof(orc_dot));
+        node = orc_make_tree(one, 1);
+        goto done;
+    }
+
+    if (tree->flags & ORC_DENSE_LEAF) {
+        if (tree->dots_num < 7) {
+            node = remake_with(tree, voxel);
+            orc_destroy_tree(tree);
+        } else {
+            node = insert_in_big_dense(tree, voxel);
+        }
+    } else if (tree->flags & ORC_LEAF) {
+        if (tree->dots_num < 7) {
+            memcpy(tree->data.dots[tree->dots_num], voxel, sizeof(orc_dot));
+            tree->dots_num++;
+        } else {
+            node = remake_with(tree, voxel);
+            orc_destroy_tree(tree);
+        }
+    } else {
+        int idx = subspace_simd(tree->data.inner.center, voxel);
+        tree->dots_num++;
+        tree_ptr = &tree->data.inner.children[idx];
+        goto again;
+    }
+done:
+    *tree_ptr = node;
+}
+
+int orc_insert_voxel(struct orc_node **tree, const float xyz[3])
+{
+    orc_dot v = {xyz[0], xyz[1], xyz[2], 0.0f};
+    int res;
+    vox_align(v);
+    res = !voxel_in_tree(*tree, v);
+    if (res) {
+        orc_tree_stats st;
+        insert_(tree, v);
+        orc_tree_stats_get(*tree, &st);
+    }
+    return res;
+}
+
+/* tree.c:628-681 */
+static struct orc_node *delete_from_dense_stripe(struct orc_node *tree,
+                                                 const orc_dot voxel)
+{
+    int idx = 0;
+    struct orc_node *node, *child;
+    struct orc_box bb;
+    orc_dot other;
+    int l;
+    if (!stripep(&tree->bounding_box, &idx)) return NULL;
+    if (dot_equalp(tree->bounding_box.min, voxel)) {
+        tree->dots_num--;
+        tree->bounding_box.min[idx] += orc_voxel[idx];
+        return tree;
+    }
+    for (l = 0; l < 4; l++) other[l] = voxel[l] + orc_voxel[l];
+    if (dot_equalp(tree->bounding_box.max, other)) {
+        tree->dots_num--;
+        tree->bounding_box.max[idx] -= orc_voxel[idx];
+        return tree;
+    }
+    node = node_alloc(0);
+    memcpy(&node->bounding_box, &tree->bounding_box, sizeof(struct orc_box));
+    node->flags = 0;
+    node->dots_num = tree->dots_num - 1;
+    memcpy(node->data.inner.center, voxel, sizeof(orc_dot));
+    divide_box(&tree->bounding_box, voxel, &bb, 1 << idx);
+    node->data.inner.children[1 << idx] = make_dense_leaf(&bb);
+    divide_box(&tree->bounding_box, voxel, &bb, 0);
+    child = make_dense_leaf(&bb);
+    node->data.inner.children[0] = child;
+    child->dots_num--;
+    child->bounding_box.min[idx] += orc_voxel[idx];
+    orc_destroy_tree(tree);
+    return node;
+}
+
+/* tree.c:683-745 */
+static struct orc_node *delete_from_big_dense(struct orc_node *tree,
+                                              const orc_dot voxel)
+{
+    struct orc_node *node;
+    struct orc_box bb;
+    int i, l;
+    node = delete_from_dense_stripe(tree, voxel);
+    if (node) return node;
+    if (dot_equalp(tree->bounding_box.min, voxel)) {
+        orc_dot other;
+        for (l = 0; l < 4; l++) other[l] = voxel[l] + orc_voxel[l];
+        node = node_alloc(0);
+        memcpy(&node->bounding_box, &tree->bounding_box,
+               sizeof(struct orc_box));
+        node->flags = 0;
+        node->dots_num = tree->dots_num - 1;
+        memcpy(node->data.inner.center, other, sizeof(orc_dot));
+        for (i = 0; i < 7; i++)
+            if (divide_box(&tree->bounding_box, other, &bb, i))
+                node->data.inner.children[i] = make_dense_leaf(&bb);
+        orc_destroy_tree(tree);
+        return node;
+    }
+    node = node_alloc(0);
+    memcpy(&node->bounding_box, &tree->bounding_box, sizeof(struct orc_box));
+    node->flags = 0;
+    node->dots_num = tree->dots_num - 1;
+    memcpy(node->data.inner.center, voxel, sizeof(orc_dot));
+    for (i = 0; i < 8; i++)
+        if (divide_box(&tree->bounding_box, voxel, &bb, i))
+            node->data.inner.children[i] = make_dense_leaf(&bb);
+    node->data.inner.children[0] =
+        delete_from_big_dense(node->data.inner.children[0], voxel);
+    orc_destroy_tree(tree);
+    return node;
+}
+
+/* tree.c:748-818 */
+static void delete_(struct orc_node **tree_ptr, const orc_dot voxel)
+{
+    struct orc_node *tree, *node;
+    unsigned i;
+again:
+    tree = *tree_ptr;
+    if (tree->dots_num == 1) {
+        orc_destroy_tree(tree);
+        *tree_ptr = NULL;
+        return;
+    }
+    if (tree->flags & ORC_LEAF) {
+        for (i = 0; i < tree->dots_num; i++)
+            if (dot_equalp(tree->data.dots[i], voxel)) break;
+        tree->dots_num--;
+        memmove(tree->data.dots + i, tree->data.dots + i + 1,
+                sizeof(orc_dot) * (tree->dots_num - i));
+    } else if (tree->flags & ORC_DENSE_LEAF) {
+        if (tree->dots_num <= 7) {
+            orc_dot set[7];
+            node = delete_from_dense_stripe(tree, voxel);
+            if (node) {
+                *tree_ptr = node;
+                return;
+            }
+            flatten_tree(tree, set);
+            for (i = 0; i < tree->dots_num; i++)
+                if (dot_equalp(set[i], voxel)) break;
+            memmove(set + i, set + i + 1,
+                    sizeof(orc_dot) * (tree->dots_num - i - 1));
+            *tree_ptr = orc_make_tree(set, tree->dots_num - 1);
+            orc_destroy_tree(tree);
+        } else {
+            *tree_ptr = delete_from_big_dense(tree, voxel);
+        }
+    } else {
+        int idx = subspace_simd(tree->data.inner.center, voxel);
+        node = tree->data.inner.children[idx];
+        if (node->dots_num == tree->dots_num) {
+            /* the only child: it replaces its parent */
+            *tree_ptr = node;
+            free(node_hdr(tree));
+            goto again;
+        }
+        tree->dots_num--;
+        tree_ptr = &tree->data.inner.children[idx];
+        goto again;
+    }
+}
+
+int orc_delete_voxel(struct orc_node **tree, const float xyz[3])
+{
+    orc_dot v = {xyz[0], xyz[1], xyz[2], 0.0f};
+    int res;
+    vox_align(v);
+    res = voxel_in_tree(*tree, v);
+    if (res) {
+        orc_tree_stats st;
+        delete_(tree, v);
+        if (*tree) orc_tree_stats_get(*tree, &st);
+    }
+    return res;
+}
+
+struct orc_node *orc_make_dense_leaf_box(const float mn[3], const float mx[3])
+{
+    struct orc_box b;
+    struct orc_node *t;
+    orc_tree_stats st;
+    memset(&b, 0, sizeof(b));
+    memcpy(b.min, mn, 3 * sizeof(float));
+    memcpy(b.max, mx, 3 * sizeof(float));
+    t = make_dense_leaf(&b);
+    orc_tree_stats_get(t, &st);
+    return t;
+}
+
+int orc_tree_flags(const struct orc_node *tree)
+{
+    return tree ? (int)tree->flags : -1;
+}

@@ -196,6 +196,24 @@ int orc_check_tree(const struct orc_node *tree);
 /* Number of online CPUs usable by this process (sched_getaffinity). */
 int orc_cpu_count(void);
 
+/* ---- tree edits (tree.c:437-835) ---- */
+/* vox_insert_voxel / vox_delete_voxel on *tree (which may become NULL or
+ * a new root): the voxel is aligned to the grid first (vox_align); insert
+ * returns 1 when the voxel was not in the tree and was added, delete 1
+ * when it was there and was removed.  Voxel ids are reassigned after the
+ * edit.  The reference's behaviour for an overflow leaf that gains a
+ * voxel is undefined (a buffer of VOX_MAX_DOTS+1 dots receives all of
+ * them); here that leaf is rebuilt from its dots and the voxel. */
+int orc_insert_voxel(struct orc_node **tree, const float xyz[3]);
+int orc_delete_voxel(struct orc_node **tree, const float xyz[3]);
+/* vox_make_dense_leaf (tree.c:229-240): a dense leaf filling the box. */
+struct orc_node *orc_make_dense_leaf_box(const float mn[3], const float mx[3]);
+/* flags of the root (ORC_LEAF / ORC_DENSE_LEAF / ORC_OVERFLOW bits, 0 for
+ * an inner node), -1 for an empty tree */
+int orc_tree_flags(const struct orc_node *tree);
+/* vox_rebuild_tree (tree.c:422-432): vox_make_tree over flatten_tree. */
+struct orc_node *orc_rebuild_tree(const struct orc_node *tree);
+
 /* ---- around the render loop and the tree ---- */
 /* copy_squares (copy-helper.c:5-40) for one frame: 4x4 squares (16
  * uint32 each, ws x hs of them) to a row-major (4 ws) x (4 hs) image. */

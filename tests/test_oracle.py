@@ -455,3 +455,126 @@ def test_cd_collide_rules(oracle):
     far = c + 100.0
     _, _, hit = T.cd_collide(c, far, rr)
     assert np.array_equal(hit, oracle.brute_balls(dots, c, rr))
+
+
+# ---- tree edits: the assertions of the reference's own tests ----
+LEAF, DENSE, MASK = 1, 2, 3
+
+
+def test_edit_ins_known_answers(oracle):
+    """tests.c:367-439 (test_tree_ins)."""
+    T = oracle.Tree.empty()
+    assert T.insert((0, 0, 0)) == 1 and T.voxels == 1
+    assert T.insert((0.5, 0.1, 0.8)) == 0 and T.voxels == 1  # same voxel
+    assert T.insert((6, 6, 6)) == 1 and T.voxels == 2
+    assert T.check() == 0
+    D = oracle.Tree.dense_box((5, 5, 5), (10, 10, 10))
+    assert D.voxels == 125
+    assert D.insert((6, 6, 6)) == 0
+    assert D.insert((0, 0, 0)) == 1 and D.voxels == 126
+    assert D.check() == 0
+    for step in (2, 1):  # sparse grid, then a dense 3x3x3 block
+        T = oracle.Tree.empty()
+        for i in range(3):
+            for j in range(3):
+                for k in range(3):
+                    assert T.insert((step * i, step * j, step * k)) == 1
+        assert T.voxels == 27 and T.check() == 0
+    assert T.flags == DENSE  # tests.c:523 relies on this
+
+
+@pytest.mark.parametrize("axis,start,step,expect_dense", [
+    (1, 0, 1, True),    # test_tree_del1 (tests.c:442-462)
+    (2, 16, -1, True),  # test_tree_del2 (tests.c:465-485)
+    (0, 1, 1, False),   # test_tree_del3 (tests.c:488-508)
+])
+def test_edit_delete_stripes(oracle, axis, start, step, expect_dense):
+    T = oracle.Tree.empty()
+    d = [0.0, 0.0, 0.0]
+    for _ in range(7 + 10):
+        T.insert(d)
+        d[axis] += 1
+    assert T.flags & DENSE
+    d = [0.0, 0.0, 0.0]
+    d[axis] = start
+    for _ in range(10):
+        assert T.delete(d) == 1
+        d[axis] += step
+        if expect_dense:
+            assert T.flags & DENSE
+        else:
+            assert not (T.flags & MASK)
+        assert T.check() == 0
+
+
+@pytest.mark.parametrize("hole", [(0, 0, 0), (1, 1, 1)])
+def test_edit_delete_from_dense_cube(oracle, hole):
+    """test_tree_del4 / del5 (tests.c:510-562)."""
+    T = oracle.Tree.empty()
+    for i in range(3):
+        for j in range(3):
+            for k in range(3):
+                T.insert((i, j, k))
+    assert T.flags & DENSE
+    assert T.delete(hole) == 1
+    assert not (T.flags & MASK)
+    assert T.check() == 0 and T.voxels == 26
+
+
+def test_edit_transitions(oracle):
+    """test_tree_ins_trans / test_tree_del_trans (tests.c:564-598)."""
+    T = oracle.Tree.empty()
+    d = [0.0, 0.0, 0.0]
+    for _ in range(7 + 4):
+        d[0] += 1
+        T.insert(d)
+        assert T.flags & DENSE
+    d = [0.0, 0.0, 0.0]
+    for _ in range(7 + 4):
+        assert T.flags & DENSE
+        d[0] += 1
+        T.delete(d)
+    assert T.ptr is None
+
+
+def test_edit_g676d50c_ray_hits(oracle):
+    """test_tree_g676d50c (tests.c:600-659): the cross built by insertions;
+    the ray from (-2,-2,0) along (1,1,-1) must hit."""
+    T = oracle.Tree.empty()
+    pts = [(0, 0, 0), (0, 0, -1), (0, 0, -2), (0, 0, 1), (0, 0, 2)]
+    pts += [(-a, 0, 0) for a in (1, 2, 3, 4)] + [(a, 0, 0) for a in (1, 2, 3, 4)]
+    pts += [(0, -a, 0) for a in (1, 2, 3, 4)] + [(0, a, 0) for a in (1, 2, 3, 4)]
+    for p in pts:
+        assert T.insert(p) == 1
+    assert T.check() == 0
+    vox, _ = T.rays(np.array([[-2, -2, 0]], np.float32),
+                    np.array([[1, 1, -1]], np.float32))
+    assert vox[0] != 0xFFFFFFFF
+
+
+def test_edits_keep_the_voxel_set(oracle):
+    """Random inserts and deletes: the tree holds exactly the edited set
+    (ray answers equal the FP64 brute force over that set, ball answers the
+    fp32 brute force) and keeps the check_tree invariants; rebuilding it
+    with vox_rebuild_tree gives the same voxel set."""
+    rng = np.random.default_rng(7)
+    pts = {tuple(p) for p in rng.integers(0, 12, (300, 3)).tolist()}
+    T = oracle.Tree(np.array(sorted(pts), np.float32))
+    cur = set(pts)
+    for _ in range(600):
+        p = tuple(rng.integers(-2, 14, 3).tolist())
+        if rng.random() < 0.5:
+            assert T.insert(p) == (p not in cur)
+            cur.add(p)
+        else:
+            assert T.delete(p) == (p in cur)
+            cur.discard(p)
+    assert T.voxels == len(cur) and T.check() == 0
+    dots = np.array(sorted(cur), np.float32)
+    c = rng.uniform(-3, 15, (2000, 3)).astype(np.float32)
+    r = rng.uniform(0.2, 2.0, 2000).astype(np.float32)
+    assert np.array_equal(T.balls(c, r), oracle.brute_balls(dots, c, r))
+    R = T.rebuild()
+    assert R.voxels == len(cur) and R.check() == 0
+    assert {tuple(x) for x in R.voxel_table().tolist()} == cur
+    assert {tuple(x) for x in T.voxel_table().tolist()} == cur
