@@ -22,6 +22,9 @@
  *   voxg_cd_collide_batch   vox_cd_collide            src/voxrnd/cd.h:47
  *   voxg_read_raw_data      vox_read_raw_data         src/voxtrees/datareader.h:35
  *   voxg_raw_select         its sample loop           src/voxtrees/datareader.c:49-75
+ *   voxg_edit_insert        vox_insert_voxel          src/voxtrees/tree.h:96
+ *   voxg_edit_delete        vox_delete_voxel          src/voxtrees/tree.h:114
+ *   voxg_edit_rebuild       vox_rebuild_tree          src/voxtrees/tree.h:86
  */
 #ifndef VOXGPU_H
 #define VOXGPU_H
@@ -149,6 +152,32 @@ int voxg_ray_batch(const voxg_tree *tree, size_t n, const float *ox,
 int voxg_ball_batch(const voxg_tree *tree, size_t n, const float *cx,
                     const float *cy, const float *cz, const float *r,
                     uint8_t *out, hipStream_t stream);
+
+/* Editable host trees (vox_insert_voxel / vox_delete_voxel,
+ * src/voxtrees/tree.h:96,114, tree.c:437-835).  The tree lives in host
+ * memory in the reference's own node layout; edits follow the reference's
+ * algorithm, so the structure after any edit sequence is the one the
+ * reference would have.  voxg_edit_commit flattens and uploads the current
+ * state as a new, independent device tree (a version): trees committed
+ * earlier stay valid and unchanged until released.  Not thread-safe per
+ * handle. */
+typedef struct voxg_edit_tree voxg_edit_tree;
+/* vox_make_tree over n points (stride 3 or 4 floats; voxel NULL = 1,1,1). */
+int voxg_edit_create(const float *xyz, size_t n, size_t stride,
+                     const float voxel[3], voxg_edit_tree **out);
+/* vox_insert_voxel: the point is aligned down to the voxel grid first;
+ * *inserted (may be NULL) = 1 when it was not in the tree and was added. */
+int voxg_edit_insert(voxg_edit_tree *t, const float xyz[3], int *inserted);
+/* vox_delete_voxel: *deleted = 1 when the voxel was there and was removed. */
+int voxg_edit_delete(voxg_edit_tree *t, const float xyz[3], int *deleted);
+/* vox_rebuild_tree (tree.c:422-432): the same voxels, rebuilt from scratch. */
+int voxg_edit_rebuild(voxg_edit_tree *t);
+/* The current root (NULL when empty), valid until the next edit. */
+const struct vox_node *voxg_edit_root(const voxg_edit_tree *t);
+size_t voxg_edit_voxels(const voxg_edit_tree *t);
+/* Upload the current state as a new device tree (voxg_tree_upload). */
+int voxg_edit_commit(const voxg_edit_tree *t, voxg_tree **out);
+int voxg_edit_release(voxg_edit_tree *t);
 
 /* vox_cd_collide (src/voxrnd/cd.c:38-55) for n bodies against one tree.
  * Body i is a ball of radius[i] at (px,py,pz)[i] that was last at
@@ -301,6 +330,12 @@ typedef struct {
 } voxg_server_stats;
 int voxg_server_create(const voxg_server_config *cfg, voxg_server **out);
 int voxg_server_add_tree(voxg_server *s, const voxg_tree *tree, int *id);
+/* Tree versioning: requests submitted after this call use `tree` for id;
+ * requests submitted before keep the version current at their submission
+ * (e.g. a voxg_edit_commit of an edited scene replaces the previous one).
+ * Release a replaced version only after the requests submitted before the
+ * replacement have completed (voxg_server_wait). */
+int voxg_server_replace_tree(voxg_server *s, int id, const voxg_tree *tree);
 int voxg_server_submit_render(voxg_server *s, const voxg_render_request *r,
                               uint64_t *ticket);
 int voxg_server_submit_balls(voxg_server *s, const voxg_ball_request *r,

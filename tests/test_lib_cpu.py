@@ -128,3 +128,103 @@ def test_camera_setter_matches_reference_formula(voxlib):
         voxlib.voxg_camera_set_window_size(c, w, h)
         mul, xs, ys = camera.set_window_size(c.fov, w, h)
         assert (c.mul, c.xsub, c.ysub) == (mul, xs, ys)
+
+
+# ---- editable trees (host code: no device needed) ----
+def _edit_pair(O, dots=None):
+    from voxvision_amd.edit import EditTree
+    if dots is None:
+        return EditTree(), O.Tree.empty()
+    return EditTree(dots), O.Tree(dots)
+
+
+def _run_script(E, T, script):
+    for op, p in script:
+        a = E.insert(p) if op == "ins" else E.delete(p)
+        b = T.insert(p) if op == "ins" else T.delete(p)
+        assert a == bool(b), (op, p)
+        assert E.voxels == T.voxels
+        assert E.hash() == T.hash(), (op, p)
+
+
+def test_edit_reference_scripts_match_oracle(oracle):
+    """The insertion/deletion sequences of tests.c:367-659 give the same
+    tree, node for node, as the oracle after every step."""
+    scripts = []
+    s = [("ins", (0, 0, 0)), ("ins", (0.5, 0.1, 0.8)), ("ins", (6, 6, 6))]
+    scripts.append(s)
+    for step in (2, 1):
+        scripts.append([("ins", (step * i, step * j, step * k))
+                        for i in range(3) for j in range(3) for k in range(3)])
+    for axis, start, d in ((1, 0, 1), (2, 16, -1), (0, 1, 1)):
+        s = []
+        p = [0.0, 0.0, 0.0]
+        for _ in range(17):
+            s.append(("ins", tuple(p)))
+            p[axis] += 1
+        p = [0.0, 0.0, 0.0]
+        p[axis] = start
+        for _ in range(10):
+            s.append(("del", tuple(p)))
+            p[axis] += d
+        scripts.append(s)
+    cube = [("ins", (i, j, k)) for i in range(3) for j in range(3)
+            for k in range(3)]
+    scripts.append(cube + [("del", (0, 0, 0))])
+    scripts.append(cube + [("del", (1, 1, 1))])
+    scripts.append([("ins", (a, 0, 0)) for a in range(1, 12)] +
+                   [("del", (a, 0, 0)) for a in range(1, 12)])
+    cross = [(0, 0, 0), (0, 0, -1), (0, 0, -2), (0, 0, 1), (0, 0, 2)]
+    cross += [(-a, 0, 0) for a in (1, 2, 3, 4)] + [(a, 0, 0) for a in (1, 2, 3, 4)]
+    cross += [(0, -a, 0) for a in (1, 2, 3, 4)] + [(0, a, 0) for a in (1, 2, 3, 4)]
+    scripts.append([("ins", p) for p in cross])
+    for s in scripts:
+        E, T = _edit_pair(oracle)
+        _run_script(E, T, s)
+    # a dense-leaf root (tests.c:390-402), built as the 5x5x5 block
+    blk = np.array([(i, j, k) for i in range(5, 10) for j in range(5, 10)
+                    for k in range(5, 10)], np.float32)
+    E, T = _edit_pair(oracle, blk)
+    assert E.hash() == T.hash()
+    _run_script(E, T, [("ins", (6, 6, 6)), ("ins", (0, 0, 0))])
+
+
+@pytest.mark.parametrize("scene", ["sparse", "dense_cube", "blobs"])
+def test_edit_random_sequences_match_oracle(oracle, scene):
+    """Random inserts and deletes (misses, hits, aligned and unaligned
+    points, negative coordinates, a non-unit voxel) on sparse, dense and
+    blob scenes: product and oracle trees are identical after every edit,
+    and vox_rebuild_tree agrees too."""
+    from voxvision_amd import scenes
+    from voxvision_amd.edit import EditTree
+    O = oracle
+    rng = np.random.default_rng({"sparse": 1, "dense_cube": 2, "blobs": 3}[scene])
+    V = (1.0, 1.0, 1.0)
+    if scene == "sparse":
+        dots = np.unique(rng.integers(0, 16, (400, 3)), axis=0).astype(np.float32)
+        lo, hi, n_ops = -2, 18, 800
+    elif scene == "dense_cube":
+        dots = np.array([(i, j, k) for i in range(10) for j in range(10)
+                         for k in range(10)], np.float32)
+        lo, hi, n_ops = -1, 11, 600
+    else:
+        occ = scenes.blobs64(2)[:32, :32, :32]
+        dots = np.argwhere(occ).astype(np.float32) * 0.5
+        V = (0.5, 0.5, 0.5)
+        lo, hi, n_ops = 0, 16, 300
+    E = EditTree(dots, voxel=V)
+    T = O.Tree(dots, voxel=V)
+    assert E.hash() == T.hash()
+    for _ in range(n_ops):
+        p = rng.uniform(lo, hi, 3)
+        if rng.random() < 0.3:
+            p = np.floor(p)
+        op = "ins" if rng.random() < 0.5 else "del"
+        a = E.insert(p) if op == "ins" else E.delete(p)
+        b = T.insert(p) if op == "ins" else T.delete(p)
+        assert a == bool(b)
+        assert E.hash() == T.hash(), (op, p)
+    assert E.voxels == T.voxels and T.check() == 0
+    R = T.rebuild()
+    E.rebuild()
+    assert E.hash() == R.hash()

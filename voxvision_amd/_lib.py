@@ -30,6 +30,9 @@ EXPORTS = [
     "voxg_server_submit_balls", "voxg_server_wait", "voxg_server_get_stats",
     "voxg_server_now", "voxg_server_destroy", "voxg_cd_collide_batch",
     "voxg_redraw", "voxg_raw_select", "voxg_read_raw_data",
+    "voxg_edit_create", "voxg_edit_insert", "voxg_edit_delete",
+    "voxg_edit_rebuild", "voxg_edit_root", "voxg_edit_voxels",
+    "voxg_edit_commit", "voxg_edit_release", "voxg_server_replace_tree",
 ]
 
 
@@ -132,6 +135,16 @@ def load():
     L.voxg_raw_select.argtypes = [P, P, C.c_uint, U64, U64, P, P, S, P, P]
     L.voxg_read_raw_data.argtypes = [C.c_char_p, P, C.c_uint, U64, U64, P,
                                      C.POINTER(P)]
+    L.voxg_edit_create.argtypes = [P, S, S, P, C.POINTER(P)]
+    L.voxg_edit_insert.argtypes = [P, P, C.POINTER(C.c_int)]
+    L.voxg_edit_delete.argtypes = [P, P, C.POINTER(C.c_int)]
+    L.voxg_edit_rebuild.argtypes = [P]
+    L.voxg_edit_root.argtypes = [P]
+    L.voxg_edit_root.restype = P
+    L.voxg_edit_voxels.argtypes = [P]
+    L.voxg_edit_voxels.restype = S
+    L.voxg_edit_commit.argtypes = [P, C.POINTER(P)]
+    L.voxg_edit_release.argtypes = [P]
     _lib = L
     return L
 

@@ -19,6 +19,7 @@ BUILD = os.path.join(PKG, "_build")
 
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 HIP_SOURCES = ["voxgpu.hip", "volume.hip"]
+CXX_SOURCES = ["tree_build.cpp", "tree_edit.cpp"]
 ARCH = "gfx950"
 
 HIP_FLAGS = [
@@ -46,9 +47,13 @@ def _run(cmd):
 def build(verbose=False):
     os.makedirs(BUILD, exist_ok=True)
     cxx = shutil.which("g++") or "g++"
-    obj = os.path.join(BUILD, "tree_build.o")
-    log = _run([cxx] + CXX_FLAGS + ["-c", os.path.join(CSRC, "tree_build.cpp"),
-                                     "-o", obj])
+    objs = []
+    log = ""
+    for name in CXX_SOURCES:
+        obj = os.path.join(BUILD, name.replace(".cpp", ".o"))
+        log += _run([cxx] + CXX_FLAGS + ["-c", os.path.join(CSRC, name),
+                                         "-o", obj])
+        objs.append(obj)
     tmp = OUT + ".tmp"
     hobjs = []
     for name in HIP_SOURCES:
@@ -57,7 +62,7 @@ def build(verbose=False):
             "-c", os.path.join(CSRC, name), "-o", hobj])
         hobjs.append(hobj)
     log += _run([HIPCC, "--offload-arch=" + ARCH, "-shared", "-o", tmp,
-                 "-x", "none"] + hobjs + [obj, "-lpthread"])
+                 "-x", "none"] + hobjs + objs + ["-lpthread"])
     os.replace(tmp, OUT)
     if verbose and log:
         print(log)

@@ -16,6 +16,7 @@
 #include <algorithm>
 #include <atomic>
 #include <cmath>
+#include <cstdlib>
 #include <cstring>
 #include <deque>
 #include <mutex>
@@ -401,6 +402,88 @@ int build_tree(const float *xyz, size_t n, size_t stride, const float V[3],
     }
     BView view{work.data()};
     return flatten(view, root, V, out, err);
+}
+
+// A BNode subtree as reference-layout nodes (struct vox_node): the same
+// boxes, centres, flags and dot order; regular leaves get room for 7 dots
+// (tree.c:267), overflow leaves exactly their own.
+static struct vox_node *to_reference(const BNode *b, const float *set) {
+    if (!b) return nullptr;
+    auto *n = static_cast<struct vox_node *>(
+        std::aligned_alloc(16, sizeof(struct vox_node)));
+    if (!n) return nullptr;
+    std::memset(n, 0, sizeof(*n));
+    for (int l = 0; l < 3; l++) {
+        n->bounding_box.min[l] = b->bmin[l];
+        n->bounding_box.max[l] = b->bmax[l];
+    }
+    n->dots_num = b->n;
+    if (b->flags == NT_DENSE) {
+        n->flags = VOX_DENSE_LEAF;
+    } else if (b->flags & NT_LEAF) {
+        n->flags = VOX_LEAF | ((b->flags & NT_OVERFLOW) ? VOX_OVERFLOW : 0u);
+        size_t cap = std::max<size_t>(b->n, 7);
+        n->data.dots = static_cast<vox_dot *>(
+            std::aligned_alloc(16, cap * sizeof(vox_dot)));
+        if (!n->data.dots) {
+            std::free(n);
+            return nullptr;
+        }
+        std::memset(n->data.dots, 0, cap * sizeof(vox_dot));
+        for (uint32_t k = 0; k < b->n; k++)
+            for (int l = 0; l < 3; l++)
+                n->data.dots[k][l] = set[4 * (b->off + k) + l];
+    } else {
+        n->flags = 0;
+        for (int l = 0; l < 3; l++) n->data.inner.center[l] = b->c[l];
+        for (int i = 0; i < 8; i++) {
+            n->data.inner.children[i] = to_reference(b->child[i], set);
+            if (b->child[i] && !n->data.inner.children[i]) {
+                destroy_reference(n);
+                return nullptr;
+            }
+        }
+    }
+    return n;
+}
+
+void destroy_reference(struct vox_node *t) {
+    if (!t) return;
+    if (t->flags & VOX_LEAF) {
+        std::free(t->data.dots);
+    } else if (!(t->flags & VOX_LEAF_MASK)) {
+        for (int i = 0; i < 8; i++) destroy_reference(t->data.inner.children[i]);
+    }
+    std::free(t);
+}
+
+int build_reference(const float *xyz, size_t n, size_t stride, const float V[3],
+                    struct vox_node **root, std::string &err) {
+    *root = nullptr;
+    if (n == 0) return 0;
+    if (n >= 0xffffffffull) {
+        err = "more than 2^32-1 voxels";
+        return -34;
+    }
+    Builder b;
+    for (int l = 0; l < 3; l++) b.V[l] = V[l];
+    std::vector<float> work(4 * n);
+    for (size_t i = 0; i < n; i++) {
+        work[4 * i + 0] = xyz[stride * i + 0];
+        work[4 * i + 1] = xyz[stride * i + 1];
+        work[4 * i + 2] = xyz[stride * i + 2];
+        work[4 * i + 3] = 0.0f;
+    }
+    b.set = work.data();
+    int hw = (int)std::thread::hardware_concurrency();
+    b.spare_threads = std::max(0, std::min(hw, 64) - 1);
+    BNode *broot = b.build(0, n);
+    *root = to_reference(broot, work.data());
+    if (!*root) {
+        err = "host allocation failed";
+        return -12;
+    }
+    return 0;
 }
 
 int flatten_reference(const struct vox_node *root, const float V[3],

@@ -1969,6 +1969,7 @@ struct SrvRequest {
     uint64_t id = 0;
     int kind = 0;  // 0 render, 1 balls
     int tree = 0;
+    dev::Tree t;   // the tree version current when the request was submitted
     voxg_render_request rr;
     size_t n = 0;
     std::vector<float> balls;  // cx | cy | cz | r
@@ -2058,7 +2059,7 @@ int srv_launch(voxg_server *s, SrvGroup &g) {
             char *base = arena + r->off;
             dev::RenderItem it;
             std::memset(&it, 0, sizeof(it));
-            it.t = s->trees[r->tree];
+            it.t = r->t;
             it.cams = nullptr;  // set after the table layout is known
             it.W = (uint32_t)r->rr.W;
             it.ws = (uint32_t)r->rr.W / 4u;
@@ -2095,7 +2096,7 @@ int srv_launch(voxg_server *s, SrvGroup &g) {
             size_t n = r->n;
             dev::BallItem it;
             std::memset(&it, 0, sizeof(it));
-            it.t = s->trees[r->tree];
+            it.t = r->t;
             it.cx = (const float *)base;
             it.cy = it.cx + n;
             it.cz = it.cy + n;
@@ -2303,21 +2304,44 @@ int voxg_server_create(const voxg_server_config *cfg, voxg_server **out) {
     return 0;
 }
 
-int voxg_server_add_tree(voxg_server *s, const voxg_tree *t, int *id) {
-    if (!s || !id) return fail(VOXG_EINVAL, "NULL argument");
-    int rc = check_tree(t);
-    if (rc) return rc;
-    // size every stream's spill buffer for the deepest tree
-    uint32_t bound = std::max(s->stack_bound, t->stack_bound);
+// Stream spill buffers sized for trees of stack bound `bound`.
+static int srv_fit_trees(voxg_server *s, uint32_t bound) {
     voxg_launch_config c = current_config();
     uint32_t slots = bound > (uint32_t)c.stack_lds ? bound - c.stack_lds : 1;
     int cus = cu_count(s->device);
     size_t threads = (size_t)cus * std::max(1, c.waves_per_cu) * 64;
     for (auto &ss : s->streams) {
         Workspace *ws = nullptr;
-        rc = get_ws(ss.s, (size_t)slots * threads * sizeof(uint4), &ws);
+        int rc = get_ws(ss.s, (size_t)slots * threads * sizeof(uint4), &ws);
         if (rc) return rc;
     }
+    return 0;
+}
+
+// Requests submitted from now on use tree t under this id; requests
+// submitted before keep the version they were submitted with.  The
+// streams' spill buffers are not touched here (the scheduler thread may be
+// launching on them): a deeper tree makes the next launch grow them.
+int voxg_server_replace_tree(voxg_server *s, int id, const voxg_tree *t) {
+    if (!s) return fail(VOXG_EINVAL, "server is NULL");
+    int rc = check_tree(t);
+    if (rc) return rc;
+    std::lock_guard<std::mutex> g(s->mu);
+    if (id < 0 || id >= (int)s->trees.size())
+        return fail(VOXG_EINVAL, "unknown tree id");
+    s->stack_bound = std::max(s->stack_bound, t->stack_bound);
+    s->trees[id] = dev_tree(t);
+    return 0;
+}
+
+int voxg_server_add_tree(voxg_server *s, const voxg_tree *t, int *id) {
+    if (!s || !id) return fail(VOXG_EINVAL, "NULL argument");
+    int rc = check_tree(t);
+    if (rc) return rc;
+    // size every stream's spill buffer for the deepest tree
+    uint32_t bound = std::max(s->stack_bound, t->stack_bound);
+    rc = srv_fit_trees(s, bound);
+    if (rc) return rc;
     std::lock_guard<std::mutex> g(s->mu);
     s->stack_bound = bound;
     s->trees.push_back(dev_tree(t));
@@ -2327,6 +2351,11 @@ int voxg_server_add_tree(voxg_server *s, const voxg_tree *t, int *id) {
 
 static int srv_submit(voxg_server *s, SrvRequest *r, uint64_t *ticket) {
     std::lock_guard<std::mutex> g(s->mu);
+    if (r->tree < 0 || r->tree >= (int)s->trees.size()) {
+        delete r;
+        return fail(VOXG_EINVAL, "unknown tree id");
+    }
+    r->t = s->trees[r->tree];
     if (r->bytes > s->arena.size()) {
         s->st.rejected++;
         delete r;
@@ -2345,8 +2374,6 @@ static int srv_submit(voxg_server *s, SrvRequest *r, uint64_t *ticket) {
 int voxg_server_submit_render(voxg_server *s, const voxg_render_request *rq,
                               uint64_t *ticket) {
     if (!s || !rq || !ticket) return fail(VOXG_EINVAL, "NULL argument");
-    if (rq->tree < 0 || rq->tree >= (int)s->trees.size())
-        return fail(VOXG_EINVAL, "unknown tree id");
     if (rq->W <= 0 || rq->H <= 0 || (rq->W & 0xf) || (rq->H & 0x3))
         return fail(VOXG_EINVAL,
                     "W must be a positive multiple of 16 and H of 4");
@@ -2371,8 +2398,6 @@ int voxg_server_submit_render(voxg_server *s, const voxg_render_request *rq,
 int voxg_server_submit_balls(voxg_server *s, const voxg_ball_request *rq,
                              uint64_t *ticket) {
     if (!s || !rq || !ticket) return fail(VOXG_EINVAL, "NULL argument");
-    if (rq->tree < 0 || rq->tree >= (int)s->trees.size())
-        return fail(VOXG_EINVAL, "unknown tree id");
     if (rq->n == 0 || !rq->cx || !rq->cy || !rq->cz || !rq->r || !rq->out)
         return fail(VOXG_EINVAL, "empty or NULL ball request");
     if (rq->n >= 0x10000000ull) return fail(VOXG_ERANGE, "too many balls");

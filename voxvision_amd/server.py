@@ -55,6 +55,7 @@ def _bind(L):
     P = C.c_void_p
     L.voxg_server_create.argtypes = [C.POINTER(ServerConfig), C.POINTER(P)]
     L.voxg_server_add_tree.argtypes = [P, P, C.POINTER(C.c_int)]
+    L.voxg_server_replace_tree.argtypes = [P, C.c_int, P]
     L.voxg_server_submit_render.argtypes = [P, C.POINTER(RenderRequest),
                                             C.POINTER(C.c_uint64)]
     L.voxg_server_submit_balls.argtypes = [P, C.POINTER(BallRequest),
@@ -106,6 +107,15 @@ class Server:
               "voxg_server_add_tree")
         self._trees.append(tree)
         return i.value
+
+    def replace_tree(self, tree_id, tree):
+        """New version of a resident tree (e.g. EditTree.commit()): requests
+        submitted from now on use it; earlier ones keep theirs.  Replaced
+        versions are kept alive as long as this server object."""
+        check(_lib.load().voxg_server_replace_tree(self._h, int(tree_id),
+                                                   tree.handle),
+              "voxg_server_replace_tree")
+        self._trees.append(tree)
 
     def submit_render(self, tree_id, cam, W, H, quality=0, colors=False,
                       hits=True, out=None):
