@@ -134,13 +134,16 @@ def batched_leg(ev0, ev1, stream, log):
             "plain_config": pcfg, "batched_config": bcfg}
 
 
-def aux_leg(G, out, W, H, nf, ev0, ev1, stream, log):
+def aux_leg(G, dots, W, H, nf, ev0, ev1, stream, log):
     """The calls on either side of the path: vox_redraw's square-to-row
     copy of the headline frames (with the clear), vox_cd_collide for 4 M
     bodies, and vox_read_raw_data's selection over a 512^3 one-byte
     volume (8% selected).  HBM bytes: redraw 12 B per pixel (read 4, write
     4 + clear 4); selection 2 x 1 B per sample read + 12 B per selected
-    voxel written."""
+    voxel written.  Tree edits: 20,000 alternating inserts and deletes on
+    the host tree of the headline scene (vox_insert_voxel /
+    vox_delete_voxel), then one commit of the result as a new device
+    version (flatten + upload)."""
     import numpy as np
     import torch
     from voxvision_amd import voxrnd, voxtrees as VT
@@ -204,6 +207,27 @@ def aux_leg(G, out, W, H, nf, ev0, ev1, stream, log):
     res["raw_select_selected"] = int(m)
     res["raw_select_gb_s"] = round((2.0 * flat.numel() + 12.0 * m) / ms / 1e6,
                                    1)
+    # tree edits on the headline scene: host-side vox_insert_voxel /
+    # vox_delete_voxel, then one commit (flatten + upload of a version)
+    from voxvision_amd.edit import EditTree
+    t0 = time.perf_counter()
+    E = EditTree(dots)
+    res["edit_tree_build_s"] = round(time.perf_counter() - t0, 3)
+    rng = np.random.default_rng(23)
+    pts = np.floor(rng.uniform(bmin, bmax, (20000, 3))).astype(np.float32)
+    t0 = time.perf_counter()
+    changed = 0
+    for i, p in enumerate(pts):
+        changed += E.insert(p) if i % 2 == 0 else E.delete(p)
+    dt = time.perf_counter() - t0
+    res["edits_per_s"] = round(len(pts) / dt, 1)
+    res["edits_changed"] = int(changed)
+    t0 = time.perf_counter()
+    G2 = E.commit()
+    torch.cuda.synchronize()
+    res["commit_ms"] = round((time.perf_counter() - t0) * 1e3, 2)
+    res["commit_device_bytes"] = int(G2.info["device_bytes"])
+    del G2, E
     log("[bench] around the path: %s" % json.dumps(res))
     return res
 
@@ -496,7 +520,7 @@ def main():
         extra["balls_4M_mqueries_s"] = round((1 << 22) / bms / 1e3, 1)
 
     if not args.quick:
-        extra["around_the_path"] = aux_leg(G, out, W, H, nf, ev0, ev1,
+        extra["around_the_path"] = aux_leg(G, dots, W, H, nf, ev0, ev1,
                                            stream, log)
     if not args.no_batched:
         extra["batched"] = batched_leg(ev0, ev1, stream, log)
