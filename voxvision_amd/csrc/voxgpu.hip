@@ -313,9 +313,12 @@ struct Ctrl {
     uint32_t next;     // next work unit to hand out
     uint32_t left;     // waves that have finished
     uint32_t overflow; // set when a stack bound was exceeded (never expected)
-    uint32_t qcount;   // length of the block queue built by block_cull
+    uint32_t qcount;   // block queue built by block_cull: blocks all of
+                       // whose rays enter the scene box, from the front
     unsigned long long visits, tests, rays, leaf_trav;
-    unsigned long long fast_blocks, best_blocks, merge_blocks, pad2;
+    unsigned long long fast_blocks, best_blocks, merge_blocks;
+    uint32_t qback;    // ... and the other entering blocks, from the back
+    uint32_t qtotal;   // qcount + qback
 };
 
 // ---------------------------------------------------------------------------
@@ -334,6 +337,7 @@ struct ChunkWork {
     Src src;
     Sink sink;
     uint32_t n, chunk;
+    __device__ __forceinline__ void prepare() {}
     struct State {
         uint32_t ray, end;
     };
@@ -375,6 +379,7 @@ struct MultiWork {
     Sink sink;
     const BatchItem *items;
     uint32_t n_items, chunk;
+    __device__ __forceinline__ void prepare() {}
     struct State {
         uint32_t ray, end;
         Tree t;
@@ -445,7 +450,16 @@ struct BlockWork {
     RenderItem one;             // the job when !MULTI
     const RenderItem *items;    // the jobs when MULTI, by unit_begin
     uint32_t n_items;
-    const uint32_t *queue;      // !MULTI: units are queue entries if set
+    const uint32_t *queue;      // !MULTI: units are queue entries if set:
+    const uint32_t *qfront;     // the first *qfront units from the front,
+    uint32_t qlen;              // the rest from the back of qlen entries
+    uint32_t qf;                // *qfront, read once by prepare()
+    // read once per thread what the block pre-pass counted on the device
+    __device__ __forceinline__ void prepare() {
+        if constexpr (!MULTI) {
+            if (queue) qf = *qfront;
+        }
+    }
     enum { CORNER1 = 0, CORNER2 = 1, LOOP = 2 };
     struct State {
         uint32_t item, blk, base;  // job, block in the job, output index
@@ -502,7 +516,7 @@ struct BlockWork {
                                           uint32_t &cur, f3 &org, f3 &d) const {
         st.item = 0;
         if constexpr (!MULTI) {
-            if (queue) unit = queue[unit];
+            if (queue) unit = queue[unit < qf ? unit : qlen - 1u - (unit - qf)];
         }
         if constexpr (MULTI) {
             uint32_t lo = 0, hi = n_items - 1;
@@ -621,11 +635,13 @@ struct BlockWork {
 // ---------------------------------------------------------------------------
 template <int K, class Work, bool COUNT>
 __global__ void __launch_bounds__(256)
-rti_kernel(Tree t, Work work, uint32_t n, const uint32_t *n_dev, Ctrl *ctrl,
-           uint4 *__restrict__ spill, uint32_t spill_slots, uint32_t refill,
-           uint32_t fin_batch, uint32_t leaf_batch) {
+rti_kernel(Tree t, Work work_arg, uint32_t n, const uint32_t *n_dev,
+           Ctrl *ctrl, uint4 *__restrict__ spill, uint32_t spill_slots,
+           uint32_t refill, uint32_t fin_batch, uint32_t leaf_batch) {
     extern __shared__ uint4 ring[];  // [K][blockDim.x]
     if (n_dev) n = *n_dev;  // unit count produced on the device
+    Work work = work_arg;
+    work.prepare();
     const uint32_t tid = threadIdx.x;
     const uint32_t bdim = blockDim.x;
     const uint32_t gtid = blockIdx.x * bdim + tid;
@@ -901,6 +917,8 @@ rti_kernel(Tree t, Work work, uint32_t n, const uint32_t *n_dev, Ctrl *ctrl,
         if (done == total - 1) {
             atomicExch(&ctrl->next, 0u);
             atomicExch(&ctrl->qcount, 0u);
+            atomicExch(&ctrl->qback, 0u);
+            atomicExch(&ctrl->qtotal, 0u);
             atomicExch(&ctrl->left, 0u);
         }
     }
@@ -922,10 +940,10 @@ constexpr uint32_t kCullTile = 256 * 16;
 template <bool COUNT>
 __global__ void __launch_bounds__(256)
 block_cull_kernel(RenderItem it, uint32_t npix, Ctrl *ctrl,
-                  uint32_t *__restrict__ queue) {
+                  uint32_t *__restrict__ queue, uint32_t qlen) {
     __shared__ uint8_t enter[kCullTile / 16];
-    __shared__ uint32_t wave_sum[4];
-    __shared__ uint32_t gbase;
+    __shared__ uint32_t wave_sum[2][4];
+    __shared__ uint32_t gbase[2];
     __shared__ unsigned long long missed;
     const uint32_t tid = threadIdx.x, lane = lane_id(), w = tid >> 6;
     if (tid == 0) missed = 0;
@@ -950,26 +968,43 @@ block_cull_kernel(RenderItem it, uint32_t npix, Ctrl *ctrl,
             enters = hit_box(lo, hi, f3{c.px, c.py, c.pz}, d, e);
         }
         const uint64_t m = __ballot(enters);
-        const bool block_enters = ((m >> (16 * (lane >> 4))) & 0xffffull) != 0;
+        const uint64_t bm = (m >> (16 * (lane >> 4))) & 0xffffull;
+        const bool block_enters = bm != 0;
         if (valid && !block_enters) it.sink.put(r, false, kEnd, 0, f3{0, 0, 0});
-        if (p == 0) enter[k * 16 + (tid >> 4)] = (valid && block_enters) ? 1 : 0;
+        // 2: every ray of the block enters the box, 1: some do
+        if (p == 0)
+            enter[k * 16 + (tid >> 4)] =
+                (valid && block_enters) ? (bm == 0xffffull ? 2 : 1) : 0;
         if (COUNT && valid && p == 0 && !block_enters) my_missed++;
     }
     __syncthreads();
-    // ordered compaction of the 256 block flags: thread t owns block t
-    const bool mine = enter[tid] != 0;
-    const uint64_t wm = __ballot(mine);
-    if (lane == 0) wave_sum[w] = (uint32_t)__popcll(wm);
+    // ordered compaction of the 256 block flags (thread t owns block t):
+    // blocks whose rays all enter go to the front of the queue, the other
+    // entering blocks to the back, so the longest work is handed out first
+    // and the end of the launch is made of the cheaper blocks
+    const uint32_t cls = enter[tid];
+    const uint64_t fm = __ballot(cls == 2), pm = __ballot(cls == 1);
+    if (lane == 0) {
+        wave_sum[0][w] = (uint32_t)__popcll(fm);
+        wave_sum[1][w] = (uint32_t)__popcll(pm);
+    }
     if (COUNT && my_missed) atomicAdd(&missed, (unsigned long long)my_missed);
     __syncthreads();
-    uint32_t off = 0, total = 0;
-    for (uint32_t i = 0; i < 4; ++i) {
-        off += (i < w) ? wave_sum[i] : 0u;
-        total += wave_sum[i];
+    uint32_t off[2] = {0, 0}, total[2] = {0, 0};
+    for (uint32_t i = 0; i < 4; ++i)
+        for (int c = 0; c < 2; ++c) {
+            off[c] += (i < w) ? wave_sum[c][i] : 0u;
+            total[c] += wave_sum[c][i];
+        }
+    if (tid == 0) {
+        gbase[0] = total[0] ? atomicAdd(&ctrl->qcount, total[0]) : 0u;
+        gbase[1] = total[1] ? atomicAdd(&ctrl->qback, total[1]) : 0u;
+        if (total[0] + total[1]) atomicAdd(&ctrl->qtotal, total[0] + total[1]);
     }
-    if (tid == 0) gbase = total ? atomicAdd(&ctrl->qcount, total) : 0u;
     __syncthreads();
-    if (mine) queue[gbase + off + lanes_below(wm)] = tile0 / 16 + tid;
+    if (cls == 2) queue[gbase[0] + off[0] + lanes_below(fm)] = tile0 / 16 + tid;
+    if (cls == 1)
+        queue[qlen - 1u - (gbase[1] + off[1] + lanes_below(pm))] = tile0 / 16 + tid;
     if (COUNT && tid == 0 && missed) {
         const unsigned long long nb = missed;
         const bool halved = it.rnd_mode == VOXG_QUALITY_ADAPTIVE &&
@@ -1781,12 +1816,15 @@ static int render_launch(const voxg_tree *t, size_t n_frames,
     hipLaunchKernelGGL((dev::block_cull_kernel<COUNT>),
                        dim3((npix + dev::kCullTile - 1) / dev::kCullTile),
                        dim3(256), 0, stream,
-                       work.one, (uint32_t)npix, ws->ctrl, ws->queue);
+                       work.one, (uint32_t)npix, ws->ctrl, ws->queue,
+                       (uint32_t)nblk);
     HIPCHK(hipGetLastError());
     work.queue = ws->queue;
+    work.qfront = &ws->ctrl->qcount;
+    work.qlen = (uint32_t)nblk;
     return launch_tree<dev::BlockWork<false>, COUNT>(
         dev_tree(t), t->stack_bound, work, nblk, stream, counts,
-        &ws->ctrl->qcount, false);
+        &ws->ctrl->qtotal, false);
 }
 
 }  // extern "C++"
