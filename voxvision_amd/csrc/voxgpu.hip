@@ -310,7 +310,7 @@ struct PixSink {
 };
 
 struct Ctrl {
-    uint32_t next;     // next work unit to hand out
+    uint32_t spare;    // (units are handed out by the per-XCD counters)
     uint32_t left;     // waves that have finished
     uint32_t overflow; // set when a stack bound was exceeded (never expected)
     uint32_t qcount;   // block queue built by block_cull: blocks all of
@@ -334,6 +334,7 @@ struct Ctrl {
 // waiting for a wave-wide refill.
 template <class Src, class Sink>
 struct ChunkWork {
+    static constexpr bool kOneRange = false;
     Src src;
     Sink sink;
     uint32_t n, chunk;
@@ -375,6 +376,7 @@ struct BatchItem {
 // traverses that item's tree.
 template <class Src, class Sink>
 struct MultiWork {
+    static constexpr bool kOneRange = false;
     Src src;
     Sink sink;
     const BatchItem *items;
@@ -447,6 +449,8 @@ struct RenderItem {
 // cameras, sizes, modes) in one launch.
 template <bool MULTI>
 struct BlockWork {
+    // one render's block queue is ordered longest-first (block_cull_kernel)
+    static constexpr bool kOneRange = !MULTI;
     RenderItem one;             // the job when !MULTI
     const RenderItem *items;    // the jobs when MULTI, by unit_begin
     uint32_t n_items;
@@ -633,13 +637,20 @@ struct BlockWork {
 // ---------------------------------------------------------------------------
 // persistent traversal kernel
 // ---------------------------------------------------------------------------
+constexpr uint32_t kXcds = 8;        // XCDs of an MI355X
+constexpr uint32_t kXcdStride = 64;  // counters 256 bytes apart
 template <int K, class Work, bool COUNT>
 __global__ void __launch_bounds__(256)
 rti_kernel(Tree t, Work work_arg, uint32_t n, const uint32_t *n_dev,
-           Ctrl *ctrl, uint4 *__restrict__ spill, uint32_t spill_slots,
-           uint32_t refill, uint32_t fin_batch, uint32_t leaf_batch) {
+           Ctrl *ctrl, uint32_t *xnext, uint4 *__restrict__ spill,
+           uint32_t spill_slots, uint32_t refill, uint32_t fin_batch,
+           uint32_t leaf_batch) {
     extern __shared__ uint4 ring[];  // [K][blockDim.x]
     if (n_dev) n = *n_dev;  // unit count produced on the device
+    // one contiguous range per XCD, or one range for all when the work
+    // unit order matters more than locality (Work::kOneRange)
+    uint32_t xch = Work::kOneRange ? n : (n + kXcds - 1) / kXcds;
+    if (xch == 0) xch = 1;
     Work work = work_arg;
     work.prepare();
     const uint32_t tid = threadIdx.x;
@@ -648,6 +659,14 @@ rti_kernel(Tree t, Work work_arg, uint32_t n, const uint32_t *n_dev,
     const uint32_t gthreads = gridDim.x * bdim;
     const uint32_t lane = lane_id();
 
+    // XCD-aware hand-out: the units are split into kXcds contiguous ranges
+    // with a counter each (own 256-byte line); a wave draws from the range
+    // of the XCD it runs on (workgroups go round-robin over the XCDs), so
+    // neighbouring units share that XCD's L2 and the counters are not all
+    // hammered from every XCD, and moves on to the next range when its own
+    // is used up.  A work type whose queue is ordered longest-first keeps
+    // one range (all waves share its order).
+    uint32_t xr = blockIdx.x % kXcds, xtries = 0;
     bool active = false;
     bool exhausted = false;
     uint32_t cur = 0;
@@ -670,15 +689,24 @@ rti_kernel(Tree t, Work work_arg, uint32_t n, const uint32_t *n_dev,
             (__popcll(idle) >= refill || idle == ~0ull)) {
             uint32_t cnt = (uint32_t)__popcll(idle);
             uint32_t leader = (uint32_t)(__ffsll((long long)idle) - 1);
+            // range x owns chunks x, x + kXcds, ... of xch units
+            const uint32_t x = xr;
+            const uint32_t nch = (n + xch - 1) / xch;
+            const uint32_t r_len =
+                x < nch ? ((nch - x + kXcds - 1) / kXcds) * xch : 0u;
             uint32_t base = 0;
-            if (lane == leader) base = atomicAdd(&ctrl->next, cnt);
+            if (lane == leader) base = atomicAdd(&xnext[x * kXcdStride], cnt);
             base = __shfl(base, (int)leader);
             base = __builtin_amdgcn_readfirstlane(base);
-            if (base + cnt >= n) exhausted = true;
+            if (base + cnt >= r_len) {  // range x is used up: go on to the next
+                xr = (x + 1) % kXcds;
+                if (++xtries == kXcds) exhausted = true;
+            }
             if (!active) {
-                uint32_t my = base + lanes_below(idle);
-                if (my < n) {
-                    work.begin(my, st, cur, org, d);
+                const uint32_t my = base + lanes_below(idle);
+                const uint32_t u = ((my / xch) * kXcds + x) * xch + my % xch;
+                if (my < r_len && u < n) {
+                    work.begin(u, st, cur, org, d);
                     sp = lo = 0;
                     ph = PH_TRAV;
                     active = true;
@@ -915,7 +943,8 @@ rti_kernel(Tree t, Work work_arg, uint32_t n, const uint32_t *n_dev,
         uint32_t total = gridDim.x * (bdim >> 6);
         uint32_t done = atomicAdd(&ctrl->left, 1u);
         if (done == total - 1) {
-            atomicExch(&ctrl->next, 0u);
+            for (uint32_t x = 0; x < kXcds; x++)
+                atomicExch(&xnext[x * kXcdStride], 0u);
             atomicExch(&ctrl->qcount, 0u);
             atomicExch(&ctrl->qback, 0u);
             atomicExch(&ctrl->qtotal, 0u);
@@ -1162,6 +1191,7 @@ thread_local int g_device = 0;
 
 struct Workspace {
     dev::Ctrl *ctrl = nullptr;
+    uint32_t *xnext = nullptr;  // per-XCD unit counters (kXcds x 256 bytes)
     uint4 *spill = nullptr;
     size_t spill_bytes = 0;
     uint32_t *queue = nullptr;  // block queue of the render pre-pass
@@ -1206,6 +1236,9 @@ int get_ws(hipStream_t s, size_t spill_bytes, Workspace **out) {
         slot.reset(new Workspace());
         HIPCHK(hipMalloc(&slot->ctrl, sizeof(dev::Ctrl)));
         HIPCHK(hipMemset(slot->ctrl, 0, sizeof(dev::Ctrl)));
+        const size_t xb = dev::kXcds * dev::kXcdStride * sizeof(uint32_t);
+        HIPCHK(hipMalloc(&slot->xnext, xb));
+        HIPCHK(hipMemset(slot->xnext, 0, xb));
     }
     if (spill_bytes > slot->spill_bytes) {
         if (slot->spill) HIPCHK(hipFree(slot->spill));
@@ -1312,7 +1345,8 @@ int launch_k(const dev::Tree &dt, uint32_t stack_bound, const Work &work,
                               8 * sizeof(unsigned long long), stream));
     }
     hipLaunchKernelGGL(kern, dim3(grid), dim3(block), lds, stream, dt, work,
-                       (uint32_t)n, n_dev, ws->ctrl, ws->spill, slots,
+                       (uint32_t)n, n_dev, ws->ctrl, ws->xnext, ws->spill,
+                       slots,
                        (uint32_t)cfg.refill, (uint32_t)cfg.finish_batch,
                        (uint32_t)cfg.leaf_batch);
     HIPCHK(hipGetLastError());
