@@ -379,3 +379,25 @@ def test_lane_batching_changes_nothing(gpu, oracle, leaf_batch, finish_batch):
     assert cnt["node_visits"] == st["node_visits"]
     assert cnt["voxel_tests"] == st["voxel_tests"]
     assert cnt["fast_blocks"] == st["fast_blocks"]
+
+
+@pytest.mark.parametrize("chunk", [1, 4])
+def test_ray_batch_every_size_is_complete(gpu, oracle, chunk):
+    """Units are handed out from per-XCD ranges with stealing; ray counts
+    around the range and chunk boundaries must still trace every ray once
+    (outputs start as a sentinel that no traced ray writes)."""
+    from voxvision_amd import scenes, voxtrees as VT
+    c1 = _c1()
+    T = oracle.Tree(c1["dots"])
+    G = VT.make_tree(c1["dots"])
+    o, d = scenes.random_rays(5000, *c1["bbox"], seed=31)
+    vox_o, hit_o = T.rays(o, d)
+    old = VT.get_launch_config()
+    try:
+        VT.set_launch_config(chunk=chunk)
+        for n in (1, 2, 7, 8, 9, 63, 64, 65, 511, 1000, 4097, 5000):
+            vox_g, hit_g = VT.ray_tree_intersection(G, o[:n], d[:n])
+            torch.cuda.synchronize()
+            _assert_rays_equal(vox_g, hit_g, vox_o[:n], hit_o[:, :n])
+    finally:
+        VT.set_launch_config(**old)

@@ -145,7 +145,9 @@ def ray_tree_intersection(tree, o, d, want_leaf=False, hit_init=None,
     n = o.shape[0]
     oc = o.t().contiguous()
     dc = d.t().contiguous()
-    vox = torch.empty(n, dtype=torch.int32, device="cuda")
+    # every traced ray overwrites its entry (an id or VOXG_MISS); -2 marks
+    # a ray that was not traced
+    vox = torch.full((n,), -2, dtype=torch.int32, device="cuda")
     leaf = torch.empty(n, dtype=torch.int32, device="cuda") if want_leaf \
         else None
     if hit_init is None:
