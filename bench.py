@@ -414,7 +414,8 @@ def main():
         from voxvision_amd import autotune
         cfg, best_ms, hist = autotune.autotune(
             lambda cfg: step, params=["stack_lds", "waves_per_cu", "refill",
-                                      "block", "leaf_batch", "finish_batch"],
+                                      "block", "leaf_batch", "finish_batch",
+                                      "split_best"],
             reps=2)
         tuned = {"config": cfg, "ms": round(best_ms, 3),
                  "candidates": len(hist)}

@@ -404,6 +404,9 @@ int voxg_read_raw_data(const char *filename, const unsigned dim[3],
  *   finish_batch  lanes of a wave whose traversal ended before the wave
  *                 stores their results and starts their next rays (1..64;
  *                 fewer when no other lane is left)
+ *   split_best    BEST-mode voxg_render: a lane takes 16 >> split_best
+ *                 pixels of a 4x4 block at a time (0..4; the other modes
+ *                 always take whole blocks)
  * None of them changes any result. */
 typedef struct {
     int stack_lds;
@@ -414,6 +417,7 @@ typedef struct {
     int render_cull;
     int leaf_batch;
     int finish_batch;
+    int split_best;
 } voxg_launch_config;
 int voxg_set_launch_config(const voxg_launch_config *cfg);
 int voxg_get_launch_config(voxg_launch_config *cfg);

@@ -317,8 +317,9 @@ def test_render_adaptive_ct_frames(gpu, oracle):
 
 @pytest.mark.parametrize("quality", [0, 1, 2, 6, 10])
 def test_render_with_and_without_block_culling(gpu, oracle, quality):
-    """The pre-pass that finishes blocks missing the scene box changes no
-    output and no counter, in every quality mode."""
+    """The pre-pass that finishes blocks missing the scene box, and BEST
+    mode's half-block units, change no output and no counter, in every
+    quality mode."""
     from voxvision_amd import voxrnd, voxtrees as VT
     c1 = _c1()
     G = VT.make_tree(c1["dots"])
@@ -327,8 +328,8 @@ def test_render_with_and_without_block_culling(gpu, oracle, quality):
     old = VT.get_launch_config()
     res = []
     try:
-        for cull in (0, 1):
-            VT.set_launch_config(render_cull=cull)
+        for cull, split in ((0, 0), (1, 0), (1, 1), (0, 2), (1, 3), (1, 4)):
+            VT.set_launch_config(render_cull=cull, split_best=split)
             out = voxrnd.render(G, [c1["camera"]], W, H, quality=quality,
                                 texture=tex)
             cnt = voxrnd.render_counts(G, [c1["camera"]], W, H,
@@ -337,12 +338,13 @@ def test_render_with_and_without_block_culling(gpu, oracle, quality):
             res.append((out, cnt))
     finally:
         VT.set_launch_config(**old)
-    (a, ca), (b, cb) = res
-    assert torch.equal(a["voxel"], b["voxel"])
-    assert torch.equal(a["squares"], b["squares"])
-    m = a["voxel"] != -1
-    assert torch.equal(a["hit"][:, m], b["hit"][:, m])
-    assert ca == cb
+    (a, ca) = res[0]
+    for b, cb in res[1:]:
+        assert torch.equal(a["voxel"], b["voxel"])
+        assert torch.equal(a["squares"], b["squares"])
+        m = a["voxel"] != -1
+        assert torch.equal(a["hit"][:, m], b["hit"][:, m])
+        assert ca == cb
     T = oracle.Tree(c1["dots"])
     _, vox_o, _, st = T.render(c1["camera"], W, H, quality, texture=tex)
     assert np.array_equal(_np_u32(b["voxel"]), vox_o)

@@ -78,7 +78,8 @@ class LaunchConfig(C.Structure):
     _fields_ = [("stack_lds", C.c_int), ("block", C.c_int),
                 ("waves_per_cu", C.c_int), ("refill", C.c_int),
                 ("chunk", C.c_int), ("render_cull", C.c_int),
-                ("leaf_batch", C.c_int), ("finish_batch", C.c_int)]
+                ("leaf_batch", C.c_int), ("finish_batch", C.c_int),
+                ("split_best", C.c_int)]
 
 
 _lib = None

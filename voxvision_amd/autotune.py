@@ -5,7 +5,8 @@ per-lane LDS stack ring (`stack_lds`), threads per workgroup (`block`),
 persistent waves per CU (`waves_per_cu`), idle lanes that trigger a refill
 (`refill`), rays a lane takes per fetch (`chunk`), the render pre-pass
 (`render_cull`) and the lanes a wave gathers before it searches leaves
-(`leaf_batch`) or finishes traversals (`finish_batch`).  The search is
+(`leaf_batch`) or finishes traversals (`finish_batch`), and the pixels of
+a 4x4 block a lane takes at a time in BEST mode (`split_best`).  The search is
 coordinate descent from the current configuration: each parameter in turn
 is swept over its candidate values with the others fixed, keeping the
 fastest, until a full pass changes nothing.  Timing uses HIP events on the
@@ -29,6 +30,7 @@ SPACE = {
     "render_cull": [0, 1],
     "leaf_batch": [1, 8, 16, 24, 32],
     "finish_batch": [1, 8, 16, 24, 32],
+    "split_best": [0, 1, 2],
 }
 
 

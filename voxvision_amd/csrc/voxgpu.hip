@@ -335,6 +335,7 @@ struct Ctrl {
 template <class Src, class Sink>
 struct ChunkWork {
     static constexpr bool kOneRange = false;
+    __device__ __forceinline__ uint32_t unit_shift() const { return 0; }
     Src src;
     Sink sink;
     uint32_t n, chunk;
@@ -377,6 +378,7 @@ struct BatchItem {
 template <class Src, class Sink>
 struct MultiWork {
     static constexpr bool kOneRange = false;
+    __device__ __forceinline__ uint32_t unit_shift() const { return 0; }
     Src src;
     Sink sink;
     const BatchItem *items;
@@ -458,6 +460,13 @@ struct BlockWork {
     const uint32_t *qfront;     // the first *qfront units from the front,
     uint32_t qlen;              // the rest from the back of qlen entries
     uint32_t qf;                // *qfront, read once by prepare()
+    uint32_t split;             // !MULTI, BEST mode: a unit is 16 >> split
+                                // pixels of a block (finer grain at the end
+                                // of the launch)
+    __device__ __forceinline__ uint32_t unit_shift() const {
+        if constexpr (MULTI) return 0;
+        else return split;
+    }
     // read once per thread what the block pre-pass counted on the device
     __device__ __forceinline__ void prepare() {
         if constexpr (!MULTI) {
@@ -472,6 +481,7 @@ struct BlockWork {
         f3 inter_last, c1_inter, c1_dir;
         int xs, ys;
         uint32_t fast, best, merged;
+        uint32_t own;  // this unit counts its block (the first half does)
     };
 
     __device__ __forceinline__ const RenderItem &item(const State &st) const {
@@ -511,15 +521,18 @@ struct BlockWork {
             trace_pixel(st, p, st.leaf_try ? st.leaf : 0u, cur, org, d);
             return true;
         }
-        st.fast += (st.mode == VOXG_QUALITY_FAST) ? 1u : 0u;
-        st.best += (st.mode == VOXG_QUALITY_FAST) ? 0u : 1u;
-        st.merged += st.merge ? 1u : 0u;
+        st.fast += (st.mode == VOXG_QUALITY_FAST) ? st.own : 0u;
+        st.best += (st.mode == VOXG_QUALITY_FAST) ? 0u : st.own;
+        st.merged += st.merge ? st.own : 0u;
         return false;
     }
     __device__ __forceinline__ void begin(uint32_t unit, State &st,
                                           uint32_t &cur, f3 &org, f3 &d) const {
         st.item = 0;
+        uint32_t part = 0;
         if constexpr (!MULTI) {
+            part = unit & ((1u << split) - 1u);
+            unit >>= split;
             if (queue) unit = queue[unit < qf ? unit : qlen - 1u - (unit - qf)];
         }
         if constexpr (MULTI) {
@@ -543,8 +556,15 @@ struct BlockWork {
         st.leaf = kEnd;
         st.vid_last = kEnd;
         st.inter_last = f3{0, 0, 0};
-        st.i = 0;
-        st.iend = 16;
+        // BEST-mode pixels do not depend on each other (renderer.c:300-308)
+        if constexpr (MULTI) {
+            st.i = 0;
+            st.iend = 16;
+        } else {
+            st.i = (int)(part * (16u >> split));
+            st.iend = st.i + (int)(16u >> split);
+        }
+        st.own = part ? 0u : 1u;
         if (st.mode == VOXG_QUALITY_ADAPTIVE) {
             // renderer.c:240-253
             st.merge = (it.merge_mode == VOXG_QUALITY_RAY_MERGE);
@@ -646,13 +666,14 @@ rti_kernel(Tree t, Work work_arg, uint32_t n, const uint32_t *n_dev,
            uint32_t spill_slots, uint32_t refill, uint32_t fin_batch,
            uint32_t leaf_batch) {
     extern __shared__ uint4 ring[];  // [K][blockDim.x]
-    if (n_dev) n = *n_dev;  // unit count produced on the device
+    Work work = work_arg;
+    work.prepare();
+    // unit count produced on the device (queue entries, split or not)
+    if (n_dev) n = *n_dev << work.unit_shift();
     // one contiguous range per XCD, or one range for all when the work
     // unit order matters more than locality (Work::kOneRange)
     uint32_t xch = Work::kOneRange ? n : (n + kXcds - 1) / kXcds;
     if (xch == 0) xch = 1;
-    Work work = work_arg;
-    work.prepare();
     const uint32_t tid = threadIdx.x;
     const uint32_t bdim = blockDim.x;
     const uint32_t gtid = blockIdx.x * bdim + tid;
@@ -1181,7 +1202,7 @@ struct voxg_batch {
 namespace {
 
 std::mutex g_mu;
-voxg_launch_config g_cfg = {8, 256, 20, 8, 4, 1, 16, 16};
+voxg_launch_config g_cfg = {8, 256, 20, 8, 4, 1, 16, 16, 1};
 
 voxg_launch_config current_config() {
     std::lock_guard<std::mutex> g(g_mu);
@@ -1619,6 +1640,8 @@ int voxg_set_launch_config(const voxg_launch_config *cfg) {
         return fail(VOXG_EINVAL, "leaf_batch must be in 1..64");
     if (cfg->finish_batch < 1 || cfg->finish_batch > 64)
         return fail(VOXG_EINVAL, "finish_batch must be in 1..64");
+    if (cfg->split_best < 0 || cfg->split_best > 4)
+        return fail(VOXG_EINVAL, "split_best must be in 0..4");
     std::lock_guard<std::mutex> g(g_mu);
     g_cfg = *cfg;
     return 0;
@@ -1834,9 +1857,11 @@ static int render_launch(const voxg_tree *t, size_t n_frames,
     work.one.merge_mode = (int)(q & 0xc);
     work.one.sink = sink;
     size_t nblk = npix / 16;
-    if (!current_config().render_cull || t->host.nodes.empty())
-        return launch<dev::BlockWork<false>, COUNT>(t, work, nblk, stream,
-                                                    counts);
+    const voxg_launch_config cfg = current_config();
+    work.split = (work.one.rnd_mode == VOXG_QUALITY_BEST) ? (uint32_t)cfg.split_best : 0u;
+    if (!cfg.render_cull || t->host.nodes.empty())
+        return launch<dev::BlockWork<false>, COUNT>(t, work, nblk << work.split,
+                                                    stream, counts);
     // pre-pass: finish the blocks whose rays all miss the root box and
     // queue the others for the traversal kernel
     Workspace *ws = nullptr;
@@ -1857,7 +1882,7 @@ static int render_launch(const voxg_tree *t, size_t n_frames,
     work.qfront = &ws->ctrl->qcount;
     work.qlen = (uint32_t)nblk;
     return launch_tree<dev::BlockWork<false>, COUNT>(
-        dev_tree(t), t->stack_bound, work, nblk, stream, counts,
+        dev_tree(t), t->stack_bound, work, nblk << work.split, stream, counts,
         &ws->ctrl->qtotal, false);
 }
 

@@ -260,7 +260,8 @@ def read_raw_data(filename, dim, samplesize, lo=None, hi=None, threshold=None,
 def set_launch_config(**fields):
     """Change fields of the traversal launch configuration
     (voxg_launch_config: stack_lds, block, waves_per_cu, refill, chunk,
-    render_cull, leaf_batch, finish_batch); returns the new configuration."""
+    render_cull, leaf_batch, finish_batch, split_best); returns the new
+    configuration."""
     L = _lib.load()
     cfg = _lib.LaunchConfig()
     check(L.voxg_get_launch_config(C.byref(cfg)))
