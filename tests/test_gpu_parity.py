@@ -403,3 +403,46 @@ def test_ray_batch_every_size_is_complete(gpu, oracle, chunk):
             _assert_rays_equal(vox_g, hit_g, vox_o[:n], hit_o[:, :n])
     finally:
         VT.set_launch_config(**old)
+
+
+def test_block_pre_pass_never_drops_a_hit(gpu, oracle):
+    """The pre-pass finishes a block only when the scene box lies outside
+    the cone of its corner rays by a margin.  Cameras close to the box, at
+    its silhouette, inside it and far away, with narrow and wide fields of
+    view: the render with the pre-pass equals the one without, pixel for
+    pixel, and the oracle's."""
+    from voxvision_amd import camera, scenes, voxrnd, voxtrees as VT
+    c1 = _c1()
+    T = oracle.Tree(c1["dots"])
+    G = VT.make_tree(c1["dots"])
+    bmin, bmax = c1["bbox"]
+    ctr = (bmin + bmax) / 2
+    diag = float(np.linalg.norm(bmax - bmin))
+    rng = np.random.default_rng(41)
+    W, H = 160, 96
+    cams = []
+    for k in range(24):
+        dist = [0.3, 0.55, 0.75, 1.0, 3.0, 20.0][k % 6] * diag
+        u = rng.normal(size=3)
+        u /= np.linalg.norm(u)
+        pos = ctr + dist * u
+        # look at the centre, or graze the box edge
+        tgt = ctr if k % 2 == 0 else bmin + (bmax - bmin) * rng.random(3)
+        fov = [np.tan(np.radians(30)), np.tan(np.radians(5)),
+               np.tan(np.radians(60))][k % 3]
+        cams.append(camera.make_camera(pos, tgt, fov, W, H))
+    old = VT.get_launch_config()
+    outs = []
+    try:
+        for cull in (0, 1):
+            VT.set_launch_config(render_cull=cull)
+            o = voxrnd.render(G, cams, W, H, want_colors=False)
+            torch.cuda.synchronize()
+            outs.append(o)
+    finally:
+        VT.set_launch_config(**old)
+    assert torch.equal(outs[0]["voxel"], outs[1]["voxel"])
+    vg = _np_u32(outs[1]["voxel"]).reshape(len(cams), -1)
+    for i in (0, 3, 7, 12, 17, 22):
+        _, vox_o, _, _ = T.render(cams[i], W, H, oracle.QUALITY_BEST)
+        assert np.array_equal(vg[i], vox_o)
