@@ -245,6 +245,9 @@ typedef struct {
     uint64_t rays;            /* searches started at the root */
     uint64_t leaf_traversals; /* FAST-mode searches started at a leaf */
     uint64_t fast_blocks, best_blocks, merge_blocks; /* render modes */
+    uint64_t stack_spills;    /* pending pairs moved from the LDS ring to
+                                 the HBM spill buffer (launch-config
+                                 dependent; not a reference counter) */
 } voxg_counts;
 int voxg_ray_batch_count(const voxg_tree *tree, size_t n, const float *ox,
                          const float *oy, const float *oz, const float *dx,
@@ -394,7 +397,8 @@ int voxg_read_raw_data(const char *filename, const unsigned dim[3],
  *   chunk         consecutive rays a lane traces before it needs new work
  *                 in voxg_ray_batch and the batched form (1..4096; lowered
  *                 for a launch too small to give every persistent lane a
- *                 unit; a render lane always takes one 4x4 block)
+ *                 unit; a render lane takes one 4x4 block, or part of
+ *                 one in BEST mode, see split_best)
  *   render_cull   1: voxg_render first finishes, in a coherent pre-pass,
  *                 the 4x4 blocks whose rays all miss the scene box; 0: every
  *                 block goes through the traversal

@@ -54,7 +54,7 @@ class Counts(C.Structure):
     _fields_ = [("node_visits", C.c_uint64), ("voxel_tests", C.c_uint64),
                 ("rays", C.c_uint64), ("leaf_traversals", C.c_uint64),
                 ("fast_blocks", C.c_uint64), ("best_blocks", C.c_uint64),
-                ("merge_blocks", C.c_uint64)]
+                ("merge_blocks", C.c_uint64), ("stack_spills", C.c_uint64)]
 
 
 class Camera(C.Structure):

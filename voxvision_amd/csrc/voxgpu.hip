@@ -317,9 +317,13 @@ struct Ctrl {
                        // whose rays enter the scene box, from the front
     unsigned long long visits, tests, rays, leaf_trav;
     unsigned long long fast_blocks, best_blocks, merge_blocks;
+    unsigned long long spills; // stack entries moved from LDS to HBM
     uint32_t qback;    // ... and the other entering blocks, from the back
     uint32_t qtotal;   // qcount + qback
 };
+// the counters cleared before a counting launch: visits .. spills
+constexpr size_t kCounterBytes =
+    offsetof(Ctrl, spills) + sizeof(unsigned long long) - offsetof(Ctrl, visits);
 
 // ---------------------------------------------------------------------------
 // work units driven by the persistent traversal kernel
@@ -702,7 +706,7 @@ rti_kernel(Tree t, Work work_arg, uint32_t n, const uint32_t *n_dev,
     typename Work::State st;
     std::memset(&st, 0, sizeof(st));
     int sp = 0, lo = 0;
-    unsigned long long nv = 0, nt = 0, nr = 0, nl = 0;
+    unsigned long long nv = 0, nt = 0, nr = 0, nl = 0, ns = 0;
 
     for (;;) {
         uint64_t idle = __ballot(!active);
@@ -835,6 +839,7 @@ rti_kernel(Tree t, Work work_arg, uint32_t n, const uint32_t *n_dev,
             } else {                                                     \
                 spill[(size_t)lo * gthreads + gtid] =                    \
                     ring[(lo & (K - 1)) * bdim + tid];                   \
+                if (COUNT) ns++;                                         \
             }                                                            \
             lo++;                                                        \
         }                                                                \
@@ -956,6 +961,7 @@ rti_kernel(Tree t, Work work_arg, uint32_t n, const uint32_t *n_dev,
         atomicAdd(&ctrl->tests, nt);
         atomicAdd(&ctrl->rays, nr);
         atomicAdd(&ctrl->leaf_trav, nl);
+        atomicAdd(&ctrl->spills, ns);
         work.flush(st, ctrl);
     }
     // self-reset of the unit counter by the last wave to leave
@@ -1069,8 +1075,7 @@ block_cull_kernel(RenderItem it, uint32_t npix, Ctrl *ctrl,
 }
 
 // ---------------------------------------------------------------------------
-// ball kernel (search.c:155-183): any-hit depth-first walk over children
-// 0..7 with escape links; the answer does not depend on the order.
+// ball kernel
 // ---------------------------------------------------------------------------
 // search.c:155-183: is there a voxel whose box comes closer than r to c
 // (rr = r * r)?  Any-hit depth-first walk over children 0..7 through the
@@ -1202,7 +1207,7 @@ struct voxg_batch {
 namespace {
 
 std::mutex g_mu;
-voxg_launch_config g_cfg = {8, 256, 20, 8, 4, 1, 16, 16, 1};
+voxg_launch_config g_cfg = {4, 256, 20, 8, 4, 1, 16, 16, 1};
 
 voxg_launch_config current_config() {
     std::lock_guard<std::mutex> g(g_mu);
@@ -1363,7 +1368,7 @@ int launch_k(const dev::Tree &dt, uint32_t stack_bound, const Work &work,
     }
     if (COUNT && reset_counts) {
         HIPCHK(hipMemsetAsync(&ws->ctrl->visits, 0,
-                              8 * sizeof(unsigned long long), stream));
+                              dev::kCounterBytes, stream));
     }
     hipLaunchKernelGGL(kern, dim3(grid), dim3(block), lds, stream, dt, work,
                        (uint32_t)n, n_dev, ws->ctrl, ws->xnext, ws->spill,
@@ -1385,6 +1390,7 @@ int launch_k(const dev::Tree &dt, uint32_t stack_bound, const Work &work,
         counts->fast_blocks = h.fast_blocks;
         counts->best_blocks = h.best_blocks;
         counts->merge_blocks = h.merge_blocks;
+        counts->stack_spills = h.spills;
     }
     return 0;
 }
@@ -1728,7 +1734,7 @@ static int ball_launch(const voxg_tree *t, size_t n, const float *cx,
     one.count = (uint32_t)n;
     if (counts) {
         HIPCHK(hipMemsetAsync(&ws->ctrl->visits, 0,
-                              8 * sizeof(unsigned long long), stream));
+                              dev::kCounterBytes, stream));
         hipLaunchKernelGGL((dev::ball_kernel<true, false>), dim3(grid),
                            dim3(block), 0, stream, one, nullptr, 0u,
                            (uint32_t)n, ws->ctrl);
@@ -1871,7 +1877,7 @@ static int render_launch(const voxg_tree *t, size_t n_frames,
     if (rc) return rc;
     if (COUNT)
         HIPCHK(hipMemsetAsync(&ws->ctrl->visits, 0,
-                              8 * sizeof(unsigned long long), stream));
+                              dev::kCounterBytes, stream));
     hipLaunchKernelGGL((dev::block_cull_kernel<COUNT>),
                        dim3((npix + dev::kCullTile - 1) / dev::kCullTile),
                        dim3(256), 0, stream,

@@ -181,7 +181,7 @@ def ray_counts(tree, o, d, stream=None):
                                  d[2].data_ptr(), C.byref(c),
                                  _stream(stream)), "voxg_ray_batch_count")
     return dict(node_visits=c.node_visits, voxel_tests=c.voxel_tests,
-                rays=c.rays)
+                rays=c.rays, stack_spills=c.stack_spills)
 
 
 def tree_ball_collidep(tree, c, r, stream=None):
