@@ -104,9 +104,10 @@ def batched_leg(ev0, ev1, stream, log):
 
     space = ["chunk", "waves_per_cu", "stack_lds", "refill", "leaf_batch",
              "finish_batch"]
-    pcfg, pms, _ = autotune.autotune(plain_runner, params=space, reps=2)
-    bcfg, bms, _ = autotune.autotune(batch_runner, params=space, reps=2)
-    VT.set_launch_config(**bcfg)
+    pcfg, pms, _ = autotune.autotune(plain_runner, params=space, reps=2,
+                                     kind="rays")
+    bcfg, bms, _ = autotune.autotune(batch_runner, params=space, reps=2,
+                                     kind="batch")
     B = batch.Batch(list(zip(trees, first, counts)))
     g = torch.cuda.CUDAGraph()
     gs = torch.cuda.Stream()
@@ -391,7 +392,7 @@ def main():
 
     L = _lib.load()
     _lib.check(L.voxg_init(0), "voxg_init")
-    default_cfg = VT.get_launch_config()
+    default_cfgs = VT.get_launch_configs()
     torch.cuda.set_device(0)
     stream = torch.cuda.current_stream()
 
@@ -416,7 +417,7 @@ def main():
             lambda cfg: step, params=["stack_lds", "waves_per_cu", "refill",
                                       "block", "leaf_batch", "finish_batch",
                                       "split_best"],
-            reps=2)
+            reps=2, kind="render_best")
         tuned = {"config": cfg, "ms": round(best_ms, 3),
                  "candidates": len(hist)}
         log("[bench] autotuned render config %s (%.3f ms)" % (cfg, best_ms))
@@ -482,12 +483,11 @@ def main():
                 G.handle, 1 << 24, *[t.data_ptr() for t in so + sd],
                 vo.data_ptr(), None, *[t.data_ptr() for t in ho],
                 C.c_void_p(stream.cuda_stream)), "voxg_ray_batch")
-        render_cfg = VT.get_launch_config()
         if not args.no_tune:
             from voxvision_amd import autotune
             icfg, _, _ = autotune.autotune(
                 lambda cfg: inc, params=["leaf_batch", "finish_batch", "chunk",
-                                         "refill"], reps=2)
+                                         "refill"], reps=2, kind="rays")
             extra["incoherent_launch"] = icfg
         inc()
         torch.cuda.synchronize()
@@ -498,7 +498,6 @@ def main():
         torch.cuda.synchronize()
         ims = ev0.elapsed_time(ev1) / 5
         extra["incoherent_16M_mrays_s"] = round((1 << 24) / ims / 1e3, 1)
-        VT.set_launch_config(**render_cfg)
         cb, rb = scenes.random_balls(1 << 22, bmin, bmax, seed=13)
         cbt = [torch.from_numpy(np.ascontiguousarray(cb[:, k])).cuda()
                for k in range(3)]
@@ -526,7 +525,7 @@ def main():
     if not args.no_batched:
         extra["batched"] = batched_leg(ev0, ev1, stream, log)
     if not args.no_serving:
-        VT.set_launch_config(**default_cfg)
+        VT.set_launch_configs(default_cfgs)
         extra["serving"] = serving_leg(log, args.serve_seconds,
                                        args.serve_rate)
 
@@ -548,8 +547,8 @@ def main():
                    "outputs": "voxel id + hit point per pixel"},
         "roofline": roofline, "cpu_baseline": cpu, "extra": extra,
     }
-    if tuned:
-        line["config"]["launch"] = tuned["config"]
+    line["config"]["launch"] = (tuned["config"] if tuned
+                                else default_cfgs["render_best"])
     print(json.dumps(line), flush=True)
 
 

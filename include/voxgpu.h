@@ -388,7 +388,15 @@ int voxg_read_raw_data(const char *filename, const unsigned dim[3],
                        const float voxel[3], voxg_tree **out);
 
 /* Launch configuration of the traversal kernels (autotuned by
- * voxvision_amd.autotune; defaults are sane).
+ * voxvision_amd.autotune).  Each kind of launch has one of its own, with a
+ * default that is the fastest measured for it on MI355X:
+ *   VOXG_LAUNCH_RAYS         voxg_ray_batch, voxg_ray_batch_count
+ *   VOXG_LAUNCH_RENDER       voxg_render and voxg_render_count in FAST and
+ *                            ADAPTIVE modes (one 4x4 block per unit)
+ *   VOXG_LAUNCH_RENDER_BEST  the same in BEST mode (one pixel per unit by
+ *                            default, handed out to large groups of lanes)
+ *   VOXG_LAUNCH_BATCH        voxg_batch_ray and the serving loop's launches
+ * Fields:
  *   stack_lds     stack entries per lane kept in LDS (power of two 2..32;
  *                 deeper stacks spill to HBM and stay exact)
  *   block         threads per workgroup (64, 128, 192 or 256)
@@ -411,7 +419,9 @@ int voxg_read_raw_data(const char *filename, const unsigned dim[3],
  *   split_best    BEST-mode voxg_render: a lane takes 16 >> split_best
  *                 pixels of a 4x4 block at a time (0..4; the other modes
  *                 always take whole blocks)
- * None of them changes any result. */
+ * A kind ignores the fields it has no use for (chunk in renders, split_best
+ * outside BEST renders, render_cull outside renders).  None of them changes
+ * any result. */
 typedef struct {
     int stack_lds;
     int block;
@@ -423,8 +433,18 @@ typedef struct {
     int finish_batch;
     int split_best;
 } voxg_launch_config;
+#define VOXG_LAUNCH_RAYS 0
+#define VOXG_LAUNCH_RENDER 1
+#define VOXG_LAUNCH_RENDER_BEST 2
+#define VOXG_LAUNCH_BATCH 3
+#define VOXG_LAUNCH_KINDS 4
+/* Sets the configuration of every kind of launch. */
 int voxg_set_launch_config(const voxg_launch_config *cfg);
+/* The configuration of VOXG_LAUNCH_RAYS. */
 int voxg_get_launch_config(voxg_launch_config *cfg);
+/* One kind's configuration (VOXG_EINVAL for an unknown kind). */
+int voxg_set_kind_launch_config(int kind, const voxg_launch_config *cfg);
+int voxg_get_kind_launch_config(int kind, voxg_launch_config *cfg);
 
 /* Human-readable text of the last error on this thread. */
 const char *voxg_last_error(void);

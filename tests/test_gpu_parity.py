@@ -207,8 +207,7 @@ def test_stack_spill_is_exact(gpu, oracle):
     T = oracle.Tree(dots)
     o, d = scenes.random_rays(60000, *scenes.bbox_of(dots), seed=12)
     vox_o, hit_o = T.rays(o, d)
-    old = VT.get_launch_config()
-    try:
+    with VT.saved_launch_configs():
         spills, visits = {}, set()
         for k in (2, 8, 32):
             VT.set_launch_config(stack_lds=k, block=64 if k == 32 else 256)
@@ -220,8 +219,33 @@ def test_stack_spill_is_exact(gpu, oracle):
             visits.add((c["node_visits"], c["voxel_tests"]))
         assert len(visits) == 1
         assert spills[2] > spills[8] >= spills[32] == 0
-    finally:
-        VT.set_launch_config(**old)
+
+
+def test_launch_kinds_reach_their_launches(gpu, oracle):
+    """Each kind of launch runs with its own configuration: a 2-entry
+    stack ring set for BEST renders only makes those spill to HBM, while
+    ray batches and ADAPTIVE renders (32 entries) do not; results stay the
+    oracle's."""
+    from voxvision_amd import scenes, voxrnd, voxtrees as VT
+    c1 = _c1()
+    T = oracle.Tree(c1["dots"])
+    G = VT.make_tree(c1["dots"])
+    W, H = c1["W"], c1["H"]
+    cam = [c1["camera"]]
+    o, d = scenes.random_rays(20000, *c1["bbox"], seed=5)
+    with VT.saved_launch_configs():
+        VT.set_launch_config(stack_lds=32, block=64)
+        VT.set_launch_config("render_best", stack_lds=2, block=256)
+        best = voxrnd.render_counts(G, cam, W, H, quality=0)
+        adap = voxrnd.render_counts(G, cam, W, H, quality=2)
+        rays = VT.ray_counts(G, o, d)
+        out = voxrnd.render(G, cam, W, H, quality=0, want_colors=False)
+        torch.cuda.synchronize()
+    assert best["stack_spills"] > 0
+    assert adap["stack_spills"] == 0 and rays["stack_spills"] == 0
+    _, vox_o, hit_o, st = T.render(c1["camera"], W, H, 0)
+    _assert_rays_equal(out["voxel"], out["hit"], vox_o, hit_o)
+    assert best["node_visits"] == st["node_visits"]
 
 
 def test_empty_tree_and_empty_batches(gpu, oracle):
@@ -332,9 +356,8 @@ def test_render_with_and_without_block_culling(gpu, oracle, quality):
     G = VT.make_tree(c1["dots"])
     tex = np.random.default_rng(4).integers(0, 256, 32 ** 3).astype(np.uint8)
     W, H = c1["W"], c1["H"]
-    old = VT.get_launch_config()
     res = []
-    try:
+    with VT.saved_launch_configs():
         for cull, split in ((0, 0), (1, 0), (1, 1), (0, 2), (1, 3), (1, 4)):
             VT.set_launch_config(render_cull=cull, split_best=split)
             out = voxrnd.render(G, [c1["camera"]], W, H, quality=quality,
@@ -343,8 +366,6 @@ def test_render_with_and_without_block_culling(gpu, oracle, quality):
                                        quality=quality)
             torch.cuda.synchronize()
             res.append((out, cnt))
-    finally:
-        VT.set_launch_config(**old)
     (a, ca) = res[0]
     for b, cb in res[1:]:
         assert torch.equal(a["voxel"], b["voxel"])
@@ -373,18 +394,19 @@ def test_lane_batching_changes_nothing(gpu, oracle, leaf_batch, finish_batch):
     W, H = c1["W"], c1["H"]
     q = 2 | 8
     _, pvox_o, phit_o, st = T.render(c1["camera"], W, H, q)
-    old = VT.get_launch_config()
-    try:
+    with VT.saved_launch_configs():
         VT.set_launch_config(leaf_batch=leaf_batch, finish_batch=finish_batch)
         vox_g, hit_g = VT.ray_tree_intersection(G, o, d)
         out = voxrnd.render(G, [c1["camera"]], W, H, quality=q,
                             want_colors=False)
         cnt = voxrnd.render_counts(G, [c1["camera"]], W, H, quality=q)
+        outb = voxrnd.render(G, [c1["camera"]], W, H, quality=0,
+                             want_colors=False)
         torch.cuda.synchronize()
-    finally:
-        VT.set_launch_config(**old)
     _assert_rays_equal(vox_g, hit_g, vox_o, hit_o)
     _assert_rays_equal(out["voxel"], out["hit"], pvox_o, phit_o)
+    _, bvox_o, bhit_o, _ = T.render(c1["camera"], W, H, 0)
+    _assert_rays_equal(outb["voxel"], outb["hit"], bvox_o, bhit_o)
     assert cnt["node_visits"] == st["node_visits"]
     assert cnt["voxel_tests"] == st["voxel_tests"]
     assert cnt["fast_blocks"] == st["fast_blocks"]
@@ -401,23 +423,19 @@ def test_ray_batch_every_size_is_complete(gpu, oracle, chunk):
     G = VT.make_tree(c1["dots"])
     o, d = scenes.random_rays(5000, *c1["bbox"], seed=31)
     vox_o, hit_o = T.rays(o, d)
-    old = VT.get_launch_config()
-    try:
+    with VT.saved_launch_configs():
         VT.set_launch_config(chunk=chunk)
         for n in (1, 2, 7, 8, 9, 63, 64, 65, 511, 1000, 4097, 5000):
             vox_g, hit_g = VT.ray_tree_intersection(G, o[:n], d[:n])
             torch.cuda.synchronize()
             _assert_rays_equal(vox_g, hit_g, vox_o[:n], hit_o[:, :n])
-    finally:
-        VT.set_launch_config(**old)
 
 
 def test_block_pre_pass_never_drops_a_hit(gpu, oracle):
-    """The pre-pass finishes a block only when the scene box lies outside
-    the cone of its corner rays by a margin.  Cameras close to the box, at
-    its silhouette, inside it and far away, with narrow and wide fields of
-    view: the render with the pre-pass equals the one without, pixel for
-    pixel, and the oracle's."""
+    """The pre-pass finishes a block only when none of its 16 rays enters
+    the scene box.  Cameras close to the box, at its silhouette, inside it
+    and far away, with narrow and wide fields of view: the render with the
+    pre-pass equals the one without, pixel for pixel, and the oracle's."""
     from voxvision_amd import camera, scenes, voxrnd, voxtrees as VT
     c1 = _c1()
     T = oracle.Tree(c1["dots"])
@@ -438,16 +456,13 @@ def test_block_pre_pass_never_drops_a_hit(gpu, oracle):
         fov = [np.tan(np.radians(30)), np.tan(np.radians(5)),
                np.tan(np.radians(60))][k % 3]
         cams.append(camera.make_camera(pos, tgt, fov, W, H))
-    old = VT.get_launch_config()
     outs = []
-    try:
+    with VT.saved_launch_configs():
         for cull in (0, 1):
             VT.set_launch_config(render_cull=cull)
             o = voxrnd.render(G, cams, W, H, want_colors=False)
             torch.cuda.synchronize()
             outs.append(o)
-    finally:
-        VT.set_launch_config(**old)
     assert torch.equal(outs[0]["voxel"], outs[1]["voxel"])
     vg = _np_u32(outs[1]["voxel"]).reshape(len(cams), -1)
     for i in (0, 3, 7, 12, 17, 22):

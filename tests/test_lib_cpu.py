@@ -2,6 +2,7 @@
 exports every symbol include/voxgpu.h declares, the reference tree layout
 matches src/voxtrees/tree.h:28-38, and the product's tree builder and
 flattener reproduce the oracle's vox_make_tree bit for bit."""
+import ctypes as C
 import os
 import re
 import subprocess
@@ -228,3 +229,35 @@ def test_edit_random_sequences_match_oracle(oracle, scene):
     R = T.rebuild()
     E.rebuild()
     assert E.hash() == R.hash()
+
+
+def test_launch_configs_per_kind():
+    """Each kind of launch keeps a configuration of its own; setting all
+    kinds at once, one kind, and the restore context manager; invalid
+    kinds and fields are refused.  Host state only, no GPU needed."""
+    from voxvision_amd import _lib, voxtrees as VT
+    with VT.saved_launch_configs() as saved:
+        assert set(saved) == {"rays", "render", "render_best", "batch"}
+        # BEST renders default to one pixel per unit, the others to blocks
+        assert saved["render_best"]["split_best"] == 4
+        VT.set_launch_config("render", refill=5)
+        assert VT.get_launch_config("render")["refill"] == 5
+        assert VT.get_launch_config("rays") == saved["rays"]
+        assert VT.get_launch_config("render_best") == saved["render_best"]
+        VT.set_launch_config(leaf_batch=7)  # every kind, other fields kept
+        for k in saved:
+            c = VT.get_launch_config(k)
+            assert c["leaf_batch"] == 7
+            assert c["split_best"] == saved[k]["split_best"]
+        with pytest.raises(ValueError):
+            VT.get_launch_config("nope")
+        with pytest.raises(_lib.VoxgError):
+            VT.set_launch_config("render_best", split_best=5)
+        L = _lib.load()
+        cfg = _lib.LaunchConfig()
+        assert L.voxg_get_kind_launch_config(4, C.byref(cfg)) != 0
+        assert L.voxg_get_kind_launch_config(-1, C.byref(cfg)) != 0
+        # the plain getter reports the ray-batch kind
+        assert L.voxg_get_launch_config(C.byref(cfg)) == 0
+        assert cfg.leaf_batch == 7 and cfg.refill == saved["rays"]["refill"]
+    assert VT.get_launch_configs() == saved

@@ -24,7 +24,8 @@ EXPORTS = [
     "voxg_tree_node_ptr", "voxg_ray_batch", "voxg_ball_batch",
     "voxg_camera_set_window_size", "voxg_render", "voxg_ray_batch_count",
     "voxg_ball_batch_count", "voxg_render_count", "voxg_set_launch_config",
-    "voxg_get_launch_config", "voxg_last_error", "voxg_batch_create",
+    "voxg_get_launch_config", "voxg_set_kind_launch_config",
+    "voxg_get_kind_launch_config", "voxg_last_error", "voxg_batch_create",
     "voxg_batch_ray", "voxg_batch_release", "voxg_server_create",
     "voxg_server_add_tree", "voxg_server_submit_render",
     "voxg_server_submit_balls", "voxg_server_wait", "voxg_server_get_stats",
@@ -126,6 +127,10 @@ def load():
                                     C.POINTER(Counts), P]
     L.voxg_set_launch_config.argtypes = [C.POINTER(LaunchConfig)]
     L.voxg_get_launch_config.argtypes = [C.POINTER(LaunchConfig)]
+    L.voxg_set_kind_launch_config.argtypes = [C.c_int,
+                                              C.POINTER(LaunchConfig)]
+    L.voxg_get_kind_launch_config.argtypes = [C.c_int,
+                                              C.POINTER(LaunchConfig)]
     L.voxg_last_error.restype = C.c_char_p
     L.voxg_batch_create.argtypes = [S, C.POINTER(BatchItem), C.POINTER(P)]
     L.voxg_batch_ray.argtypes = [P] + [P] * 6 + [P] * 5 + [P]

@@ -6,8 +6,9 @@ persistent waves per CU (`waves_per_cu`), idle lanes that trigger a refill
 (`refill`), rays a lane takes per fetch (`chunk`), the render pre-pass
 (`render_cull`) and the lanes a wave gathers before it searches leaves
 (`leaf_batch`) or finishes traversals (`finish_batch`), and the pixels of
-a 4x4 block a lane takes at a time in BEST mode (`split_best`).  The search is
-coordinate descent from the current configuration: each parameter in turn
+a 4x4 block a lane takes at a time in BEST mode (`split_best`), for one
+kind of launch ("rays", "render", "render_best", "batch").  The search is
+coordinate descent from the kind's current configuration: each parameter in turn
 is swept over its candidate values with the others fixed, keeping the
 fastest, until a full pass changes nothing.  Timing uses HIP events on the
 current stream; each candidate is warmed up once and timed over `reps`
@@ -25,12 +26,12 @@ SPACE = {
     "stack_lds": [2, 4, 8, 16],
     "block": [64, 128, 256],
     "waves_per_cu": [8, 12, 16, 20, 24, 32],
-    "refill": [1, 4, 8, 16, 32],
+    "refill": [1, 4, 8, 16, 32, 48, 64],
     "chunk": [1, 2, 4, 8, 16, 32],
     "render_cull": [0, 1],
-    "leaf_batch": [1, 8, 16, 24, 32],
-    "finish_batch": [1, 8, 16, 24, 32],
-    "split_best": [0, 1, 2],
+    "leaf_batch": [1, 8, 16, 24, 32, 48, 64],
+    "finish_batch": [1, 8, 16, 24, 32, 48, 64],
+    "split_best": [0, 1, 2, 3, 4],
 }
 
 
@@ -38,9 +39,10 @@ def _valid(cfg):
     return cfg["stack_lds"] * cfg["block"] * 16 <= 64 * 1024
 
 
-def time_config(make_runner, cfg, reps=3):
-    """Milliseconds per run of the workload under `cfg`."""
-    VT.set_launch_config(**cfg)
+def time_config(make_runner, cfg, reps=3, kind=None):
+    """Milliseconds per run of the workload under `cfg` (set for `kind`, or
+    for every kind when None)."""
+    VT.set_launch_config(kind, **cfg)
     run = make_runner(cfg)
     run()
     torch.cuda.synchronize()
@@ -55,13 +57,15 @@ def time_config(make_runner, cfg, reps=3):
 
 
 def autotune(make_runner, params=None, start=None, reps=3, max_passes=2,
-             log=None):
+             log=None, kind=None):
     """make_runner(cfg) -> zero-argument callable running the workload once
     (built after the configuration is set, for objects such as batches
-    whose layout depends on it).  Returns (best_cfg, best_ms, history)."""
+    whose layout depends on it); `kind` is the kind of launch the workload
+    makes.  Leaves the best configuration set for that kind and returns
+    (best_cfg, best_ms, history)."""
     params = list(params or SPACE.keys())
-    cfg = dict(start or VT.get_launch_config())
-    best = time_config(make_runner, cfg, reps)
+    cfg = dict(start or VT.get_launch_config(kind or "rays"))
+    best = time_config(make_runner, cfg, reps, kind)
     history = [(dict(cfg), best)]
     if log:
         log("start %s %.3f ms" % (json.dumps(cfg), best))
@@ -75,7 +79,7 @@ def autotune(make_runner, params=None, start=None, reps=3, max_passes=2,
                 key = json.dumps(cand, sort_keys=True)
                 if key in seen or not _valid(cand):
                     continue
-                ms = time_config(make_runner, cand, reps)
+                ms = time_config(make_runner, cand, reps, kind)
                 seen[key] = ms
                 history.append((dict(cand), ms))
                 if ms < best * 0.99:
@@ -84,7 +88,7 @@ def autotune(make_runner, params=None, start=None, reps=3, max_passes=2,
                         log("  %s=%s -> %.3f ms" % (p, v, ms))
         if not changed:
             break
-    VT.set_launch_config(**cfg)
+    VT.set_launch_config(kind, **cfg)
     return cfg, best, history
 
 

@@ -338,7 +338,8 @@ constexpr size_t kCounterBytes =
 // waiting for a wave-wide refill.
 template <class Src, class Sink>
 struct ChunkWork {
-    static constexpr bool kOneRange = false;
+    // XCD hand-out: one contiguous range of the units per XCD
+    __device__ __forceinline__ uint32_t xcd_group() const { return 0; }
     __device__ __forceinline__ uint32_t unit_shift() const { return 0; }
     Src src;
     Sink sink;
@@ -381,7 +382,7 @@ struct BatchItem {
 // traverses that item's tree.
 template <class Src, class Sink>
 struct MultiWork {
-    static constexpr bool kOneRange = false;
+    __device__ __forceinline__ uint32_t xcd_group() const { return 0; }
     __device__ __forceinline__ uint32_t unit_shift() const { return 0; }
     Src src;
     Sink sink;
@@ -456,7 +457,6 @@ struct RenderItem {
 template <bool MULTI>
 struct BlockWork {
     // one render's block queue is ordered longest-first (block_cull_kernel)
-    static constexpr bool kOneRange = !MULTI;
     RenderItem one;             // the job when !MULTI
     const RenderItem *items;    // the jobs when MULTI, by unit_begin
     uint32_t n_items;
@@ -467,6 +467,14 @@ struct BlockWork {
     uint32_t split;             // !MULTI, BEST mode: a unit is 16 >> split
                                 // pixels of a block (finer grain at the end
                                 // of the launch)
+    uint32_t group;             // !MULTI: units dealt to the XCDs in turns
+                                // of `group` (the queue is ordered
+                                // longest-first, so every XCD should follow
+                                // that order); MULTI: contiguous ranges
+    __device__ __forceinline__ uint32_t xcd_group() const {
+        if constexpr (MULTI) return 0;
+        else return group;
+    }
     __device__ __forceinline__ uint32_t unit_shift() const {
         if constexpr (MULTI) return 0;
         else return split;
@@ -674,9 +682,10 @@ rti_kernel(Tree t, Work work_arg, uint32_t n, const uint32_t *n_dev,
     work.prepare();
     // unit count produced on the device (queue entries, split or not)
     if (n_dev) n = *n_dev << work.unit_shift();
-    // one contiguous range per XCD, or one range for all when the work
-    // unit order matters more than locality (Work::kOneRange)
-    uint32_t xch = Work::kOneRange ? n : (n + kXcds - 1) / kXcds;
+    // one contiguous range per XCD, or groups of xcd_group() units dealt
+    // to the XCDs in turn when the order of the units matters
+    const uint32_t xg = work.xcd_group();
+    uint32_t xch = xg ? xg : (n + kXcds - 1) / kXcds;
     if (xch == 0) xch = 1;
     const uint32_t tid = threadIdx.x;
     const uint32_t bdim = blockDim.x;
@@ -684,13 +693,15 @@ rti_kernel(Tree t, Work work_arg, uint32_t n, const uint32_t *n_dev,
     const uint32_t gthreads = gridDim.x * bdim;
     const uint32_t lane = lane_id();
 
-    // XCD-aware hand-out: the units are split into kXcds contiguous ranges
-    // with a counter each (own 256-byte line); a wave draws from the range
-    // of the XCD it runs on (workgroups go round-robin over the XCDs), so
-    // neighbouring units share that XCD's L2 and the counters are not all
-    // hammered from every XCD, and moves on to the next range when its own
-    // is used up.  A work type whose queue is ordered longest-first keeps
-    // one range (all waves share its order).
+    // XCD-aware hand-out: the units are cut into chunks of xch units and
+    // chunk j goes to range j % kXcds, each range with a counter of its own
+    // (own 256-byte line).  A wave draws from the range of the XCD it runs
+    // on (workgroups go round-robin over the XCDs), so neighbouring units
+    // share that XCD's L2 and no counter takes the atomics of the whole
+    // chip, and moves on to the next range when its own is used up.  With
+    // xch = n / kXcds each range is one contiguous eighth; a work type whose
+    // units are ordered longest-first deals small chunks instead, so every
+    // range follows that order.
     uint32_t xr = blockIdx.x % kXcds, xtries = 0;
     bool active = false;
     bool exhausted = false;
@@ -1207,11 +1218,19 @@ struct voxg_batch {
 namespace {
 
 std::mutex g_mu;
-voxg_launch_config g_cfg = {4, 256, 20, 8, 4, 1, 16, 16, 1};
+// stack_lds, block, waves_per_cu, refill, chunk, render_cull, leaf_batch,
+// finish_batch, split_best per kind of launch (fastest measured on the
+// 256^3 benchmark scene; every field is validated by voxg_set_*)
+voxg_launch_config g_cfg[VOXG_LAUNCH_KINDS] = {
+    {4, 256, 20, 8, 4, 1, 16, 16, 0},   // VOXG_LAUNCH_RAYS
+    {4, 256, 20, 8, 4, 1, 16, 16, 0},   // VOXG_LAUNCH_RENDER
+    {4, 256, 20, 48, 4, 1, 32, 64, 4},  // VOXG_LAUNCH_RENDER_BEST
+    {4, 256, 20, 8, 4, 1, 16, 16, 0},   // VOXG_LAUNCH_BATCH
+};
 
-voxg_launch_config current_config() {
+voxg_launch_config current_config(int kind) {
     std::lock_guard<std::mutex> g(g_mu);
-    return g_cfg;
+    return g_cfg[kind];
 }
 thread_local int g_device = 0;
 
@@ -1242,8 +1261,8 @@ int cu_count(int device) {
 
 // Rays per work unit for a launch of n rays: the configured chunk, cut so
 // that a small launch still has a unit for every persistent lane.
-uint32_t effective_chunk(size_t n) {
-    voxg_launch_config cfg = current_config();
+uint32_t effective_chunk(size_t n, int kind) {
+    voxg_launch_config cfg = current_config(kind);
     int devid = 0;
     if (hipGetDevice(&devid) != hipSuccess) devid = 0;
     uint64_t lanes = (uint64_t)cu_count(devid) *
@@ -1396,10 +1415,11 @@ int launch_k(const dev::Tree &dt, uint32_t stack_bound, const Work &work,
 }
 
 template <class Work, bool COUNT>
-int launch_tree(const dev::Tree &t, uint32_t stack_bound, const Work &work,
-                size_t n, hipStream_t stream, voxg_counts *counts,
-                const uint32_t *n_dev = nullptr, bool reset_counts = true) {
-    voxg_launch_config cfg = current_config();
+int launch_tree(int kind, const dev::Tree &t, uint32_t stack_bound,
+                const Work &work, size_t n, hipStream_t stream,
+                voxg_counts *counts, const uint32_t *n_dev = nullptr,
+                bool reset_counts = true) {
+    voxg_launch_config cfg = current_config(kind);
     switch (cfg.stack_lds) {
     case 2: return launch_k<2, Work, COUNT>(t, stack_bound, work, n, stream, cfg, counts, n_dev, reset_counts);
     case 4: return launch_k<4, Work, COUNT>(t, stack_bound, work, n, stream, cfg, counts, n_dev, reset_counts);
@@ -1411,10 +1431,10 @@ int launch_tree(const dev::Tree &t, uint32_t stack_bound, const Work &work,
 }
 
 template <class Work, bool COUNT>
-int launch(const voxg_tree *t, const Work &work, size_t n,
+int launch(int kind, const voxg_tree *t, const Work &work, size_t n,
            hipStream_t stream, voxg_counts *counts) {
-    return launch_tree<Work, COUNT>(dev_tree(t), t->stack_bound, work, n,
-                                    stream, counts);
+    return launch_tree<Work, COUNT>(kind, dev_tree(t), t->stack_bound, work,
+                                    n, stream, counts);
 }
 
 int check_tree(const voxg_tree *t) {
@@ -1625,7 +1645,7 @@ void voxg_camera_set_window_size(voxg_camera *cam, int w, int h) {
     }
 }
 
-int voxg_set_launch_config(const voxg_launch_config *cfg) {
+static int check_launch_config(const voxg_launch_config *cfg) {
     if (!cfg) return fail(VOXG_EINVAL, "cfg is NULL");
     int k = cfg->stack_lds;
     if (k != 2 && k != 4 && k != 8 && k != 16 && k != 32)
@@ -1648,15 +1668,42 @@ int voxg_set_launch_config(const voxg_launch_config *cfg) {
         return fail(VOXG_EINVAL, "finish_batch must be in 1..64");
     if (cfg->split_best < 0 || cfg->split_best > 4)
         return fail(VOXG_EINVAL, "split_best must be in 0..4");
+    return 0;
+}
+
+static int check_kind(int kind) {
+    if (kind < 0 || kind >= VOXG_LAUNCH_KINDS)
+        return fail(VOXG_EINVAL, "unknown kind of launch");
+    return 0;
+}
+
+int voxg_set_launch_config(const voxg_launch_config *cfg) {
+    int rc = check_launch_config(cfg);
+    if (rc) return rc;
     std::lock_guard<std::mutex> g(g_mu);
-    g_cfg = *cfg;
+    for (int k = 0; k < VOXG_LAUNCH_KINDS; k++) g_cfg[k] = *cfg;
     return 0;
 }
 
 int voxg_get_launch_config(voxg_launch_config *cfg) {
+    return voxg_get_kind_launch_config(VOXG_LAUNCH_RAYS, cfg);
+}
+
+int voxg_set_kind_launch_config(int kind, const voxg_launch_config *cfg) {
+    int rc = check_kind(kind);
+    if (!rc) rc = check_launch_config(cfg);
+    if (rc) return rc;
+    std::lock_guard<std::mutex> g(g_mu);
+    g_cfg[kind] = *cfg;
+    return 0;
+}
+
+int voxg_get_kind_launch_config(int kind, voxg_launch_config *cfg) {
+    int rc = check_kind(kind);
+    if (rc) return rc;
     if (!cfg) return fail(VOXG_EINVAL, "cfg is NULL");
     std::lock_guard<std::mutex> g(g_mu);
-    *cfg = g_cfg;
+    *cfg = g_cfg[kind];
     return 0;
 }
 
@@ -1681,12 +1728,12 @@ int voxg_ray_batch(const voxg_tree *t, size_t n, const float *ox,
     if ((hx || hy || hz) && !(hx && hy && hz))
         return fail(VOXG_EINVAL, "hx, hy, hz must be all set or all NULL");
     if (n == 0) return 0;
-    uint32_t c = effective_chunk(n);
+    uint32_t c = effective_chunk(n, VOXG_LAUNCH_RAYS);
     dev::ChunkWork<dev::SoaSrc, dev::SoaSink> work{
         dev::SoaSrc{ox, oy, oz, dx, dy, dz},
         dev::SoaSink{voxel_out, leaf_out, hx, hy, hz}, (uint32_t)n, c};
-    return launch<decltype(work), false>(t, work, (n + c - 1) / c, stream,
-                                         nullptr);
+    return launch<decltype(work), false>(VOXG_LAUNCH_RAYS, t, work,
+                                         (n + c - 1) / c, stream, nullptr);
 }
 
 int voxg_ray_batch_count(const voxg_tree *t, size_t n, const float *ox,
@@ -1698,13 +1745,13 @@ int voxg_ray_batch_count(const voxg_tree *t, size_t n, const float *ox,
     if (!out) return fail(VOXG_EINVAL, "out is NULL");
     std::memset(out, 0, sizeof(*out));
     if (n == 0) return 0;
-    uint32_t c = effective_chunk(n);
+    uint32_t c = effective_chunk(n, VOXG_LAUNCH_RAYS);
     dev::ChunkWork<dev::SoaSrc, dev::SoaSink> work{
         dev::SoaSrc{ox, oy, oz, dx, dy, dz},
         dev::SoaSink{nullptr, nullptr, nullptr, nullptr, nullptr},
         (uint32_t)n, c};
-    return launch<decltype(work), true>(t, work, (n + c - 1) / c, stream,
-                                        out);
+    return launch<decltype(work), true>(VOXG_LAUNCH_RAYS, t, work,
+                                        (n + c - 1) / c, stream, out);
 }
 
 static int ball_launch(const voxg_tree *t, size_t n, const float *cx,
@@ -1848,9 +1895,11 @@ static int render_launch(const voxg_tree *t, size_t n_frames,
     static_assert(sizeof(voxg_camera) == sizeof(dev::CamDev), "camera layout");
     unsigned q = params ? params->quality : VOXG_QUALITY_BEST;
     const dev::CamDev *cd = (const dev::CamDev *)cams;
-    // one 4x4 block per lane (renderer.c:214): in FAST / ADAPTIVE mode the
+    // one 4x4 block per unit (renderer.c:214): in FAST / ADAPTIVE mode the
     // leaf of one pixel is tried first for the next; in BEST mode every
-    // pixel is a root search, traced one after the other by the lane
+    // pixel is a root search, so a unit may be part of a block
+    // (split_best), down to one pixel: the lanes that a refill starts
+    // together then trace neighbouring pixels, coherent rays
     dev::BlockWork<false> work;
     std::memset(&work, 0, sizeof(work));
     work.one.t = dev_tree(t);
@@ -1863,11 +1912,17 @@ static int render_launch(const voxg_tree *t, size_t n_frames,
     work.one.merge_mode = (int)(q & 0xc);
     work.one.sink = sink;
     size_t nblk = npix / 16;
-    const voxg_launch_config cfg = current_config();
-    work.split = (work.one.rnd_mode == VOXG_QUALITY_BEST) ? (uint32_t)cfg.split_best : 0u;
+    const bool best = work.one.rnd_mode == VOXG_QUALITY_BEST;
+    const int kind = best ? VOXG_LAUNCH_RENDER_BEST : VOXG_LAUNCH_RENDER;
+    const voxg_launch_config cfg = current_config(kind);
+    work.split = best ? (uint32_t)cfg.split_best : 0u;
+    // units dealt to the XCDs in turns of 256 blocks' worth keep the
+    // longest-first order of the block queue on every XCD, and spread the
+    // refill atomics over the 8 per-XCD counters
+    work.group = 256u << work.split;
     if (!cfg.render_cull || t->host.nodes.empty())
-        return launch<dev::BlockWork<false>, COUNT>(t, work, nblk << work.split,
-                                                    stream, counts);
+        return launch<dev::BlockWork<false>, COUNT>(
+            kind, t, work, nblk << work.split, stream, counts);
     // pre-pass: finish the blocks whose rays all miss the root box and
     // queue the others for the traversal kernel
     Workspace *ws = nullptr;
@@ -1888,8 +1943,8 @@ static int render_launch(const voxg_tree *t, size_t n_frames,
     work.qfront = &ws->ctrl->qcount;
     work.qlen = (uint32_t)nblk;
     return launch_tree<dev::BlockWork<false>, COUNT>(
-        dev_tree(t), t->stack_bound, work, nblk << work.split, stream, counts,
-        &ws->ctrl->qtotal, false);
+        kind, dev_tree(t), t->stack_bound, work, nblk << work.split, stream,
+        counts, &ws->ctrl->qtotal, false);
 }
 
 }  // extern "C++"
@@ -1933,7 +1988,7 @@ int voxg_batch_create(size_t n_items, const voxg_batch_item *items,
     HIPCHK(hipGetDevice(&devid));
     uint64_t total = 0;
     for (size_t i = 0; i < n_items; i++) total += items[i].count;
-    uint32_t chunk = effective_chunk((size_t)total);
+    uint32_t chunk = effective_chunk((size_t)total, VOXG_LAUNCH_BATCH);
     std::vector<dev::BatchItem> h;
     uint64_t units = 0, rays_end = 0;
     uint32_t bound = 1;
@@ -2005,8 +2060,9 @@ int voxg_batch_ray(const voxg_batch *b, const float *ox, const float *oy,
         b->n_items, b->chunk};
     dev::Tree none;
     std::memset(&none, 0, sizeof(none));
-    return launch_tree<decltype(work), false>(none, b->stack_bound, work,
-                                              b->units, stream, nullptr);
+    return launch_tree<decltype(work), false>(VOXG_LAUNCH_BATCH, none,
+                                              b->stack_bound, work, b->units,
+                                              stream, nullptr);
 }
 
 }  // extern "C"
@@ -2238,7 +2294,8 @@ int srv_launch(voxg_server *s, SrvGroup &g) {
         dev::Tree none;
         std::memset(&none, 0, sizeof(none));
         int rc = launch_tree<dev::BlockWork<true>, false>(
-            none, s->stack_bound, work, units, ss.s, nullptr);
+            VOXG_LAUNCH_BATCH, none, s->stack_bound, work, units, ss.s,
+            nullptr);
         if (rc) return rc;
     }
     if (balls) {
@@ -2409,7 +2466,7 @@ int voxg_server_create(const voxg_server_config *cfg, voxg_server **out) {
 
 // Stream spill buffers sized for trees of stack bound `bound`.
 static int srv_fit_trees(voxg_server *s, uint32_t bound) {
-    voxg_launch_config c = current_config();
+    voxg_launch_config c = current_config(VOXG_LAUNCH_BATCH);
     uint32_t slots = bound > (uint32_t)c.stack_lds ? bound - c.stack_lds : 1;
     int cus = cu_count(s->device);
     size_t threads = (size_t)cus * std::max(1, c.waves_per_cu) * 64;

@@ -257,25 +257,65 @@ def read_raw_data(filename, dim, samplesize, lo=None, hi=None, threshold=None,
     return Tree(h, voxel)
 
 
-def set_launch_config(**fields):
+# kinds of traversal launch (voxg_set_kind_launch_config)
+LAUNCH_KINDS = {"rays": 0, "render": 1, "render_best": 2, "batch": 3}
+
+
+def _kind(kind):
+    if kind not in LAUNCH_KINDS:
+        raise ValueError("unknown kind of launch %r (one of %s)"
+                         % (kind, ", ".join(LAUNCH_KINDS)))
+    return LAUNCH_KINDS[kind]
+
+
+def set_launch_config(kind=None, **fields):
     """Change fields of the traversal launch configuration
     (voxg_launch_config: stack_lds, block, waves_per_cu, refill, chunk,
-    render_cull, leaf_batch, finish_batch, split_best); returns the new
-    configuration."""
+    render_cull, leaf_batch, finish_batch, split_best) of one kind of launch
+    ("rays", "render", "render_best", "batch"), or of every kind when kind
+    is None; the other fields keep each kind's values.  Returns the new
+    configuration of the kind (of "rays" when kind is None)."""
     L = _lib.load()
-    cfg = _lib.LaunchConfig()
-    check(L.voxg_get_launch_config(C.byref(cfg)))
-    names = [k for k, _ in cfg._fields_]
-    for k, v in fields.items():
-        if k not in names:
-            raise TypeError("unknown launch configuration field %r" % k)
-        if v is not None:
-            setattr(cfg, k, int(v))
-    check(L.voxg_set_launch_config(C.byref(cfg)), "voxg_set_launch_config")
-    return get_launch_config()
+    kinds = list(LAUNCH_KINDS) if kind is None else [kind]
+    for k in kinds:
+        cfg = _lib.LaunchConfig()
+        check(L.voxg_get_kind_launch_config(_kind(k), C.byref(cfg)))
+        names = [n for n, _ in cfg._fields_]
+        for n, v in fields.items():
+            if n not in names:
+                raise TypeError("unknown launch configuration field %r" % n)
+            if v is not None:
+                setattr(cfg, n, int(v))
+        check(L.voxg_set_kind_launch_config(_kind(k), C.byref(cfg)),
+              "voxg_set_kind_launch_config")
+    return get_launch_config(kind or "rays")
 
 
-def get_launch_config():
+def get_launch_config(kind="rays"):
     cfg = _lib.LaunchConfig()
-    check(_lib.load().voxg_get_launch_config(C.byref(cfg)))
+    check(_lib.load().voxg_get_kind_launch_config(_kind(kind),
+                                                  C.byref(cfg)))
     return {k: getattr(cfg, k) for k, _ in cfg._fields_}
+
+
+def get_launch_configs():
+    """Every kind's configuration, {kind: fields}."""
+    return {k: get_launch_config(k) for k in LAUNCH_KINDS}
+
+
+def set_launch_configs(cfgs):
+    """Restore configurations returned by get_launch_configs()."""
+    for k, f in cfgs.items():
+        set_launch_config(k, **f)
+
+
+class saved_launch_configs:
+    """Context manager that restores every kind's launch configuration."""
+
+    def __enter__(self):
+        self._saved = get_launch_configs()
+        return self._saved
+
+    def __exit__(self, *exc):
+        set_launch_configs(self._saved)
+        return False
