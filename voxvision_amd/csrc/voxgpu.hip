@@ -38,6 +38,7 @@
 #include <thread>
 #include <vector>
 
+#include "fastdiv.h"
 #include "tree_build.h"
 #include "voxg_internal.h"
 #include "voxg_layout.h"
@@ -444,7 +445,8 @@ __device__ __forceinline__ float sqr_metric(f3 a, f3 b) {
 struct RenderItem {
     Tree t;
     const CamDev *cams;
-    uint32_t W, ws, bpf, unit_begin;
+    uint32_t W, unit_begin;
+    FastDiv ws, bpf;  // blocks per block row, blocks per frame
     int rnd_mode, merge_mode;
     PixSink sink;
 };
@@ -487,7 +489,7 @@ struct BlockWork {
     }
     enum { CORNER1 = 0, CORNER2 = 1, LOOP = 2 };
     struct State {
-        uint32_t item, blk, base;  // job, block in the job, output index
+        uint32_t item, frame, base;  // job, frame in the job, output index
         int i, iend, stage, mode, merge, leaf_try, p;
         uint32_t leaf, vid_last;
         f3 inter_last, c1_inter, c1_dir;
@@ -502,7 +504,7 @@ struct BlockWork {
     }
     __device__ __forceinline__ const CamDev &cam(const State &st) const {
         const RenderItem &it = item(st);
-        return it.cams[st.blk / it.bpf];
+        return it.cams[st.frame];
     }
     __device__ __forceinline__ void emit(const State &st, int p, bool hit,
                                          uint32_t vid, f3 e) const {
@@ -558,11 +560,13 @@ struct BlockWork {
         }
         const RenderItem &it = item(st);
         uint32_t u = unit - it.unit_begin;
-        uint32_t cs = u % it.bpf;
-        st.blk = u;
-        st.base = (u / it.bpf) * it.bpf * 16u + cs * 16u;
-        st.xs = (int)(cs % it.ws) * 4;
-        st.ys = (int)(cs / it.ws) * 4;
+        // frames are stored one after the other in the square layout
+        st.frame = it.bpf.div(u);
+        const uint32_t cs = u - st.frame * it.bpf.d;
+        const uint32_t row = it.ws.div(cs);
+        st.base = u * 16u;
+        st.xs = (int)(cs - row * it.ws.d) * 4;
+        st.ys = (int)row * 4;
         st.mode = it.rnd_mode;
         st.merge = 0;
         st.leaf = kEnd;
@@ -1026,10 +1030,12 @@ block_cull_kernel(RenderItem it, uint32_t npix, Ctrl *ctrl,
         const uint32_t blk = r >> 4, p = r & 15u;
         bool enters = false;
         if (valid && !it.t.empty) {
-            const uint32_t cs = blk % it.bpf;
-            const CamDev &c = it.cams[blk / it.bpf];
-            const int x = (int)(cs % it.ws) * 4 + (int)(p & 3u);
-            const int y = (int)(cs / it.ws) * 4 + (int)(p >> 2);
+            const uint32_t f = it.bpf.div(blk);
+            const uint32_t cs = blk - f * it.bpf.d;
+            const uint32_t row = it.ws.div(cs);
+            const CamDev &c = it.cams[f];
+            const int x = (int)(cs - row * it.ws.d) * 4 + (int)(p & 3u);
+            const int y = (int)row * 4 + (int)(p >> 2);
             const f3 d = screen2world(c, x, y);
             f3 e;
             enters = hit_box(lo, hi, f3{c.px, c.py, c.pz}, d, e);
@@ -1222,7 +1228,7 @@ std::mutex g_mu;
 // finish_batch, split_best per kind of launch (fastest measured on the
 // 256^3 benchmark scene; every field is validated by voxg_set_*)
 voxg_launch_config g_cfg[VOXG_LAUNCH_KINDS] = {
-    {4, 256, 20, 8, 4, 1, 16, 16, 0},   // VOXG_LAUNCH_RAYS
+    {4, 256, 20, 8, 1, 1, 24, 16, 0},   // VOXG_LAUNCH_RAYS
     {4, 256, 20, 8, 4, 1, 16, 16, 0},   // VOXG_LAUNCH_RENDER
     {4, 256, 20, 48, 4, 1, 32, 64, 4},  // VOXG_LAUNCH_RENDER_BEST
     {4, 256, 20, 8, 4, 1, 16, 16, 0},   // VOXG_LAUNCH_BATCH
@@ -1905,8 +1911,8 @@ static int render_launch(const voxg_tree *t, size_t n_frames,
     work.one.t = dev_tree(t);
     work.one.cams = cd;
     work.one.W = (uint32_t)W;
-    work.one.ws = (uint32_t)W / 4u;
-    work.one.bpf = (uint32_t)(W / 4) * (uint32_t)(H / 4);
+    work.one.ws = make_fastdiv((uint32_t)W / 4u);
+    work.one.bpf = make_fastdiv((uint32_t)(W / 4) * (uint32_t)(H / 4));
     work.one.unit_begin = 0;
     work.one.rnd_mode = (int)(q & 3);
     work.one.merge_mode = (int)(q & 0xc);
@@ -2221,8 +2227,8 @@ int srv_launch(voxg_server *s, SrvGroup &g) {
             it.t = r->t;
             it.cams = nullptr;  // set after the table layout is known
             it.W = (uint32_t)r->rr.W;
-            it.ws = (uint32_t)r->rr.W / 4u;
-            it.bpf = (uint32_t)(np / 16);
+            it.ws = make_fastdiv((uint32_t)r->rr.W / 4u);
+            it.bpf = make_fastdiv((uint32_t)(np / 16));
             it.unit_begin = units;
             it.rnd_mode = (int)(r->rr.quality & 3);
             it.merge_mode = (int)(r->rr.quality & 0xc);
@@ -2245,7 +2251,7 @@ int srv_launch(voxg_server *s, SrvGroup &g) {
             it.sink.sh.ivx = it.t.ivx;
             it.sink.sh.ivy = it.t.ivy;
             it.sink.sh.ivz = it.t.ivz;
-            units += it.bpf;
+            units += it.bpf.d;
             ritems.push_back(it);
             dev::CamDev c;
             std::memcpy(&c, &r->rr.cam, sizeof(c));
