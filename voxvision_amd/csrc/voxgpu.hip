@@ -454,8 +454,9 @@ struct RenderItem {
 // One 4x4 block per unit (renderer.c:214-318): in FAST / ADAPTIVE mode the
 // leaf found for one pixel is tried first for the next, so the pixels of a
 // block are traced in order by one lane; in BEST mode every pixel is a
-// root search.  MULTI: units of several render jobs (different trees,
-// cameras, sizes, modes) in one launch.
+// root search (a single BEST render uses BestWork instead).  MULTI: units
+// of several render jobs (different trees, cameras, sizes, modes) in one
+// launch.
 template <bool MULTI>
 struct BlockWork {
     // one render's block queue is ordered longest-first (block_cull_kernel)
@@ -466,9 +467,6 @@ struct BlockWork {
     const uint32_t *qfront;     // the first *qfront units from the front,
     uint32_t qlen;              // the rest from the back of qlen entries
     uint32_t qf;                // *qfront, read once by prepare()
-    uint32_t split;             // !MULTI, BEST mode: a unit is 16 >> split
-                                // pixels of a block (finer grain at the end
-                                // of the launch)
     uint32_t group;             // !MULTI: units dealt to the XCDs in turns
                                 // of `group` (the queue is ordered
                                 // longest-first, so every XCD should follow
@@ -477,10 +475,8 @@ struct BlockWork {
         if constexpr (MULTI) return 0;
         else return group;
     }
-    __device__ __forceinline__ uint32_t unit_shift() const {
-        if constexpr (MULTI) return 0;
-        else return split;
-    }
+    __device__ __forceinline__ uint32_t unit_shift() const { return 0; }
+    uint32_t unit_shift_host() const { return 0; }
     // read once per thread what the block pre-pass counted on the device
     __device__ __forceinline__ void prepare() {
         if constexpr (!MULTI) {
@@ -495,7 +491,6 @@ struct BlockWork {
         f3 inter_last, c1_inter, c1_dir;
         int xs, ys;
         uint32_t fast, best, merged;
-        uint32_t own;  // this unit counts its block (the first half does)
     };
 
     __device__ __forceinline__ const RenderItem &item(const State &st) const {
@@ -535,18 +530,15 @@ struct BlockWork {
             trace_pixel(st, p, st.leaf_try ? st.leaf : 0u, cur, org, d);
             return true;
         }
-        st.fast += (st.mode == VOXG_QUALITY_FAST) ? st.own : 0u;
-        st.best += (st.mode == VOXG_QUALITY_FAST) ? 0u : st.own;
-        st.merged += st.merge ? st.own : 0u;
+        st.fast += (st.mode == VOXG_QUALITY_FAST) ? 1u : 0u;
+        st.best += (st.mode == VOXG_QUALITY_FAST) ? 0u : 1u;
+        st.merged += st.merge ? 1u : 0u;
         return false;
     }
     __device__ __forceinline__ void begin(uint32_t unit, State &st,
                                           uint32_t &cur, f3 &org, f3 &d) const {
         st.item = 0;
-        uint32_t part = 0;
         if constexpr (!MULTI) {
-            part = unit & ((1u << split) - 1u);
-            unit >>= split;
             if (queue) unit = queue[unit < qf ? unit : qlen - 1u - (unit - qf)];
         }
         if constexpr (MULTI) {
@@ -572,15 +564,8 @@ struct BlockWork {
         st.leaf = kEnd;
         st.vid_last = kEnd;
         st.inter_last = f3{0, 0, 0};
-        // BEST-mode pixels do not depend on each other (renderer.c:300-308)
-        if constexpr (MULTI) {
-            st.i = 0;
-            st.iend = 16;
-        } else {
-            st.i = (int)(part * (16u >> split));
-            st.iend = st.i + (int)(16u >> split);
-        }
-        st.own = part ? 0u : 1u;
+        st.i = 0;
+        st.iend = 16;
         if (st.mode == VOXG_QUALITY_ADAPTIVE) {
             // renderer.c:240-253
             st.merge = (it.merge_mode == VOXG_QUALITY_RAY_MERGE);
@@ -667,6 +652,77 @@ struct BlockWork {
         atomicAdd(&ctrl->fast_blocks, (unsigned long long)st.fast);
         atomicAdd(&ctrl->best_blocks, (unsigned long long)st.best);
         atomicAdd(&ctrl->merge_blocks, (unsigned long long)st.merged);
+    }
+};
+
+// BEST-mode render of one job: every pixel is a root search
+// (renderer.c:300-308), so a unit is 16 >> split pixels of a 4x4 block,
+// down to one pixel, taken in the block's order (renderer.c:23-25).  With
+// one pixel per unit, the lanes that one refill starts trace neighbouring
+// pixels together: coherent rays, the same nodes at the same time.  Keeps
+// only the state BEST mode needs, which leaves registers for more waves
+// than BlockWork.
+struct BestWork {
+    RenderItem one;
+    const uint32_t *queue;  // block queue of the pre-pass, or null: the
+    const uint32_t *qfront; // first *qfront entries from the front, the
+    uint32_t qlen, qf;      // rest from the back of qlen entries
+    uint32_t split;         // a unit is 16 >> split pixels
+    uint32_t group;         // units dealt to the XCDs in turns of `group`
+    __device__ __forceinline__ uint32_t xcd_group() const { return group; }
+    __device__ __forceinline__ uint32_t unit_shift() const { return split; }
+    uint32_t unit_shift_host() const { return split; }
+    __device__ __forceinline__ void prepare() {
+        if (queue) qf = *qfront;
+    }
+    struct State {
+        uint32_t frame, base;  // camera, output index of the block
+        int xs, ys, i, iend;   // block corner; pixels i..iend-1 of the order
+        uint32_t own;          // the unit holding pixel 0 counts the block
+        uint32_t best;         // blocks counted (counting launch)
+    };
+    __device__ __forceinline__ void trace(const State &st, uint32_t &cur,
+                                          f3 &org, f3 &d) const {
+        const int p = render_order(st.i);
+        const CamDev &c = one.cams[st.frame];
+        d = screen2world(c, st.xs + (p & 3), st.ys + (p >> 2));
+        org = f3{c.px, c.py, c.pz};
+        cur = 0;
+    }
+    __device__ __forceinline__ void begin(uint32_t unit, State &st,
+                                          uint32_t &cur, f3 &org, f3 &d) const {
+        const uint32_t part = unit & ((1u << split) - 1u);
+        uint32_t b = unit >> split;
+        if (queue) b = queue[b < qf ? b : qlen - 1u - (b - qf)];
+        // frames are stored one after the other in the square layout
+        st.frame = one.bpf.div(b);
+        const uint32_t cs = b - st.frame * one.bpf.d;
+        const uint32_t row = one.ws.div(cs);
+        st.base = b * 16u;
+        st.xs = (int)(cs - row * one.ws.d) * 4;
+        st.ys = (int)row * 4;
+        st.i = (int)(part * (16u >> split));
+        st.iend = st.i + (int)(16u >> split);
+        st.own = part ? 0u : 1u;
+        trace(st, cur, org, d);
+    }
+    __device__ __forceinline__ bool finish(State &st, bool hit, uint32_t vid,
+                                           uint32_t, f3 e, uint32_t &cur,
+                                           f3 &org, f3 &d) const {
+        one.sink.put(st.base + (uint32_t)render_order(st.i), hit, vid, 0, e);
+        if (++st.i < st.iend) {
+            trace(st, cur, org, d);
+            return true;
+        }
+        st.best += st.own;
+        return false;
+    }
+    __device__ __forceinline__ const Tree &tree(const State &,
+                                                const Tree &t) const {
+        return t;
+    }
+    __device__ __forceinline__ void flush(const State &st, Ctrl *ctrl) const {
+        atomicAdd(&ctrl->best_blocks, (unsigned long long)st.best);
     }
 };
 
@@ -1230,7 +1286,7 @@ std::mutex g_mu;
 voxg_launch_config g_cfg[VOXG_LAUNCH_KINDS] = {
     {4, 256, 20, 8, 1, 1, 24, 16, 0},   // VOXG_LAUNCH_RAYS
     {4, 256, 20, 8, 4, 1, 16, 16, 0},   // VOXG_LAUNCH_RENDER
-    {4, 256, 20, 48, 4, 1, 32, 64, 4},  // VOXG_LAUNCH_RENDER_BEST
+    {4, 256, 24, 48, 4, 1, 32, 64, 4},  // VOXG_LAUNCH_RENDER_BEST
     {4, 256, 20, 8, 4, 1, 16, 16, 0},   // VOXG_LAUNCH_BATCH
 };
 
@@ -1892,43 +1948,16 @@ static dev::PixSink make_sink(const voxg_tree *t,
     return sink;
 }
 
-template <bool COUNT>
-static int render_launch(const voxg_tree *t, size_t n_frames,
-                         const voxg_camera *cams, int W, int H,
-                         const voxg_render_params *params,
-                         const dev::PixSink &sink, size_t npix,
-                         voxg_counts *counts, hipStream_t stream) {
-    static_assert(sizeof(voxg_camera) == sizeof(dev::CamDev), "camera layout");
-    unsigned q = params ? params->quality : VOXG_QUALITY_BEST;
-    const dev::CamDev *cd = (const dev::CamDev *)cams;
-    // one 4x4 block per unit (renderer.c:214): in FAST / ADAPTIVE mode the
-    // leaf of one pixel is tried first for the next; in BEST mode every
-    // pixel is a root search, so a unit may be part of a block
-    // (split_best), down to one pixel: the lanes that a refill starts
-    // together then trace neighbouring pixels, coherent rays
-    dev::BlockWork<false> work;
-    std::memset(&work, 0, sizeof(work));
-    work.one.t = dev_tree(t);
-    work.one.cams = cd;
-    work.one.W = (uint32_t)W;
-    work.one.ws = make_fastdiv((uint32_t)W / 4u);
-    work.one.bpf = make_fastdiv((uint32_t)(W / 4) * (uint32_t)(H / 4));
-    work.one.unit_begin = 0;
-    work.one.rnd_mode = (int)(q & 3);
-    work.one.merge_mode = (int)(q & 0xc);
-    work.one.sink = sink;
-    size_t nblk = npix / 16;
-    const bool best = work.one.rnd_mode == VOXG_QUALITY_BEST;
-    const int kind = best ? VOXG_LAUNCH_RENDER_BEST : VOXG_LAUNCH_RENDER;
-    const voxg_launch_config cfg = current_config(kind);
-    work.split = best ? (uint32_t)cfg.split_best : 0u;
-    // units dealt to the XCDs in turns of 256 blocks' worth keep the
-    // longest-first order of the block queue on every XCD, and spread the
-    // refill atomics over the 8 per-XCD counters
-    work.group = 256u << work.split;
+// One render: the block pre-pass (when enabled) and the traversal over the
+// blocks it queued, with units of `Work` (BlockWork<false> or BestWork).
+template <class Work, bool COUNT>
+static int render_run(const voxg_tree *t, Work &work, int kind,
+                      const voxg_launch_config &cfg, size_t npix,
+                      voxg_counts *counts, hipStream_t stream) {
+    const size_t nblk = npix / 16;
+    const size_t units = nblk << work.unit_shift_host();
     if (!cfg.render_cull || t->host.nodes.empty())
-        return launch<dev::BlockWork<false>, COUNT>(
-            kind, t, work, nblk << work.split, stream, counts);
+        return launch<Work, COUNT>(kind, t, work, units, stream, counts);
     // pre-pass: finish the blocks whose rays all miss the root box and
     // queue the others for the traversal kernel
     Workspace *ws = nullptr;
@@ -1948,9 +1977,55 @@ static int render_launch(const voxg_tree *t, size_t n_frames,
     work.queue = ws->queue;
     work.qfront = &ws->ctrl->qcount;
     work.qlen = (uint32_t)nblk;
-    return launch_tree<dev::BlockWork<false>, COUNT>(
-        kind, dev_tree(t), t->stack_bound, work, nblk << work.split, stream,
-        counts, &ws->ctrl->qtotal, false);
+    return launch_tree<Work, COUNT>(kind, dev_tree(t), t->stack_bound, work,
+                                    units, stream, counts, &ws->ctrl->qtotal,
+                                    false);
+}
+
+template <bool COUNT>
+static int render_launch(const voxg_tree *t, size_t n_frames,
+                         const voxg_camera *cams, int W, int H,
+                         const voxg_render_params *params,
+                         const dev::PixSink &sink, size_t npix,
+                         voxg_counts *counts, hipStream_t stream) {
+    static_assert(sizeof(voxg_camera) == sizeof(dev::CamDev), "camera layout");
+    unsigned q = params ? params->quality : VOXG_QUALITY_BEST;
+    dev::RenderItem one;
+    std::memset(&one, 0, sizeof(one));
+    one.t = dev_tree(t);
+    one.cams = (const dev::CamDev *)cams;
+    one.W = (uint32_t)W;
+    one.ws = make_fastdiv((uint32_t)W / 4u);
+    one.bpf = make_fastdiv((uint32_t)(W / 4) * (uint32_t)(H / 4));
+    one.unit_begin = 0;
+    one.rnd_mode = (int)(q & 3);
+    one.merge_mode = (int)(q & 0xc);
+    one.sink = sink;
+    if (one.rnd_mode == VOXG_QUALITY_BEST) {
+        // every pixel is a root search: units of 16 >> split_best pixels,
+        // down to one; the lanes that a refill starts together then trace
+        // neighbouring pixels, coherent rays
+        const voxg_launch_config cfg = current_config(VOXG_LAUNCH_RENDER_BEST);
+        dev::BestWork work;
+        std::memset(&work, 0, sizeof(work));
+        work.one = one;
+        work.split = (uint32_t)cfg.split_best;
+        // units dealt to the XCDs in turns of 256 blocks' worth keep the
+        // longest-first order of the block queue on every XCD, and spread
+        // the refill atomics over the 8 per-XCD counters
+        work.group = 256u << work.split;
+        return render_run<dev::BestWork, COUNT>(
+            t, work, VOXG_LAUNCH_RENDER_BEST, cfg, npix, counts, stream);
+    }
+    // one 4x4 block per unit (renderer.c:214): the leaf of one pixel is
+    // tried first for the next
+    const voxg_launch_config cfg = current_config(VOXG_LAUNCH_RENDER);
+    dev::BlockWork<false> work;
+    std::memset(&work, 0, sizeof(work));
+    work.one = one;
+    work.group = 256u;
+    return render_run<dev::BlockWork<false>, COUNT>(
+        t, work, VOXG_LAUNCH_RENDER, cfg, npix, counts, stream);
 }
 
 }  // extern "C++"
