@@ -339,6 +339,8 @@ constexpr size_t kCounterBytes =
 // waiting for a wave-wide refill.
 template <class Src, class Sink>
 struct ChunkWork {
+    // no occupancy floor asked of the register allocator
+    static constexpr int kMinWavesPerSimd = 1;
     // XCD hand-out: one contiguous range of the units per XCD
     __device__ __forceinline__ uint32_t xcd_group() const { return 0; }
     __device__ __forceinline__ uint32_t unit_shift() const { return 0; }
@@ -383,6 +385,7 @@ struct BatchItem {
 // traverses that item's tree.
 template <class Src, class Sink>
 struct MultiWork {
+    static constexpr int kMinWavesPerSimd = 1;
     __device__ __forceinline__ uint32_t xcd_group() const { return 0; }
     __device__ __forceinline__ uint32_t unit_shift() const { return 0; }
     Src src;
@@ -459,6 +462,7 @@ struct RenderItem {
 // launch.
 template <bool MULTI>
 struct BlockWork {
+    static constexpr int kMinWavesPerSimd = 1;
     // one render's block queue is ordered longest-first (block_cull_kernel)
     RenderItem one;             // the job when !MULTI
     const RenderItem *items;    // the jobs when MULTI, by unit_begin
@@ -663,6 +667,9 @@ struct BlockWork {
 // only the state BEST mode needs, which leaves registers for more waves
 // than BlockWork.
 struct BestWork {
+    // 7 waves per SIMD (at most 72 VGPRs): the compiler fits this kernel
+    // without spilling, and the extra wave hides node-load latency
+    static constexpr int kMinWavesPerSimd = 7;
     RenderItem one;
     const uint32_t *queue;  // block queue of the pre-pass, or null: the
     const uint32_t *qfront; // first *qfront entries from the front, the
@@ -677,15 +684,18 @@ struct BestWork {
     }
     struct State {
         uint32_t frame, base;  // camera, output index of the block
-        int xs, ys, i, iend;   // block corner; pixels i..iend-1 of the order
-        uint32_t own;          // the unit holding pixel 0 counts the block
+        uint32_t xy;           // block corner: x | y << 16
+        uint32_t pos;          // pixel i | end << 8 | own << 16, where
+                               // pixels i..end-1 of the block's order are
+                               // this unit's, and own: it counts the block
         uint32_t best;         // blocks counted (counting launch)
     };
     __device__ __forceinline__ void trace(const State &st, uint32_t &cur,
                                           f3 &org, f3 &d) const {
-        const int p = render_order(st.i);
+        const int p = render_order((int)(st.pos & 0xffu));
         const CamDev &c = one.cams[st.frame];
-        d = screen2world(c, st.xs + (p & 3), st.ys + (p >> 2));
+        d = screen2world(c, (int)(st.xy & 0xffffu) + (p & 3),
+                         (int)(st.xy >> 16) + (p >> 2));
         org = f3{c.px, c.py, c.pz};
         cur = 0;
     }
@@ -699,22 +709,22 @@ struct BestWork {
         const uint32_t cs = b - st.frame * one.bpf.d;
         const uint32_t row = one.ws.div(cs);
         st.base = b * 16u;
-        st.xs = (int)(cs - row * one.ws.d) * 4;
-        st.ys = (int)row * 4;
-        st.i = (int)(part * (16u >> split));
-        st.iend = st.i + (int)(16u >> split);
-        st.own = part ? 0u : 1u;
+        st.xy = ((cs - row * one.ws.d) * 4u) | ((row * 4u) << 16);
+        const uint32_t i = part * (16u >> split);
+        st.pos = i | ((i + (16u >> split)) << 8) | ((part ? 0u : 1u) << 16);
         trace(st, cur, org, d);
     }
     __device__ __forceinline__ bool finish(State &st, bool hit, uint32_t vid,
                                            uint32_t, f3 e, uint32_t &cur,
                                            f3 &org, f3 &d) const {
-        one.sink.put(st.base + (uint32_t)render_order(st.i), hit, vid, 0, e);
-        if (++st.i < st.iend) {
+        one.sink.put(st.base + (uint32_t)render_order((int)(st.pos & 0xffu)),
+                     hit, vid, 0, e);
+        st.pos++;
+        if ((st.pos & 0xffu) < ((st.pos >> 8) & 0xffu)) {
             trace(st, cur, org, d);
             return true;
         }
-        st.best += st.own;
+        st.best += st.pos >> 16;
         return false;
     }
     __device__ __forceinline__ const Tree &tree(const State &,
@@ -733,6 +743,7 @@ constexpr uint32_t kXcds = 8;        // XCDs of an MI355X
 constexpr uint32_t kXcdStride = 64;  // counters 256 bytes apart
 template <int K, class Work, bool COUNT>
 __global__ void __launch_bounds__(256)
+__attribute__((amdgpu_waves_per_eu(COUNT ? 1 : Work::kMinWavesPerSimd, 8)))
 rti_kernel(Tree t, Work work_arg, uint32_t n, const uint32_t *n_dev,
            Ctrl *ctrl, uint32_t *xnext, uint4 *__restrict__ spill,
            uint32_t spill_slots, uint32_t refill, uint32_t fin_batch,
@@ -1286,7 +1297,7 @@ std::mutex g_mu;
 voxg_launch_config g_cfg[VOXG_LAUNCH_KINDS] = {
     {4, 256, 20, 8, 1, 1, 24, 16, 0},   // VOXG_LAUNCH_RAYS
     {4, 256, 20, 8, 4, 1, 16, 16, 0},   // VOXG_LAUNCH_RENDER
-    {4, 256, 24, 48, 4, 1, 32, 64, 4},  // VOXG_LAUNCH_RENDER_BEST
+    {4, 256, 28, 48, 4, 1, 32, 64, 4},  // VOXG_LAUNCH_RENDER_BEST
     {4, 256, 20, 8, 4, 1, 16, 16, 0},   // VOXG_LAUNCH_BATCH
 };
 
