@@ -339,7 +339,8 @@ constexpr size_t kCounterBytes =
 // waiting for a wave-wide refill.
 template <class Src, class Sink>
 struct ChunkWork {
-    // no occupancy floor asked of the register allocator
+    // no occupancy floor asked of the register allocator (it fits 7 waves
+    // per SIMD as it is; 8 would spill)
     static constexpr int kMinWavesPerSimd = 1;
     // XCD hand-out: one contiguous range of the units per XCD
     __device__ __forceinline__ uint32_t xcd_group() const { return 0; }
@@ -385,7 +386,8 @@ struct BatchItem {
 // traverses that item's tree.
 template <class Src, class Sink>
 struct MultiWork {
-    static constexpr int kMinWavesPerSimd = 1;
+    // 6 waves per SIMD (at most 80 VGPRs) fit without spilling
+    static constexpr int kMinWavesPerSimd = 6;
     __device__ __forceinline__ uint32_t xcd_group() const { return 0; }
     __device__ __forceinline__ uint32_t unit_shift() const { return 0; }
     Src src;
@@ -462,6 +464,7 @@ struct RenderItem {
 // launch.
 template <bool MULTI>
 struct BlockWork {
+    // 5 waves per SIMD as compiled; a floor of 6 spills and is slower
     static constexpr int kMinWavesPerSimd = 1;
     // one render's block queue is ordered longest-first (block_cull_kernel)
     RenderItem one;             // the job when !MULTI
@@ -1295,10 +1298,10 @@ std::mutex g_mu;
 // finish_batch, split_best per kind of launch (fastest measured on the
 // 256^3 benchmark scene; every field is validated by voxg_set_*)
 voxg_launch_config g_cfg[VOXG_LAUNCH_KINDS] = {
-    {4, 256, 20, 8, 1, 1, 24, 16, 0},   // VOXG_LAUNCH_RAYS
+    {4, 256, 28, 8, 1, 1, 24, 16, 0},   // VOXG_LAUNCH_RAYS
     {4, 256, 20, 8, 4, 1, 16, 16, 0},   // VOXG_LAUNCH_RENDER
     {4, 256, 28, 48, 4, 1, 32, 64, 4},  // VOXG_LAUNCH_RENDER_BEST
-    {4, 256, 20, 8, 4, 1, 16, 16, 0},   // VOXG_LAUNCH_BATCH
+    {4, 256, 24, 8, 1, 1, 16, 16, 0},   // VOXG_LAUNCH_BATCH
 };
 
 voxg_launch_config current_config(int kind) {
