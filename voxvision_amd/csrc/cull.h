@@ -1,0 +1,79 @@
+// Conservative ray / box test of the render pre-pass.
+//
+// may_enter(lo, hi, o, d, rcp) is false only when the ray o + t d, t >= 0,
+// misses the box [lo, hi] enlarged by M = 2^-17 S on every side, where S
+// is the largest magnitude among the coordinates of o, lo and hi.  Every
+// ray that the exact test of the traversal (hit_box, geom-sse.c:84-116)
+// lets in passes within 2^-21 S of the box: its hit point is the correctly
+// rounded o + m d (or a clamped coordinate of o) for a quotient m rounded
+// once.  This test's slab bounds ((lo - M - o) * rcp(d), two rounded
+// subtractions, a reciprocal within 1 ulp, a rounded product) are off by
+// less than 2^-20 S / |d_i|, so a ray that the exact test lets in always
+// passes here.  Inputs for which those bounds do not hold (non-finite
+// values, coordinates beyond 2^40, a direction with all components below
+// 2^-30) give true; a component below 2^-60 is treated as zero (the ray is
+// then parallel to that slab for any t at which the other coordinates
+// could still be inside the box).
+//
+// rcp(x) must return 1/x within 1 ulp for 2^-60 <= |x| <= 2^128.
+#pragma once
+#include <math.h>
+
+// What may_enter needs of the box and the origin, computed once per origin
+// (a camera position is the origin of all of its pixels' rays).
+struct CullOrigin {
+    float a[3], b[3];  // (lo - M) - o and (hi + M) - o per axis
+    bool unsure;       // o not finite or too large: every ray may enter
+};
+
+__host__ __device__ inline CullOrigin cull_origin(const float lo[3],
+                                                  const float hi[3],
+                                                  const float o[3]) {
+    float S = 0.0f;
+    bool finite = true;
+    for (int i = 0; i < 3; ++i) {
+        finite = finite && fabsf(o[i]) <= 0x1p127f;
+        S = fmaxf(S, fmaxf(fabsf(o[i]), fmaxf(fabsf(lo[i]), fabsf(hi[i]))));
+    }
+    const float M = fmaxf(S * 0x1p-17f, 0x1p-100f);
+    CullOrigin c;
+    for (int i = 0; i < 3; ++i) {
+        c.a[i] = (lo[i] - M) - o[i];
+        c.b[i] = (hi[i] + M) - o[i];
+    }
+    c.unsure = !finite || S > 0x1p40f;
+    return c;
+}
+
+// written without branches: the pre-pass runs it for every pixel
+template <class Rcp>
+__host__ __device__ inline bool may_enter_dir(const CullOrigin &c,
+                                              const float d[3], Rcp rcp) {
+    float dmax = 0.0f;
+    bool finite = true;
+    for (int i = 0; i < 3; ++i) {
+        finite = finite && fabsf(d[i]) <= 0x1p127f;
+        dmax = fmaxf(dmax, fabsf(d[i]));
+    }
+    float t0 = 0.0f, t1 = INFINITY;
+    for (int i = 0; i < 3; ++i) {
+        const bool par = !(fabsf(d[i]) >= 0x1p-60f);
+        const float r = rcp(par ? 1.0f : d[i]);
+        float ta = c.a[i] * r, tb = c.b[i] * r;
+        // parallel to the slab: all t, or none when outside it (an
+        // interval before t = 0, so that t1 < 0 <= t0)
+        const bool out = c.a[i] > 0.0f || c.b[i] < 0.0f;
+        ta = par ? (out ? -1.0f : -INFINITY) : ta;
+        tb = par ? (out ? -1.0f : INFINITY) : tb;
+        t0 = fmaxf(t0, fminf(ta, tb));
+        t1 = fminf(t1, fmaxf(ta, tb));
+    }
+    return c.unsure || !finite || dmax < 0x1p-30f || t0 <= t1;
+}
+
+template <class Rcp>
+__host__ __device__ inline bool may_enter(const float lo[3], const float hi[3],
+                                          const float o[3], const float d[3],
+                                          Rcp rcp) {
+    return may_enter_dir(cull_origin(lo, hi, o), d, rcp);
+}

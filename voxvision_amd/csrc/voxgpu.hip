@@ -38,6 +38,7 @@
 #include <thread>
 #include <vector>
 
+#include "cull.h"
 #include "fastdiv.h"
 #include "tree_build.h"
 #include "voxg_internal.h"
@@ -314,13 +315,15 @@ struct Ctrl {
     uint32_t spare;    // (units are handed out by the per-XCD counters)
     uint32_t left;     // waves that have finished
     uint32_t overflow; // set when a stack bound was exceeded (never expected)
-    uint32_t qcount;   // block queue built by block_cull: blocks all of
-                       // whose rays enter the scene box, from the front
+    uint32_t pad;
     unsigned long long visits, tests, rays, leaf_trav;
     unsigned long long fast_blocks, best_blocks, merge_blocks;
     unsigned long long spills; // stack entries moved from LDS to HBM
-    uint32_t qback;    // ... and the other entering blocks, from the back
-    uint32_t qtotal;   // qcount + qback
+    // block queue built by block_cull_kernel, claimed with one 64-bit
+    // atomic per workgroup: low word, blocks all of whose rays enter the
+    // scene box (queued from the front); high word, the other entering
+    // blocks (queued from the back)
+    unsigned long long qpair;
 };
 // the counters cleared before a counting launch: visits .. spills
 constexpr size_t kCounterBytes =
@@ -348,7 +351,7 @@ struct ChunkWork {
     Src src;
     Sink sink;
     uint32_t n, chunk;
-    __device__ __forceinline__ void prepare() {}
+    __device__ __forceinline__ uint32_t prepare(uint32_t n) { return n; }
     struct State {
         uint32_t ray, end;
     };
@@ -394,7 +397,7 @@ struct MultiWork {
     Sink sink;
     const BatchItem *items;
     uint32_t n_items, chunk;
-    __device__ __forceinline__ void prepare() {}
+    __device__ __forceinline__ uint32_t prepare(uint32_t n) { return n; }
     struct State {
         uint32_t ray, end;
         Tree t;
@@ -471,9 +474,9 @@ struct BlockWork {
     const RenderItem *items;    // the jobs when MULTI, by unit_begin
     uint32_t n_items;
     const uint32_t *queue;      // !MULTI: units are queue entries if set:
-    const uint32_t *qfront;     // the first *qfront units from the front,
-    uint32_t qlen;              // the rest from the back of qlen entries
-    uint32_t qf;                // *qfront, read once by prepare()
+    const unsigned long long *qpair;  // the first (low word) units from
+    uint32_t qlen;              // the front, the rest from the back of qlen
+    uint32_t qf;                // entries; qf: the low word, from prepare()
     uint32_t group;             // !MULTI: units dealt to the XCDs in turns
                                 // of `group` (the queue is ordered
                                 // longest-first, so every XCD should follow
@@ -484,11 +487,17 @@ struct BlockWork {
     }
     __device__ __forceinline__ uint32_t unit_shift() const { return 0; }
     uint32_t unit_shift_host() const { return 0; }
-    // read once per thread what the block pre-pass counted on the device
-    __device__ __forceinline__ void prepare() {
+    // reads once per thread what the block pre-pass counted on the device;
+    // returns the number of units
+    __device__ __forceinline__ uint32_t prepare(uint32_t n) {
         if constexpr (!MULTI) {
-            if (queue) qf = *qfront;
+            if (queue) {
+                const unsigned long long pr = *qpair;
+                qf = (uint32_t)pr;
+                return (uint32_t)pr + (uint32_t)(pr >> 32);
+            }
         }
+        return n;
     }
     enum { CORNER1 = 0, CORNER2 = 1, LOOP = 2 };
     struct State {
@@ -675,15 +684,20 @@ struct BestWork {
     static constexpr int kMinWavesPerSimd = 7;
     RenderItem one;
     const uint32_t *queue;  // block queue of the pre-pass, or null: the
-    const uint32_t *qfront; // first *qfront entries from the front, the
-    uint32_t qlen, qf;      // rest from the back of qlen entries
+    const unsigned long long *qpair;  // first (low word) entries from the
+    uint32_t qlen, qf;      // front, the rest from the back of qlen entries
     uint32_t split;         // a unit is 16 >> split pixels
     uint32_t group;         // units dealt to the XCDs in turns of `group`
     __device__ __forceinline__ uint32_t xcd_group() const { return group; }
     __device__ __forceinline__ uint32_t unit_shift() const { return split; }
     uint32_t unit_shift_host() const { return split; }
-    __device__ __forceinline__ void prepare() {
-        if (queue) qf = *qfront;
+    // reads once per thread what the block pre-pass counted on the device;
+    // returns the number of units
+    __device__ __forceinline__ uint32_t prepare(uint32_t n) {
+        if (!queue) return n;
+        const unsigned long long pr = *qpair;
+        qf = (uint32_t)pr;
+        return ((uint32_t)pr + (uint32_t)(pr >> 32)) << split;
     }
     struct State {
         uint32_t frame, base;  // camera, output index of the block
@@ -747,15 +761,13 @@ constexpr uint32_t kXcdStride = 64;  // counters 256 bytes apart
 template <int K, class Work, bool COUNT>
 __global__ void __launch_bounds__(256)
 __attribute__((amdgpu_waves_per_eu(COUNT ? 1 : Work::kMinWavesPerSimd, 8)))
-rti_kernel(Tree t, Work work_arg, uint32_t n, const uint32_t *n_dev,
-           Ctrl *ctrl, uint32_t *xnext, uint4 *__restrict__ spill,
+rti_kernel(Tree t, Work work_arg, uint32_t n, Ctrl *ctrl, uint32_t *xnext, uint4 *__restrict__ spill,
            uint32_t spill_slots, uint32_t refill, uint32_t fin_batch,
            uint32_t leaf_batch) {
     extern __shared__ uint4 ring[];  // [K][blockDim.x]
     Work work = work_arg;
-    work.prepare();
-    // unit count produced on the device (queue entries, split or not)
-    if (n_dev) n = *n_dev << work.unit_shift();
+    // the unit count may come from the device (the render's block queue)
+    n = work.prepare(n);
     // one contiguous range per XCD, or groups of xcd_group() units dealt
     // to the XCDs in turn when the order of the units matters
     const uint32_t xg = work.xcd_group();
@@ -1057,33 +1069,37 @@ rti_kernel(Tree t, Work work_arg, uint32_t n, const uint32_t *n_dev,
         if (done == total - 1) {
             for (uint32_t x = 0; x < kXcds; x++)
                 atomicExch(&xnext[x * kXcdStride], 0u);
-            atomicExch(&ctrl->qcount, 0u);
-            atomicExch(&ctrl->qback, 0u);
-            atomicExch(&ctrl->qtotal, 0u);
+            atomicExch(&ctrl->qpair, 0ull);
             atomicExch(&ctrl->left, 0u);
         }
     }
 }
 
 // Pre-pass of a render launch: one thread per pixel tests its camera ray
-// against the root box.  A block none of whose 16 rays enters the box is
-// finished here -- every pixel is a miss in every quality mode -- and the
+// against the root box, with the conservative may_enter (cull.h: a
+// reciprocal instead of three divisions; false only for rays that the
+// traversal's exact hit_box rejects too).  A block none of whose 16 rays
+// can enter the box is finished here -- every pixel is a miss in every
+// quality mode -- and the
 // others go, in block order, to the queue the traversal kernel takes its
-// blocks from.  A workgroup covers kCullTile pixels (256 blocks): it marks
+// blocks from.  A workgroup covers kCullTile pixels (1024 blocks): it marks
 // the entering blocks in LDS, compacts them with a workgroup scan and
-// reserves its queue range with a single atomic.  In counting mode it adds
-// what the loop would have counted for each finished block: one root
-// search per traced pixel (ADAPTIVE with ray merging traces 8 of the 16
-// pixels of a block whose first corner misses), one node visit per search,
-// and the block's final mode.
-constexpr uint32_t kCullTile = 256 * 16;
+// reserves its queue ranges with a single 64-bit atomic (the queue
+// counters share one address, so their atomics are serialised: one per
+// workgroup of 1024 blocks keeps them off the critical path).  In counting
+// mode it adds what the loop would have counted for each finished block:
+// one root search per traced pixel (ADAPTIVE with ray merging traces 8 of
+// the 16 pixels of a block whose first corner misses), one node visit per
+// search, and the block's final mode.
+constexpr uint32_t kCullBlocks = 1024;
+constexpr uint32_t kCullTile = kCullBlocks * 16;
 
 template <bool COUNT>
 __global__ void __launch_bounds__(256)
 block_cull_kernel(RenderItem it, uint32_t npix, Ctrl *ctrl,
                   uint32_t *__restrict__ queue, uint32_t qlen) {
-    __shared__ uint8_t enter[kCullTile / 16];
-    __shared__ uint32_t wave_sum[2][4];
+    __shared__ uint8_t enter[kCullBlocks];
+    __shared__ uint32_t wave_sum[4];
     __shared__ uint32_t gbase[2];
     __shared__ unsigned long long missed;
     const uint32_t tid = threadIdx.x, lane = lane_id(), w = tid >> 6;
@@ -1093,8 +1109,18 @@ block_cull_kernel(RenderItem it, uint32_t npix, Ctrl *ctrl,
     // the host launches this kernel only for a tree with a root record
     const float4 *q = reinterpret_cast<const float4 *>(it.t.nodes);
     const float4 a = q[0], b = q[1];
-    const f3 lo = f3{a.x, a.y, a.z}, hi = f3{a.w, b.x, b.y};
-    for (uint32_t k = 0; k < 16; ++k) {
+    const float lo3[3] = {a.x, a.y, a.z}, hi3[3] = {a.w, b.x, b.y};
+    // the camera of the tile's first pixel (tile0 < npix, so f0 is a frame
+    // of the job); a tile that reaches into the next frame recomputes
+    const uint32_t f0 = it.bpf.div(tile0 >> 4);
+    CullOrigin co0;
+    {
+        const CamDev &c = it.cams[f0];
+        const float o3[3] = {c.px, c.py, c.pz};
+        co0 = cull_origin(lo3, hi3, o3);
+    }
+    // pixel r = tile0 + k * 256 + tid: a wave covers 4 whole blocks
+    for (uint32_t k = 0; k < kCullTile / 256; ++k) {
         const uint32_t r = tile0 + k * 256 + tid;
         const bool valid = r < npix;
         const uint32_t blk = r >> 4, p = r & 15u;
@@ -1107,8 +1133,16 @@ block_cull_kernel(RenderItem it, uint32_t npix, Ctrl *ctrl,
             const int x = (int)(cs - row * it.ws.d) * 4 + (int)(p & 3u);
             const int y = (int)row * 4 + (int)(p >> 2);
             const f3 d = screen2world(c, x, y);
-            f3 e;
-            enters = hit_box(lo, hi, f3{c.px, c.py, c.pz}, d, e);
+            // conservative: false only for rays that hit_box rejects too
+            CullOrigin co = co0;
+            if (f != f0) {
+                const float o3[3] = {c.px, c.py, c.pz};
+                co = cull_origin(lo3, hi3, o3);
+            }
+            const float d3[3] = {d.x, d.y, d.z};
+            enters = may_enter_dir(co, d3, [](float v) {
+                return __builtin_amdgcn_rcpf(v);
+            });
         }
         const uint64_t m = __ballot(enters);
         const uint64_t bm = (m >> (16 * (lane >> 4))) & 0xffffull;
@@ -1121,33 +1155,46 @@ block_cull_kernel(RenderItem it, uint32_t npix, Ctrl *ctrl,
         if (COUNT && valid && p == 0 && !block_enters) my_missed++;
     }
     __syncthreads();
-    // ordered compaction of the 256 block flags (thread t owns block t):
-    // blocks whose rays all enter go to the front of the queue, the other
-    // entering blocks to the back, so the longest work is handed out first
-    // and the end of the launch is made of the cheaper blocks
-    const uint32_t cls = enter[tid];
-    const uint64_t fm = __ballot(cls == 2), pm = __ballot(cls == 1);
-    if (lane == 0) {
-        wave_sum[0][w] = (uint32_t)__popcll(fm);
-        wave_sum[1][w] = (uint32_t)__popcll(pm);
+    // ordered compaction of the 1024 block flags (thread t owns blocks
+    // 4t..4t+3): blocks whose rays all enter go to the front of the queue,
+    // the other entering blocks to the back, so the longest work is handed
+    // out first and the end of the launch is made of the cheaper blocks.
+    // Both counts are scanned at once, packed in the two halves of a word.
+    uint32_t cls[4], mine = 0;
+    for (uint32_t j = 0; j < 4; ++j) {
+        cls[j] = enter[4 * tid + j];
+        mine += cls[j] == 2 ? 1u : (cls[j] == 1 ? 0x10000u : 0u);
     }
+    uint32_t incl = mine;
+    for (uint32_t o = 1; o < 64; o <<= 1) {
+        const uint32_t v = __shfl_up(incl, o);
+        if (lane >= o) incl += v;
+    }
+    if (lane == 63) wave_sum[w] = incl;
     if (COUNT && my_missed) atomicAdd(&missed, (unsigned long long)my_missed);
     __syncthreads();
-    uint32_t off[2] = {0, 0}, total[2] = {0, 0};
-    for (uint32_t i = 0; i < 4; ++i)
-        for (int c = 0; c < 2; ++c) {
-            off[c] += (i < w) ? wave_sum[c][i] : 0u;
-            total[c] += wave_sum[c][i];
-        }
+    uint32_t before = 0, total = 0;
+    for (uint32_t i = 0; i < 4; ++i) {
+        before += (i < w) ? wave_sum[i] : 0u;
+        total += wave_sum[i];
+    }
     if (tid == 0) {
-        gbase[0] = total[0] ? atomicAdd(&ctrl->qcount, total[0]) : 0u;
-        gbase[1] = total[1] ? atomicAdd(&ctrl->qback, total[1]) : 0u;
-        if (total[0] + total[1]) atomicAdd(&ctrl->qtotal, total[0] + total[1]);
+        unsigned long long base = 0;
+        if (total)
+            base = atomicAdd(&ctrl->qpair,
+                             (unsigned long long)(total & 0xffffu) |
+                                 ((unsigned long long)(total >> 16) << 32));
+        gbase[0] = (uint32_t)base;
+        gbase[1] = (uint32_t)(base >> 32);
     }
     __syncthreads();
-    if (cls == 2) queue[gbase[0] + off[0] + lanes_below(fm)] = tile0 / 16 + tid;
-    if (cls == 1)
-        queue[qlen - 1u - (gbase[1] + off[1] + lanes_below(pm))] = tile0 / 16 + tid;
+    uint32_t off = before + incl - mine;  // exclusive, both halves
+    uint32_t of = gbase[0] + (off & 0xffffu), ob = gbase[1] + (off >> 16);
+    for (uint32_t j = 0; j < 4; ++j) {
+        const uint32_t id = tile0 / 16 + 4 * tid + j;
+        if (cls[j] == 2) queue[of++] = id;
+        if (cls[j] == 1) queue[qlen - 1u - ob++] = id;
+    }
     if (COUNT && tid == 0 && missed) {
         const unsigned long long nb = missed;
         const bool halved = it.rnd_mode == VOXG_QUALITY_ADAPTIVE &&
@@ -1434,7 +1481,7 @@ dev::Tree dev_tree(const voxg_tree *t) {
 template <int K, class Work, bool COUNT>
 int launch_k(const dev::Tree &dt, uint32_t stack_bound, const Work &work,
              size_t n, hipStream_t stream, const voxg_launch_config &cfg,
-             voxg_counts *counts, const uint32_t *n_dev, bool reset_counts) {
+             voxg_counts *counts, bool reset_counts) {
     int devid = 0;
     HIPCHK(hipGetDevice(&devid));
     int cus = cu_count(devid);
@@ -1466,7 +1513,7 @@ int launch_k(const dev::Tree &dt, uint32_t stack_bound, const Work &work,
                               dev::kCounterBytes, stream));
     }
     hipLaunchKernelGGL(kern, dim3(grid), dim3(block), lds, stream, dt, work,
-                       (uint32_t)n, n_dev, ws->ctrl, ws->xnext, ws->spill,
+                       (uint32_t)n, ws->ctrl, ws->xnext, ws->spill,
                        slots,
                        (uint32_t)cfg.refill, (uint32_t)cfg.finish_batch,
                        (uint32_t)cfg.leaf_batch);
@@ -1493,15 +1540,15 @@ int launch_k(const dev::Tree &dt, uint32_t stack_bound, const Work &work,
 template <class Work, bool COUNT>
 int launch_tree(int kind, const dev::Tree &t, uint32_t stack_bound,
                 const Work &work, size_t n, hipStream_t stream,
-                voxg_counts *counts, const uint32_t *n_dev = nullptr,
+                voxg_counts *counts,
                 bool reset_counts = true) {
     voxg_launch_config cfg = current_config(kind);
     switch (cfg.stack_lds) {
-    case 2: return launch_k<2, Work, COUNT>(t, stack_bound, work, n, stream, cfg, counts, n_dev, reset_counts);
-    case 4: return launch_k<4, Work, COUNT>(t, stack_bound, work, n, stream, cfg, counts, n_dev, reset_counts);
-    case 8: return launch_k<8, Work, COUNT>(t, stack_bound, work, n, stream, cfg, counts, n_dev, reset_counts);
-    case 16: return launch_k<16, Work, COUNT>(t, stack_bound, work, n, stream, cfg, counts, n_dev, reset_counts);
-    case 32: return launch_k<32, Work, COUNT>(t, stack_bound, work, n, stream, cfg, counts, n_dev, reset_counts);
+    case 2: return launch_k<2, Work, COUNT>(t, stack_bound, work, n, stream, cfg, counts, reset_counts);
+    case 4: return launch_k<4, Work, COUNT>(t, stack_bound, work, n, stream, cfg, counts, reset_counts);
+    case 8: return launch_k<8, Work, COUNT>(t, stack_bound, work, n, stream, cfg, counts, reset_counts);
+    case 16: return launch_k<16, Work, COUNT>(t, stack_bound, work, n, stream, cfg, counts, reset_counts);
+    case 32: return launch_k<32, Work, COUNT>(t, stack_bound, work, n, stream, cfg, counts, reset_counts);
     default: return fail(VOXG_EINVAL, "stack_lds must be 2, 4, 8, 16 or 32");
     }
 }
@@ -1989,11 +2036,10 @@ static int render_run(const voxg_tree *t, Work &work, int kind,
                        (uint32_t)nblk);
     HIPCHK(hipGetLastError());
     work.queue = ws->queue;
-    work.qfront = &ws->ctrl->qcount;
+    work.qpair = &ws->ctrl->qpair;
     work.qlen = (uint32_t)nblk;
     return launch_tree<Work, COUNT>(kind, dev_tree(t), t->stack_bound, work,
-                                    units, stream, counts, &ws->ctrl->qtotal,
-                                    false);
+                                    units, stream, counts, false);
 }
 
 template <bool COUNT>
