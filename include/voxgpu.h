@@ -225,7 +225,8 @@ typedef struct {
 
 /* The vox_render per-pixel loop for n_frames camera poses at W x H
  * in any quality mode (BEST, FAST, ADAPTIVE with or without ray merging).
- * (W % 16 == 0, H % 4 == 0, renderer.c:97-100).  cams: device array of
+ * (W % 16 == 0, H % 4 == 0, renderer.c:97-100; W, H < 65536 and
+ * n_frames * W * H < 2^31, else VOXG_ERANGE).  cams: device array of
  * n_frames descriptors.  Pixel (x, y) of frame f lives at index
  * f*W*H + cs*16 + p with block cs = (y/4)*(W/4) + x/4 and p = (y%4)*4 + x%4
  * (the reference's square layout, renderer.h:14).  Outputs (each may be

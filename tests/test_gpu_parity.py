@@ -468,3 +468,17 @@ def test_block_pre_pass_never_drops_a_hit(gpu, oracle):
     for i in (0, 3, 7, 12, 17, 22):
         _, vox_o, _, _ = T.render(cams[i], W, H, oracle.QUALITY_BEST)
         assert np.array_equal(vg[i], vox_o)
+
+
+def test_render_rejects_oversized_frames(gpu):
+    """Block corners are 16-bit: a side of 65536 pixels or more is refused
+    before any launch, as is a frame count that makes 2^31 pixels."""
+    from voxvision_amd import _lib, scenes, voxrnd, voxtrees as VT
+    c1 = _c1()
+    G = VT.make_tree(c1["dots"])
+    with pytest.raises(_lib.VoxgError, match="65536"):
+        voxrnd.render(G, [c1["camera"]], 65536, 4, want_colors=False,
+                      want_hits=False)
+    with pytest.raises(_lib.VoxgError, match="65536"):
+        voxrnd.render(G, [c1["camera"]], 16, 65536, want_colors=False,
+                      want_hits=False)

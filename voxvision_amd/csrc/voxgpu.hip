@@ -467,8 +467,9 @@ struct RenderItem {
 // launch.
 template <bool MULTI>
 struct BlockWork {
-    // 5 waves per SIMD as compiled; a floor of 6 spills and is slower
-    static constexpr int kMinWavesPerSimd = 1;
+    // 6 waves per SIMD (at most 80 VGPRs) fit without spilling for one
+    // render; the multi-job form would spill at that floor
+    static constexpr int kMinWavesPerSimd = MULTI ? 1 : 6;
     // one render's block queue is ordered longest-first (block_cull_kernel)
     RenderItem one;             // the job when !MULTI
     const RenderItem *items;    // the jobs when MULTI, by unit_begin
@@ -502,10 +503,13 @@ struct BlockWork {
     enum { CORNER1 = 0, CORNER2 = 1, LOOP = 2 };
     struct State {
         uint32_t item, frame, base;  // job, frame in the job, output index
-        int i, iend, stage, mode, merge, leaf_try, p;
+        // packed in one word: the pixel position i in the block's order and
+        // its end, the stage, the block's mode, ray merging, a leaf try
+        uint32_t i : 5, iend : 5, stage : 2, mode : 2, merge : 1,
+            leaf_try : 1;
         uint32_t leaf, vid_last;
-        f3 inter_last, c1_inter, c1_dir;
-        int xs, ys;
+        f3 inter_last;  // at the second corner: the first corner's hit
+        uint32_t xs : 16, ys : 16;  // block corner (W, H < 65536)
         uint32_t fast, best, merged;
     };
 
@@ -541,7 +545,6 @@ struct BlockWork {
                 st.i++;
                 continue;
             }
-            st.p = p;
             st.leaf_try = (st.mode == VOXG_QUALITY_FAST && st.leaf != kEnd);
             trace_pixel(st, p, st.leaf_try ? st.leaf : 0u, cur, org, d);
             return true;
@@ -588,9 +591,7 @@ struct BlockWork {
             st.i = 1;
             st.stage = CORNER1;
             st.leaf_try = 0;
-            st.p = 0;
             trace_pixel(st, 0, 0u, cur, org, d);
-            st.c1_dir = d;
             return;
         }
         st.stage = LOOP;
@@ -607,10 +608,8 @@ struct BlockWork {
             if (hit) {
                 st.vid_last = vid;
                 st.inter_last = e;
-                st.c1_inter = e;
                 st.iend = 15;
                 st.stage = CORNER2;
-                st.p = 15;
                 trace_pixel(st, 15, 0u, cur, org, d);
                 return true;
             }
@@ -621,13 +620,15 @@ struct BlockWork {
             // renderer.c:260-279
             emit(st, 15, hit, vid, e);
             if (hit) {
+                // the first corner's hit point and direction: inter_last
+                // has not changed since, and the direction is recomputed
                 const CamDev &c = cam(st);
                 f3 origin = f3{c.px, c.py, c.pz};
-                float d1 = sqr_metric(st.c1_inter, origin);
-                f3 dir1 = st.c1_dir;
+                float d1 = sqr_metric(st.inter_last, origin);
+                f3 dir1 = screen2world(c, st.xs, st.ys);
                 float d2 = (dir1.x * dir1.x + dir1.y * dir1.y) + dir1.z * dir1.z;
                 float criteria = d1 / d2 * sqr_metric(dir1, d);
-                float dist = sqr_metric(st.c1_inter, e);
+                float dist = sqr_metric(st.inter_last, e);
                 if ((double)(dist / criteria) > 1.69) st.mode = VOXG_QUALITY_BEST;
                 if (d1 > 22500.0f) {
                     st.merge = (st.mode == VOXG_QUALITY_FAST &&
@@ -655,7 +656,7 @@ struct BlockWork {
             st.vid_last = vid;
             st.inter_last = e;
         }
-        emit(st, st.p, hit, vid, e);
+        emit(st, render_order(st.i), hit, vid, e);
         st.i++;
         return next(st, cur, org, d);
     }
@@ -1346,7 +1347,7 @@ std::mutex g_mu;
 // 256^3 benchmark scene; every field is validated by voxg_set_*)
 voxg_launch_config g_cfg[VOXG_LAUNCH_KINDS] = {
     {4, 256, 28, 8, 1, 1, 24, 16, 0},   // VOXG_LAUNCH_RAYS
-    {4, 256, 20, 8, 4, 1, 16, 16, 0},   // VOXG_LAUNCH_RENDER
+    {4, 256, 24, 8, 4, 1, 16, 16, 0},   // VOXG_LAUNCH_RENDER
     {4, 256, 28, 48, 4, 1, 32, 64, 4},  // VOXG_LAUNCH_RENDER_BEST
     {4, 256, 24, 8, 1, 1, 16, 16, 0},   // VOXG_LAUNCH_BATCH
 };
@@ -1977,6 +1978,9 @@ static int render_common(const voxg_tree *t, size_t n_frames,
         return fail(VOXG_EINVAL, "reserved quality bits set");
     if (squares && (!params || !params->texture))
         return fail(VOXG_EINVAL, "colour output needs a texture");
+    // block corners are kept in 16-bit fields
+    if (W >= 65536 || H >= 65536)
+        return fail(VOXG_ERANGE, "W and H must be < 65536");
     size_t per = (size_t)W * (size_t)H;
     if (n_frames * per >= 0x80000000ull)
         return fail(VOXG_ERANGE, "n_frames * W * H must be < 2^31");
@@ -2685,7 +2689,8 @@ int voxg_server_submit_render(voxg_server *s, const voxg_render_request *rq,
         return fail(VOXG_EINVAL, "reserved quality bits set");
     if (rq->color_out && !s->d_texture)
         return fail(VOXG_EINVAL, "colour output needs a server texture");
-    if ((size_t)rq->W * rq->H >= 0x40000000ull)
+    if ((size_t)rq->W * rq->H >= 0x40000000ull || rq->W >= 65536 ||
+        rq->H >= 65536)
         return fail(VOXG_ERANGE, "frame too large");
     SrvRequest *r = new (std::nothrow) SrvRequest();
     if (!r) return fail(VOXG_ENOMEM, "host allocation failed");
