@@ -324,16 +324,17 @@ def test_render_fast_adaptive_merge_match_oracle(gpu, oracle, quality):
     assert cnt["voxel_tests"] == st["voxel_tests"]
 
 
-def test_render_adaptive_ct_frames(gpu, oracle):
-    """ADAPTIVE with accurate ray merging on two 1080p poses of the 256^3
-    scene (the reference's default quality is ADAPTIVE, renderer.c:78)."""
+@pytest.mark.parametrize("q", [2, 2 | 8, 1])
+def test_render_adaptive_ct_frames(gpu, oracle, q):
+    """ADAPTIVE (the reference's default quality, renderer.c:78), ADAPTIVE
+    with accurate ray merging and FAST on two 1080p poses of the 256^3
+    scene: many blocks of both modes, long leaf-reuse chains."""
     from voxvision_amd import scenes, voxrnd, voxtrees as VT
     c2 = scenes.config_c2(frames=64)
     T = oracle.Tree(c2["dots"])
     G = VT.make_tree(c2["dots"])
     W, H = c2["W"], c2["H"]
     cams = [c2["cameras"][k] for k in (5, 50)]
-    q = 2 | 8
     out = voxrnd.render(G, cams, W, H, quality=q, want_colors=False)
     torch.cuda.synchronize()
     vg = _np_u32(out["voxel"]).reshape(2, -1)

@@ -312,7 +312,7 @@ struct PixSink {
 };
 
 struct Ctrl {
-    uint32_t spare;    // (units are handed out by the per-XCD counters)
+    uint32_t rdone;    // workgroups of resolve_kernel that have finished
     uint32_t left;     // waves that have finished
     uint32_t overflow; // set when a stack bound was exceeded (never expected)
     uint32_t pad;
@@ -345,6 +345,7 @@ struct ChunkWork {
     // no occupancy floor asked of the register allocator (it fits 7 waves
     // per SIMD as it is; 8 would spill)
     static constexpr int kMinWavesPerSimd = 1;
+    __device__ __forceinline__ bool keeps_queue() const { return false; }
     // XCD hand-out: one contiguous range of the units per XCD
     __device__ __forceinline__ uint32_t xcd_group() const { return 0; }
     __device__ __forceinline__ uint32_t unit_shift() const { return 0; }
@@ -391,6 +392,7 @@ template <class Src, class Sink>
 struct MultiWork {
     // 6 waves per SIMD (at most 80 VGPRs) fit without spilling
     static constexpr int kMinWavesPerSimd = 6;
+    __device__ __forceinline__ bool keeps_queue() const { return false; }
     __device__ __forceinline__ uint32_t xcd_group() const { return 0; }
     __device__ __forceinline__ uint32_t unit_shift() const { return 0; }
     Src src;
@@ -448,6 +450,48 @@ __device__ __forceinline__ float sqr_metric(f3 a, f3 b) {
     return (x + y) + z;
 }
 
+// search.c:31-81: the voxel of a leaf that the ray, entering the leaf's
+// box at e, hits first: a dense leaf's voxel at e, or the voxel whose entry
+// point is L1-closest to e (strictly, in stored order; an entry at e ends
+// the search).  Writes the voxel id and the hit point; false: no voxel.
+template <bool COUNT>
+__device__ __forceinline__ bool leaf_search(const Tree &T, const Node &nd,
+                                            f3 e, f3 d, uint32_t &vid,
+                                            f3 &bres,
+                                            unsigned long long &nt) {
+    bres = e;
+    if ((nd.info & NT_TYPE_MASK) == NT_DENSE) {
+        vid = dense_voxel(nd, e, T);
+        return true;
+    }
+    const f3 V = f3{T.vx, T.vy, T.vz};
+    const uint32_t cnt = nd.info >> 8;
+    float best = INFINITY;
+    bool hit = false;
+    for (uint32_t k = 0; k < cnt; k++) {
+        const float *p = T.dots + 3u * (nd.ref + k);
+        f3 vmin = f3{p[0], p[1], p[2]};
+        f3 vmax = f3{vmin.x + V.x, vmin.y + V.y, vmin.z + V.z};
+        f3 far;
+        if (COUNT) nt++;
+        if (hit_box(vmin, vmax, e, d, far)) {
+            float m = abs_metric(e, far);
+            if (m == 0.0f) {
+                bres = far;
+                vid = nd.ref + k;
+                return true;
+            }
+            if (m < best) {
+                best = m;
+                bres = far;
+                vid = nd.ref + k;
+                hit = true;
+            }
+        }
+    }
+    return hit;
+}
+
 // A render job: one tree, a run of camera poses of one size and quality,
 // its output buffers.  Units are 4x4 blocks, `bpf` per frame.
 struct RenderItem {
@@ -470,6 +514,7 @@ struct BlockWork {
     // 6 waves per SIMD (at most 80 VGPRs) fit without spilling for one
     // render; the multi-job form would spill at that floor
     static constexpr int kMinWavesPerSimd = MULTI ? 1 : 6;
+    __device__ __forceinline__ bool keeps_queue() const { return false; }
     // one render's block queue is ordered longest-first (block_cull_kernel)
     RenderItem one;             // the job when !MULTI
     const RenderItem *items;    // the jobs when MULTI, by unit_begin
@@ -672,6 +717,28 @@ struct BlockWork {
     }
 };
 
+// Root-search results of the pixels of the queued blocks: the first pass
+// of FAST and ADAPTIVE renders writes them, resolve_kernel reads them.
+// Stored pixel-major, index p * stride + block (stride: the blocks of the
+// render), so that the resolve threads, one per block, read consecutive
+// words when they all take their blocks' pixel p.
+struct SpecOut {
+    uint32_t *vid, *node;  // voxel id or kEnd; leaf node of a hit
+    float *hx, *hy, *hz;   // hit point
+    uint32_t stride;
+    __device__ __forceinline__ void put(uint32_t blk, int px, bool hit,
+                                        uint32_t v, uint32_t nd, f3 p) const {
+        const uint32_t r = (uint32_t)px * stride + blk;
+        vid[r] = hit ? v : kEnd;
+        if (hit) {
+            node[r] = nd;
+            hx[r] = p.x;
+            hy[r] = p.y;
+            hz[r] = p.z;
+        }
+    }
+};
+
 // BEST-mode render of one job: every pixel is a root search
 // (renderer.c:300-308), so a unit is 16 >> split pixels of a 4x4 block,
 // down to one pixel, taken in the block's order (renderer.c:23-25).  With
@@ -684,6 +751,12 @@ struct BestWork {
     // without spilling, and the extra wave hides node-load latency
     static constexpr int kMinWavesPerSimd = 7;
     RenderItem one;
+    SpecOut spec;            // set: root searches of all queued pixels for
+                             // resolve_kernel instead of the job's outputs
+    uint32_t keep_queue;     // the queue counts are still needed afterwards
+    __device__ __forceinline__ bool keeps_queue() const {
+        return keep_queue != 0;
+    }
     const uint32_t *queue;  // block queue of the pre-pass, or null: the
     const unsigned long long *qpair;  // first (low word) entries from the
     uint32_t qlen, qf;      // front, the rest from the back of qlen entries
@@ -733,10 +806,11 @@ struct BestWork {
         trace(st, cur, org, d);
     }
     __device__ __forceinline__ bool finish(State &st, bool hit, uint32_t vid,
-                                           uint32_t, f3 e, uint32_t &cur,
+                                           uint32_t node, f3 e, uint32_t &cur,
                                            f3 &org, f3 &d) const {
-        one.sink.put(st.base + (uint32_t)render_order((int)(st.pos & 0xffu)),
-                     hit, vid, 0, e);
+        const int p = render_order((int)(st.pos & 0xffu));
+        if (spec.vid) spec.put(st.base >> 4, p, hit, vid, node, e);
+        else one.sink.put(st.base + (uint32_t)p, hit, vid, 0, e);
         st.pos++;
         if ((st.pos & 0xffu) < ((st.pos >> 8) & 0xffu)) {
             trace(st, cur, org, d);
@@ -965,40 +1039,9 @@ rti_kernel(Tree t, Work work_arg, uint32_t n, Ctrl *ctrl, uint32_t *xnext, uint4
         // or no lane is traversing
         if (ph == PH_LEAF && ((uint32_t)__popcll(lm) >= leaf_batch || tm == 0)) {
             const Node nd = load_node(T.nodes, cur);
-            const f3 e = org;
-            bool hit = false;
             uint32_t vid = kEnd;
-            f3 bres = e;
-            if ((nd.info & NT_TYPE_MASK) == NT_DENSE) {
-                hit = true;
-                vid = dense_voxel(nd, e, T);
-            } else {
-                const f3 V = f3{T.vx, T.vy, T.vz};
-                const uint32_t cnt = nd.info >> 8;
-                float best = INFINITY;
-                for (uint32_t k = 0; k < cnt; k++) {
-                    const float *p = T.dots + 3u * (nd.ref + k);
-                    f3 vmin = f3{p[0], p[1], p[2]};
-                    f3 vmax = f3{vmin.x + V.x, vmin.y + V.y, vmin.z + V.z};
-                    f3 far;
-                    if (COUNT) nt++;
-                    if (hit_box(vmin, vmax, e, d, far)) {
-                        float m = abs_metric(e, far);
-                        if (m == 0.0f) {
-                            bres = far;
-                            vid = nd.ref + k;
-                            hit = true;
-                            break;
-                        }
-                        if (m < best) {
-                            best = m;
-                            bres = far;
-                            vid = nd.ref + k;
-                            hit = true;
-                        }
-                    }
-                }
-            }
+            f3 bres = org;
+            const bool hit = leaf_search<COUNT>(T, nd, org, d, vid, bres, nt);
             if (hit) {
                 ph = PH_DONE;
                 p_hit = true;
@@ -1070,7 +1113,7 @@ rti_kernel(Tree t, Work work_arg, uint32_t n, Ctrl *ctrl, uint32_t *xnext, uint4
         if (done == total - 1) {
             for (uint32_t x = 0; x < kXcds; x++)
                 atomicExch(&xnext[x * kXcdStride], 0u);
-            atomicExch(&ctrl->qpair, 0ull);
+            if (!work.keeps_queue()) atomicExch(&ctrl->qpair, 0ull);
             atomicExch(&ctrl->left, 0u);
         }
     }
@@ -1206,6 +1249,127 @@ block_cull_kernel(RenderItem it, uint32_t npix, Ctrl *ctrl,
         if (it.rnd_mode == VOXG_QUALITY_BEST) atomicAdd(&ctrl->best_blocks, nb);
         else atomicAdd(&ctrl->fast_blocks, nb);
         if (halved) atomicAdd(&ctrl->merge_blocks, nb);
+    }
+}
+
+// Second pass of a FAST or ADAPTIVE render without ray merging
+// (renderer.c:240-316).  The first pass traced every pixel of the queued
+// blocks as a root search, coherently (BestWork into SpecOut); here one
+// thread per queued block replays the block's pixels in the reference's
+// order -- ADAPTIVE's two corners and its mode decision, then each pixel:
+// in FAST mode a try of the previous pixel's leaf first, else (or when the
+// try misses) the root search -- taking every root search from the first
+// pass.  A root search depends on its ray alone, so each pixel gets the
+// result the reference computes; root searches that the reference skips
+// are traced in vain.  A leaf try is the traversal from a leaf node: the
+// leaf's box test, then its voxels.  Results stay in SpecOut (a leaf try
+// that hits replaces the root search there); emit_kernel writes them out.
+__global__ void __launch_bounds__(256)
+resolve_kernel(RenderItem it, SpecOut spec, const uint32_t *__restrict__ queue,
+               uint32_t qlen, const Ctrl *ctrl) {
+    const unsigned long long pr = ctrl->qpair;
+    const uint32_t qf = (uint32_t)pr, qt = qf + (uint32_t)(pr >> 32);
+    const Tree &T = it.t;
+    unsigned long long nt = 0;
+    for (uint32_t b = blockIdx.x * blockDim.x + threadIdx.x; b < qt;
+         b += gridDim.x * blockDim.x) {
+        const uint32_t id = queue[b < qf ? b : qlen - 1u - (b - qf)];
+        const uint32_t f = it.bpf.div(id);
+        const uint32_t cs = id - f * it.bpf.d;
+        const uint32_t row = it.ws.div(cs);
+        const int xs = (int)(cs - row * it.ws.d) * 4, ys = (int)row * 4;
+        const CamDev &c = it.cams[f];
+        const f3 o = f3{c.px, c.py, c.pz};
+        // the first pass's root search of pixel p (all five words loaded
+        // together: the node and point are only meaningful on a hit)
+        auto root = [&](int p, uint32_t &nd, f3 &pt) {
+            const uint32_t r = (uint32_t)p * spec.stride + id;
+            const uint32_t v = spec.vid[r];
+            nd = spec.node[r];
+            pt = f3{spec.hx[r], spec.hy[r], spec.hz[r]};
+            return v != kEnd;
+        };
+        int mode = it.rnd_mode;
+        uint32_t leaf = kEnd;
+        int i = 0, iend = 16;
+        if (mode == VOXG_QUALITY_ADAPTIVE) {
+            // renderer.c:240-281: corners 0 and 15, then FAST or BEST
+            mode = VOXG_QUALITY_FAST;
+            uint32_t n0 = kEnd, n15 = kEnd;
+            f3 p0 = o, p15 = o;
+            const bool h0 = root(0, n0, p0);
+            leaf = h0 ? n0 : kEnd;
+            i = 1;
+            if (h0) {
+                iend = 15;
+                if (root(15, n15, p15)) {
+                    const f3 dir1 = screen2world(c, xs, ys);
+                    const f3 dir15 = screen2world(c, xs + 3, ys + 3);
+                    const float d1 = sqr_metric(p0, o);
+                    const float d2 =
+                        (dir1.x * dir1.x + dir1.y * dir1.y) + dir1.z * dir1.z;
+                    const float criteria = d1 / d2 * sqr_metric(dir1, dir15);
+                    const float dist = sqr_metric(p0, p15);
+                    if ((double)(dist / criteria) > 1.69)
+                        mode = VOXG_QUALITY_BEST;
+                }
+            }
+        }
+        if (mode != VOXG_QUALITY_FAST) continue;  // BEST: the root searches
+        for (; i < iend; ++i) {
+            const int p = render_order(i);
+            if (leaf != kEnd) {
+                // renderer.c:294-300: the previous pixel's leaf first
+                const f3 d = screen2world(c, xs + (p & 3), ys + (p >> 2));
+                const Node ln = load_node(T.nodes, leaf);
+                f3 e, pt;
+                uint32_t vid = kEnd;
+                if (hit_box(ln.bmin, ln.bmax, o, d, e) &&
+                    leaf_search<false>(T, ln, e, d, vid, pt, nt)) {
+                    const uint32_t r = (uint32_t)p * spec.stride + id;
+                    spec.vid[r] = vid;
+                    spec.hx[r] = pt.x;
+                    spec.hy[r] = pt.y;
+                    spec.hz[r] = pt.z;
+                    continue;  // the leaf stays the same
+                }
+            }
+            uint32_t nd = kEnd;
+            f3 pt;
+            leaf = root(p, nd, pt) ? nd : kEnd;
+        }
+    }
+}
+
+// Third pass: the final results of the queued blocks from SpecOut to the
+// render's outputs (square layout), 16 threads per block so that the
+// stores of a block are one contiguous run (one thread per block, with
+// 16 stores each, left lines partly written in L2 and was 3.6x slower).
+// The last workgroup clears the queue counts for the next render (every
+// workgroup read them when it started).
+__global__ void __launch_bounds__(256)
+emit_kernel(RenderItem it, SpecOut spec, const uint32_t *__restrict__ queue,
+            uint32_t qlen, Ctrl *ctrl) {
+    const unsigned long long pr = ctrl->qpair;
+    const uint32_t qf = (uint32_t)pr, qt = qf + (uint32_t)(pr >> 32);
+    for (uint32_t t = blockIdx.x * blockDim.x + threadIdx.x; t < qt * 16u;
+         t += gridDim.x * blockDim.x) {
+        const uint32_t b = t >> 4, p = t & 15u;
+        const uint32_t id = queue[b < qf ? b : qlen - 1u - (b - qf)];
+        const uint32_t r = p * spec.stride + id;
+        const uint32_t vid = spec.vid[r];
+        const bool hit = vid != kEnd;
+        f3 pt = f3{0, 0, 0};
+        if (hit) pt = f3{spec.hx[r], spec.hy[r], spec.hz[r]};
+        it.sink.put(id * 16u + p, hit, vid, 0, pt);
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        __threadfence();
+        if (atomicAdd(&ctrl->rdone, 1u) == gridDim.x - 1u) {
+            atomicExch(&ctrl->qpair, 0ull);
+            atomicExch(&ctrl->rdone, 0u);
+        }
     }
 }
 
@@ -2046,6 +2210,55 @@ static int render_run(const voxg_tree *t, Work &work, int kind,
                                     units, stream, counts, false);
 }
 
+// FAST and ADAPTIVE renders without ray merging, after the pre-pass: every
+// pixel of the blocks that the pre-pass queues is root-searched as in a
+// BEST render (coherent groups, BEST-render launch configuration) into a
+// per-stream scratch buffer of 20 bytes per pixel, resolve_kernel replays
+// each block's pixels in the reference's order, with its leaf tries, from
+// those results, and emit_kernel writes the outputs.
+static int render_speculative(const voxg_tree *t, const dev::RenderItem &one,
+                              size_t npix, hipStream_t stream) {
+    voxg_launch_config cfg = current_config(VOXG_LAUNCH_RENDER_BEST);
+    cfg.render_cull = 1;  // the resolve pass reads the pre-pass's queue
+    void *sp = nullptr;
+    int rc = voxg_internal::scratch(stream, npix * 20, &sp);
+    if (rc) return rc;
+    dev::SpecOut spec;
+    spec.vid = (uint32_t *)sp;
+    spec.node = spec.vid + npix;
+    spec.hx = (float *)(spec.node + npix);
+    spec.hy = spec.hx + npix;
+    spec.hz = spec.hy + npix;
+    spec.stride = (uint32_t)(npix / 16);
+    dev::BestWork work;
+    std::memset(&work, 0, sizeof(work));
+    work.one = one;
+    work.spec = spec;
+    work.keep_queue = 1;
+    work.split = 4;
+    work.group = 256u << work.split;
+    rc = render_run<dev::BestWork, false>(t, work, VOXG_LAUNCH_RENDER_BEST,
+                                          cfg, npix, nullptr, stream);
+    if (rc) return rc;
+    Workspace *ws = nullptr;
+    rc = get_ws(stream, 16, &ws);
+    if (rc) return rc;
+    int devid = 0;
+    HIPCHK(hipGetDevice(&devid));
+    const size_t nblk = npix / 16;
+    const size_t grid = std::max<size_t>(
+        1, std::min<size_t>((nblk + 255) / 256, (size_t)cu_count(devid) * 8));
+    hipLaunchKernelGGL(dev::resolve_kernel, dim3((unsigned)grid), dim3(256),
+                       0, stream, one, spec, ws->queue, (uint32_t)nblk,
+                       ws->ctrl);
+    HIPCHK(hipGetLastError());
+    hipLaunchKernelGGL(dev::emit_kernel, dim3((unsigned)grid), dim3(256), 0,
+                       stream, one, spec, ws->queue, (uint32_t)nblk,
+                       ws->ctrl);
+    HIPCHK(hipGetLastError());
+    return 0;
+}
+
 template <bool COUNT>
 static int render_launch(const voxg_tree *t, size_t n_frames,
                          const voxg_camera *cams, int W, int H,
@@ -2081,9 +2294,12 @@ static int render_launch(const voxg_tree *t, size_t n_frames,
         return render_run<dev::BestWork, COUNT>(
             t, work, VOXG_LAUNCH_RENDER_BEST, cfg, npix, counts, stream);
     }
-    // one 4x4 block per unit (renderer.c:214): the leaf of one pixel is
-    // tried first for the next
     const voxg_launch_config cfg = current_config(VOXG_LAUNCH_RENDER);
+    if (!COUNT && !(q & 0xc) && cfg.render_cull && !t->host.nodes.empty())
+        return render_speculative(t, one, npix, stream);
+    // one 4x4 block per unit (renderer.c:214): the leaf of one pixel is
+    // tried first for the next (the counting launch and ray merging take
+    // this path)
     dev::BlockWork<false> work;
     std::memset(&work, 0, sizeof(work));
     work.one = one;
