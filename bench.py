@@ -468,6 +468,25 @@ def main():
 
     extra = {"hit_rate": round(hits / n, 4)}
     if not args.quick:
+        # the other quality modes on the same frames and outputs (default
+        # configurations of their kinds); ADAPTIVE is the reference's
+        # default (renderer.c:78)
+        modes = {}
+        for name, q in (("adaptive", 2), ("fast", 1),
+                        ("adaptive_ray_merge", 2 | 4),
+                        ("adaptive_ray_merge_accurate", 2 | 8)):
+            voxrnd.render(G, cams, W, H, quality=q, out=out)
+            torch.cuda.synchronize()
+            ev0.record(stream)
+            for _ in range(3):
+                voxrnd.render(G, cams, W, H, quality=q, out=out)
+            ev1.record(stream)
+            torch.cuda.synchronize()
+            qms = ev0.elapsed_time(ev1) / 3
+            modes[name] = {"ms": round(qms, 3),
+                           "mrays_s": round(n / qms / 1e3, 1)}
+        extra["render_modes"] = modes
+        log("[bench] other quality modes: %s" % json.dumps(modes))
         bmin, bmax = c2["bbox"]
         o, d = scenes.random_rays(1 << 24, bmin, bmax, seed=11)
         so = [torch.from_numpy(np.ascontiguousarray(o[:, k])).cuda()

@@ -492,6 +492,17 @@ __device__ __forceinline__ bool leaf_search(const Tree &T, const Node &nd,
     return hit;
 }
 
+// The queue counts of the render pre-pass (Ctrl::qpair): entries from the
+// front and in all, clamped to the queue's length, so that counts left by
+// a render whose later launches failed cannot send a reader past the end.
+__device__ __forceinline__ void queue_counts(unsigned long long pr,
+                                             uint32_t qlen, uint32_t &qf,
+                                             uint32_t &qt) {
+    const uint32_t lo = (uint32_t)pr, hi = (uint32_t)(pr >> 32);
+    qf = min(lo, qlen);
+    qt = min(qf + min(hi, qlen), qlen);
+}
+
 // A render job: one tree, a run of camera poses of one size and quality,
 // its output buffers.  Units are 4x4 blocks, `bpf` per frame.
 struct RenderItem {
@@ -538,9 +549,9 @@ struct BlockWork {
     __device__ __forceinline__ uint32_t prepare(uint32_t n) {
         if constexpr (!MULTI) {
             if (queue) {
-                const unsigned long long pr = *qpair;
-                qf = (uint32_t)pr;
-                return (uint32_t)pr + (uint32_t)(pr >> 32);
+                uint32_t qt;
+                queue_counts(*qpair, qlen, qf, qt);
+                return qt;
             }
         }
         return n;
@@ -769,9 +780,9 @@ struct BestWork {
     // returns the number of units
     __device__ __forceinline__ uint32_t prepare(uint32_t n) {
         if (!queue) return n;
-        const unsigned long long pr = *qpair;
-        qf = (uint32_t)pr;
-        return ((uint32_t)pr + (uint32_t)(pr >> 32)) << split;
+        uint32_t qt;
+        queue_counts(*qpair, qlen, qf, qt);
+        return qt << split;
     }
     struct State {
         uint32_t frame, base;  // camera, output index of the block
@@ -1252,14 +1263,14 @@ block_cull_kernel(RenderItem it, uint32_t npix, Ctrl *ctrl,
     }
 }
 
-// Second pass of a FAST or ADAPTIVE render without ray merging
-// (renderer.c:240-316).  The first pass traced every pixel of the queued
-// blocks as a root search, coherently (BestWork into SpecOut); here one
-// thread per queued block replays the block's pixels in the reference's
-// order -- ADAPTIVE's two corners and its mode decision, then each pixel:
-// in FAST mode a try of the previous pixel's leaf first, else (or when the
-// try misses) the root search -- taking every root search from the first
-// pass.  A root search depends on its ray alone, so each pixel gets the
+// Second pass of a FAST or ADAPTIVE render (renderer.c:240-316).  The
+// first pass traced every pixel of the queued blocks as a root search,
+// coherently (BestWork into SpecOut); here one thread per queued block
+// replays the block's pixels in the reference's order -- ADAPTIVE's two
+// corners, its mode and merge decisions, then each pixel: a copy of its
+// predecessor when merging, in FAST mode a try of the previous pixel's
+// leaf first, else (or when the try misses) the root search -- taking
+// every root search from the first pass.  A root search depends on its ray alone, so each pixel gets the
 // result the reference computes; root searches that the reference skips
 // are traced in vain.  A leaf try is the traversal from a leaf node: the
 // leaf's box test, then its voxels.  Results stay in SpecOut (a leaf try
@@ -1267,8 +1278,8 @@ block_cull_kernel(RenderItem it, uint32_t npix, Ctrl *ctrl,
 __global__ void __launch_bounds__(256)
 resolve_kernel(RenderItem it, SpecOut spec, const uint32_t *__restrict__ queue,
                uint32_t qlen, const Ctrl *ctrl) {
-    const unsigned long long pr = ctrl->qpair;
-    const uint32_t qf = (uint32_t)pr, qt = qf + (uint32_t)(pr >> 32);
+    uint32_t qf, qt;
+    queue_counts(ctrl->qpair, qlen, qf, qt);
     const Tree &T = it.t;
     unsigned long long nt = 0;
     for (uint32_t b = blockIdx.x * blockDim.x + threadIdx.x; b < qt;
@@ -1282,27 +1293,39 @@ resolve_kernel(RenderItem it, SpecOut spec, const uint32_t *__restrict__ queue,
         const f3 o = f3{c.px, c.py, c.pz};
         // the first pass's root search of pixel p (all five words loaded
         // together: the node and point are only meaningful on a hit)
-        auto root = [&](int p, uint32_t &nd, f3 &pt) {
+        auto root = [&](int p, uint32_t &vid, uint32_t &nd, f3 &pt) {
             const uint32_t r = (uint32_t)p * spec.stride + id;
-            const uint32_t v = spec.vid[r];
+            vid = spec.vid[r];
             nd = spec.node[r];
             pt = f3{spec.hx[r], spec.hy[r], spec.hz[r]};
-            return v != kEnd;
+            return vid != kEnd;
         };
-        int mode = it.rnd_mode;
-        uint32_t leaf = kEnd;
+        auto store = [&](int p, uint32_t vid, f3 pt) {
+            const uint32_t r = (uint32_t)p * spec.stride + id;
+            spec.vid[r] = vid;
+            spec.hx[r] = pt.x;
+            spec.hy[r] = pt.y;
+            spec.hz[r] = pt.z;
+        };
+        int mode = it.rnd_mode, merge = 0;
+        uint32_t leaf = kEnd, vid_last = kEnd;
+        f3 inter_last = o;
         int i = 0, iend = 16;
         if (mode == VOXG_QUALITY_ADAPTIVE) {
-            // renderer.c:240-281: corners 0 and 15, then FAST or BEST
+            // renderer.c:240-281: corners 0 and 15, then FAST or BEST, and
+            // whether every other pixel copies its predecessor
             mode = VOXG_QUALITY_FAST;
-            uint32_t n0 = kEnd, n15 = kEnd;
-            f3 p0 = o, p15 = o;
-            const bool h0 = root(0, n0, p0);
+            merge = it.merge_mode == VOXG_QUALITY_RAY_MERGE;
+            uint32_t v0, n0, v15, n15;
+            f3 p0, p15;
+            const bool h0 = root(0, v0, n0, p0);
             leaf = h0 ? n0 : kEnd;
             i = 1;
             if (h0) {
+                vid_last = v0;
+                inter_last = p0;
                 iend = 15;
-                if (root(15, n15, p15)) {
+                if (root(15, v15, n15, p15)) {
                     const f3 dir1 = screen2world(c, xs, ys);
                     const f3 dir15 = screen2world(c, xs + 3, ys + 3);
                     const float d1 = sqr_metric(p0, o);
@@ -1312,13 +1335,26 @@ resolve_kernel(RenderItem it, SpecOut spec, const uint32_t *__restrict__ queue,
                     const float dist = sqr_metric(p0, p15);
                     if ((double)(dist / criteria) > 1.69)
                         mode = VOXG_QUALITY_BEST;
+                    if (d1 > 22500.0f) {
+                        if (mode == VOXG_QUALITY_FAST &&
+                            it.merge_mode == VOXG_QUALITY_RAY_MERGE_ACCURATE)
+                            merge = 1;
+                    } else {
+                        merge = 0;
+                    }
                 }
             }
         }
-        if (mode != VOXG_QUALITY_FAST) continue;  // BEST: the root searches
+        // BEST without merging: the root searches are the results
+        if (mode != VOXG_QUALITY_FAST && !merge) continue;
         for (; i < iend; ++i) {
             const int p = render_order(i);
-            if (leaf != kEnd) {
+            if (merge && (i & 1)) {
+                // renderer.c:290, 311-314: a copy of the previous pixel
+                store(p, leaf != kEnd ? vid_last : kEnd, inter_last);
+                continue;
+            }
+            if (mode == VOXG_QUALITY_FAST && leaf != kEnd) {
                 // renderer.c:294-300: the previous pixel's leaf first
                 const f3 d = screen2world(c, xs + (p & 3), ys + (p >> 2));
                 const Node ln = load_node(T.nodes, leaf);
@@ -1326,17 +1362,21 @@ resolve_kernel(RenderItem it, SpecOut spec, const uint32_t *__restrict__ queue,
                 uint32_t vid = kEnd;
                 if (hit_box(ln.bmin, ln.bmax, o, d, e) &&
                     leaf_search<false>(T, ln, e, d, vid, pt, nt)) {
-                    const uint32_t r = (uint32_t)p * spec.stride + id;
-                    spec.vid[r] = vid;
-                    spec.hx[r] = pt.x;
-                    spec.hy[r] = pt.y;
-                    spec.hz[r] = pt.z;
+                    store(p, vid, pt);
+                    vid_last = vid;
+                    inter_last = pt;
                     continue;  // the leaf stays the same
                 }
             }
-            uint32_t nd = kEnd;
+            uint32_t vid, nd;
             f3 pt;
-            leaf = root(p, nd, pt) ? nd : kEnd;
+            if (root(p, vid, nd, pt)) {
+                leaf = nd;
+                vid_last = vid;
+                inter_last = pt;
+            } else {
+                leaf = kEnd;
+            }
         }
     }
 }
@@ -1350,8 +1390,8 @@ resolve_kernel(RenderItem it, SpecOut spec, const uint32_t *__restrict__ queue,
 __global__ void __launch_bounds__(256)
 emit_kernel(RenderItem it, SpecOut spec, const uint32_t *__restrict__ queue,
             uint32_t qlen, Ctrl *ctrl) {
-    const unsigned long long pr = ctrl->qpair;
-    const uint32_t qf = (uint32_t)pr, qt = qf + (uint32_t)(pr >> 32);
+    uint32_t qf, qt;
+    queue_counts(ctrl->qpair, qlen, qf, qt);
     for (uint32_t t = blockIdx.x * blockDim.x + threadIdx.x; t < qt * 16u;
          t += gridDim.x * blockDim.x) {
         const uint32_t b = t >> 4, p = t & 15u;
@@ -2210,7 +2250,7 @@ static int render_run(const voxg_tree *t, Work &work, int kind,
                                     units, stream, counts, false);
 }
 
-// FAST and ADAPTIVE renders without ray merging, after the pre-pass: every
+// FAST and ADAPTIVE renders (not ray merging), after the pre-pass: every
 // pixel of the blocks that the pre-pass queues is root-searched as in a
 // BEST render (coherent groups, BEST-render launch configuration) into a
 // per-stream scratch buffer of 20 bytes per pixel, resolve_kernel replays
@@ -2295,7 +2335,13 @@ static int render_launch(const voxg_tree *t, size_t n_frames,
             t, work, VOXG_LAUNCH_RENDER_BEST, cfg, npix, counts, stream);
     }
     const voxg_launch_config cfg = current_config(VOXG_LAUNCH_RENDER);
-    if (!COUNT && !(q & 0xc) && cfg.render_cull && !t->host.nodes.empty())
+    // three passes (render_speculative) except for the counting launch
+    // and for ray merging, where every other pixel of most blocks is a
+    // copy that the sequential path never traces (4.1 against 4.3 ms on
+    // the 256^3 benchmark frames; accurate merging, which merges far fewer
+    // blocks: 4.4 against 6.3 ms in three passes)
+    if (!COUNT && !(q & VOXG_QUALITY_RAY_MERGE) && cfg.render_cull &&
+        !t->host.nodes.empty())
         return render_speculative(t, one, npix, stream);
     // one 4x4 block per unit (renderer.c:214): the leaf of one pixel is
     // tried first for the next (the counting launch and ray merging take
