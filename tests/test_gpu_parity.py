@@ -349,8 +349,10 @@ def test_render_adaptive_ct_frames(gpu, oracle, q):
 
 @pytest.mark.parametrize("quality", [0, 1, 2, 6, 10])
 def test_render_with_and_without_block_culling(gpu, oracle, quality):
-    """The pre-pass that finishes blocks missing the scene box, and BEST
-    mode's half-block units, change no output and no counter, in every
+    """The pre-pass that finishes blocks missing the scene box, BEST
+    mode's units of 16 >> split_best pixels, and the three-pass path of
+    FAST and ADAPTIVE renders (taken only with the pre-pass; without it
+    they trace block by block) change no output and no counter, in every
     quality mode."""
     from voxvision_amd import voxrnd, voxtrees as VT
     c1 = _c1()
