@@ -1382,26 +1382,58 @@ resolve_kernel(RenderItem it, SpecOut spec, const uint32_t *__restrict__ queue,
 }
 
 // Third pass: the final results of the queued blocks from SpecOut to the
-// render's outputs (square layout), 16 threads per block so that the
-// stores of a block are one contiguous run (one thread per block, with
-// 16 stores each, left lines partly written in L2 and was 3.6x slower).
-// The last workgroup clears the queue counts for the next render (every
-// workgroup read them when it started).
+// render's outputs (square layout).  A workgroup takes 64 blocks at a time
+// and transposes through LDS: it reads the blocks' 16 pixels row by row of
+// SpecOut (neighbouring lanes, neighbouring blocks: consecutive words) and
+// writes each block's run of the outputs with 16 neighbouring lanes.
+// (Without the transpose, 16 lanes per block reading 16 rows took 0.42 ms
+// on the benchmark frames; one thread per block writing its 16 pixels left
+// lines partly written in L2 and was 3.6x slower.)  The last workgroup
+// clears the queue counts for the next render (every workgroup read them
+// when it started).
+constexpr uint32_t kEmitBlocks = 64;
+
 __global__ void __launch_bounds__(256)
 emit_kernel(RenderItem it, SpecOut spec, const uint32_t *__restrict__ queue,
             uint32_t qlen, Ctrl *ctrl) {
+    __shared__ uint32_t sid[kEmitBlocks];
+    __shared__ uint32_t svid[kEmitBlocks * 16];
+    __shared__ float spx[kEmitBlocks * 16], spy[kEmitBlocks * 16],
+        spz[kEmitBlocks * 16];
     uint32_t qf, qt;
     queue_counts(ctrl->qpair, qlen, qf, qt);
-    for (uint32_t t = blockIdx.x * blockDim.x + threadIdx.x; t < qt * 16u;
-         t += gridDim.x * blockDim.x) {
-        const uint32_t b = t >> 4, p = t & 15u;
-        const uint32_t id = queue[b < qf ? b : qlen - 1u - (b - qf)];
-        const uint32_t r = p * spec.stride + id;
-        const uint32_t vid = spec.vid[r];
-        const bool hit = vid != kEnd;
-        f3 pt = f3{0, 0, 0};
-        if (hit) pt = f3{spec.hx[r], spec.hy[r], spec.hz[r]};
-        it.sink.put(id * 16u + p, hit, vid, 0, pt);
+    const uint32_t tid = threadIdx.x;
+    for (uint32_t b0 = blockIdx.x * kEmitBlocks; b0 < qt;
+         b0 += gridDim.x * kEmitBlocks) {
+        const uint32_t nb = min(kEmitBlocks, qt - b0);
+        if (tid < nb) {
+            const uint32_t b = b0 + tid;
+            sid[tid] = queue[b < qf ? b : qlen - 1u - (b - qf)];
+        }
+        __syncthreads();
+        // rows p of SpecOut, 4 at a time: lane k reads block k
+        const uint32_t k = tid & (kEmitBlocks - 1u);
+        for (uint32_t p = tid / kEmitBlocks; p < 16u; p += 256u / kEmitBlocks) {
+            if (k < nb) {
+                const uint32_t r = p * spec.stride + sid[k];
+                const uint32_t v = spec.vid[r];
+                svid[k * 16u + p] = v;
+                if (v != kEnd) {
+                    spx[k * 16u + p] = spec.hx[r];
+                    spy[k * 16u + p] = spec.hy[r];
+                    spz[k * 16u + p] = spec.hz[r];
+                }
+            }
+        }
+        __syncthreads();
+        for (uint32_t j = tid; j < nb * 16u; j += 256u) {
+            const uint32_t v = svid[j];
+            const bool hit = v != kEnd;
+            f3 pt = f3{0, 0, 0};
+            if (hit) pt = f3{spx[j], spy[j], spz[j]};
+            it.sink.put(sid[j >> 4] * 16u + (j & 15u), hit, v, 0, pt);
+        }
+        __syncthreads();
     }
     __syncthreads();
     if (threadIdx.x == 0) {
