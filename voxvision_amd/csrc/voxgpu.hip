@@ -1347,6 +1347,9 @@ resolve_kernel(RenderItem it, SpecOut spec, const uint32_t *__restrict__ queue,
         }
         // BEST without merging: the root searches are the results
         if (mode != VOXG_QUALITY_FAST && !merge) continue;
+        // the record of the leaf last tried (a FAST chain often keeps it)
+        uint32_t loaded = kEnd;
+        Node ln;
         for (; i < iend; ++i) {
             const int p = render_order(i);
             if (merge && (i & 1)) {
@@ -1354,10 +1357,18 @@ resolve_kernel(RenderItem it, SpecOut spec, const uint32_t *__restrict__ queue,
                 store(p, leaf != kEnd ? vid_last : kEnd, inter_last);
                 continue;
             }
+            // the root search is loaded before the leaf try, so that the
+            // two memory round trips overlap
+            uint32_t rvid, rnd;
+            f3 rpt;
+            const bool rhit = root(p, rvid, rnd, rpt);
             if (mode == VOXG_QUALITY_FAST && leaf != kEnd) {
                 // renderer.c:294-300: the previous pixel's leaf first
                 const f3 d = screen2world(c, xs + (p & 3), ys + (p >> 2));
-                const Node ln = load_node(T.nodes, leaf);
+                if (leaf != loaded) {
+                    ln = load_node(T.nodes, leaf);
+                    loaded = leaf;
+                }
                 f3 e, pt;
                 uint32_t vid = kEnd;
                 if (hit_box(ln.bmin, ln.bmax, o, d, e) &&
@@ -1368,12 +1379,10 @@ resolve_kernel(RenderItem it, SpecOut spec, const uint32_t *__restrict__ queue,
                     continue;  // the leaf stays the same
                 }
             }
-            uint32_t vid, nd;
-            f3 pt;
-            if (root(p, vid, nd, pt)) {
-                leaf = nd;
-                vid_last = vid;
-                inter_last = pt;
+            if (rhit) {
+                leaf = rnd;
+                vid_last = rvid;
+                inter_last = rpt;
             } else {
                 leaf = kEnd;
             }
