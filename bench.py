@@ -54,12 +54,38 @@ def cpu_baseline(dots, cams, W, H, budget_s):
     t1 = time.perf_counter()
     T.render(cams[0], W, H, O.QUALITY_BEST, nthreads=1)
     dt1 = time.perf_counter() - t1
+    # the other timings of the CPU restatement: ADAPTIVE frames, looped ray
+    # queries (incoherent, as in the 16 M-ray workload) and ball queries,
+    # on all cores and on one, each on a bounded sample
+    import numpy as np
+    from voxvision_amd import scenes
+    other = {}
+    t1 = time.perf_counter()
+    T.render(cams[1 % len(cams)], W, H, O.QUALITY_ADAPTIVE, nthreads=cores)
+    other["render_adaptive_mrays_s"] = round(
+        W * H / (time.perf_counter() - t1) / 1e6, 2)
+    bmin = dots.min(0).astype(np.float32)
+    bmax = (dots.max(0) + 1).astype(np.float32)
+    o, d = scenes.random_rays(1 << 20, bmin, bmax, seed=11)
+    for name, n, th in (("rays_incoherent", 1 << 20, cores),
+                        ("rays_incoherent_1core", 1 << 17, 1)):
+        t1 = time.perf_counter()
+        T.rays(o[:n], d[:n], nthreads=th)
+        other[name + "_mrays_s"] = round(n / (time.perf_counter() - t1) / 1e6,
+                                         2)
+    c, r = scenes.random_balls(1 << 20, bmin, bmax, seed=13)
+    for name, n, th in (("balls", 1 << 20, cores), ("balls_1core", 1 << 17, 1)):
+        t1 = time.perf_counter()
+        T.balls(c[:n], r[:n], nthreads=th)
+        other[name + "_mqueries_s"] = round(
+            n / (time.perf_counter() - t1) / 1e6, 2)
     T.close()
     return {"value": rays / dt / 1e6, "unit": "Mrays/s", "cores": cores,
             "kind": "port",
             "sample": "%d of the 64 poses at %dx%d (%.1f s, %d threads, "
                       "oracle vox_render loop, BEST); 1 core: %.2f Mrays/s"
-                      % (frames, W, H, dt, cores, W * H / dt1 / 1e6)}
+                      % (frames, W, H, dt, cores, W * H / dt1 / 1e6),
+            "other": other}
 
 
 def batched_leg(ev0, ev1, stream, log):
@@ -462,6 +488,9 @@ def main():
                 "frac": round(achieved / HBM_PEAK_GBS, 5),
                 "traffic": tr.get("hbm_bytes_per_launch") if tr else None,
                 "algorithmic_bytes_per_launch": alg_bytes,
+                # 40 B of I/O per ray alone (24 B in, 16 B out)
+                "compulsory_gb_s": round(BYTES_PER_RAY * n / (ms / 1e3) / 1e9,
+                                         2),
                 "timed": "render call: block_cull_kernel + rti_kernel",
                 "node_visits_per_ray": round(cnt["node_visits"] / n, 4),
                 "voxel_tests_per_ray": round(cnt["voxel_tests"] / n, 4)}
