@@ -27,6 +27,8 @@ METRIC = ("Mrays/s coherent+incoherent on 256³ scene @1080p; "
           "batched vs plain; p99 latency")
 HBM_PEAK_GBS = 8000.0
 BYTES_PER_RAY, BYTES_PER_VISIT, BYTES_PER_TEST = 40, 32, 12
+QUALITY_NAMES = {0: "best", 1: "fast", 2: "adaptive",
+                 6: "adaptive_ray_merge", 10: "adaptive_ray_merge_accurate"}
 
 
 def log(*a):
@@ -400,6 +402,10 @@ def main():
     ap.add_argument("--serve-seconds", type=float, default=3.0)
     ap.add_argument("--serve-rate", type=float, default=1000.0,
                     help="Poisson arrival rate of the serving leg (req/s)")
+    ap.add_argument("--quality", type=int, default=0,
+                    help="render quality of the timed step (0 BEST, the "
+                         "headline; 1 FAST, 2 ADAPTIVE, 6 / 10 with ray "
+                         "merging), for profiling the other modes")
     ap.add_argument("--no-tune", action="store_true",
                     help="use the library's default launch configuration")
     ap.add_argument("--copy-calib-mib", type=int, default=0,
@@ -434,7 +440,7 @@ def main():
     log("[bench] tree: %s" % json.dumps(G.info))
 
     def step():
-        voxrnd.render(G, cams, W, H, out=out)
+        voxrnd.render(G, cams, W, H, quality=args.quality, out=out)
 
     tuned = None
     if not args.no_tune:
@@ -476,11 +482,13 @@ def main():
         % (args.steps, ms, wall))
 
     hits = int((out["voxel"] != -1).sum().item())
-    cnt = voxrnd.render_counts(G, cams, W, H)
+    cnt = voxrnd.render_counts(G, cams, W, H, quality=args.quality)
     alg_bytes = (BYTES_PER_RAY * n + BYTES_PER_VISIT * cnt["node_visits"] +
                  BYTES_PER_TEST * cnt["voxel_tests"])
     achieved = alg_bytes / (ms / 1e3) / 1e9
-    tr = load_traffic("traffic_ct256_render.json")
+    # the committed HBM traffic was measured on the BEST render
+    tr = load_traffic("traffic_ct256_render.json") if args.quality == 0 \
+        else None
     # one render call = the block pre-pass (block_cull_kernel) + the
     # traversal (rti_kernel); both are timed together by the events above
     roofline = {"bound": "hbm", "achieved": round(achieved, 2),
@@ -588,7 +596,8 @@ def main():
         "ms_per_step": round(ms, 4), "higher_is_better": True,
         "scaling": "weak", "vs_baseline": None, "dtype": "f32",
         "data": "synthetic",
-        "config": {"workload": "ct256_orbit64_1080p_best",
+        "config": {"workload": "ct256_orbit64_1080p_" + QUALITY_NAMES.get(
+                       args.quality, "q%d" % args.quality),
                    "scene": "256^3 4-octave value-noise CT shell+interior, "
                             "8%% occupancy, %d voxels" % len(dots),
                    "frames": nf, "W": W, "H": H, "rays_per_step": n,
