@@ -485,3 +485,27 @@ def test_render_rejects_oversized_frames(gpu):
     with pytest.raises(_lib.VoxgError, match="65536"):
         voxrnd.render(G, [c1["camera"]], 16, 65536, want_colors=False,
                       want_hits=False)
+
+
+def test_large_ball_batch(gpu, oracle):
+    """300 K balls on the 256^3 scene: every answer is the oracle's,
+    including centres outside the scene box and centres that are not
+    finite."""
+    from voxvision_amd import scenes, voxtrees as VT
+    c2 = scenes.config_c2(frames=1)
+    T = oracle.Tree(c2["dots"])
+    G = VT.make_tree(c2["dots"])
+    bmin, bmax = c2["bbox"]
+    ext = bmax - bmin
+    rng = np.random.default_rng(17)
+    n = 300000
+    c = rng.uniform(bmin - 0.2 * ext, bmax + 0.2 * ext, (n, 3)).astype(np.float32)
+    r = rng.uniform(0.25, 8.0, n).astype(np.float32)
+    c[::997, 0] = np.nan
+    c[1::991, 1] = np.inf
+    c[2::983, 2] = -np.inf
+    got = VT.tree_ball_collidep(G, c, r)
+    torch.cuda.synchronize()
+    want = T.balls(c, r)
+    assert np.array_equal(got.cpu().numpy(), want)
+    assert 0.05 < want.mean() < 0.95

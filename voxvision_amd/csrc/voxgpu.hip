@@ -3085,8 +3085,12 @@ int voxg_server_destroy(voxg_server *s) {
             std::lock_guard<std::mutex> g(g_mu);
             auto it = g_ws.find({s->device, ss.s});
             if (it != g_ws.end()) {
-                if (it->second->ctrl) (void)hipFree(it->second->ctrl);
-                if (it->second->spill) (void)hipFree(it->second->spill);
+                Workspace *w = it->second.get();
+                if (w->ctrl) (void)hipFree(w->ctrl);
+                if (w->xnext) (void)hipFree(w->xnext);
+                if (w->spill) (void)hipFree(w->spill);
+                if (w->queue) (void)hipFree(w->queue);
+                if (w->aux) (void)hipFree(w->aux);
                 g_ws.erase(it);
             }
         }
