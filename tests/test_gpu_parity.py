@@ -509,3 +509,49 @@ def test_large_ball_batch(gpu, oracle):
     want = T.balls(c, r)
     assert np.array_equal(got.cpu().numpy(), want)
     assert 0.05 < want.mean() < 0.95
+
+
+@pytest.mark.parametrize("quality", [1, 2, 2 | 8])
+def test_render_three_pass_edge_cases(gpu, oracle, quality):
+    """FAST, ADAPTIVE and ADAPTIVE with accurate merging take three passes
+    (root searches, in-order replay, output); edge cases against the
+    oracle: a camera looking away (no block queued), a camera inside the
+    scene box, and 40 frames of 16x4 pixels (one block row per frame, so
+    a pre-pass tile spans many frames)."""
+    from voxvision_amd import camera, voxrnd, voxtrees as VT
+    c1 = _c1()
+    T = oracle.Tree(c1["dots"])
+    G = VT.make_tree(c1["dots"])
+    bmin, bmax = c1["bbox"]
+    ctr = (bmin + bmax) / 2
+    diag = float(np.linalg.norm(bmax - bmin))
+    fov = np.tan(np.radians(30))
+    cases = []
+    away = camera.make_camera(ctr + [0, -diag, 0], ctr + [0, -3 * diag, 0],
+                              fov, 64, 48)
+    inside = camera.make_camera(ctr, ctr + [diag, 0.3 * diag, 0.1 * diag],
+                                fov, 64, 48)
+    cases.append(([away], 64, 48))
+    cases.append(([inside], 64, 48))
+    rng = np.random.default_rng(3)
+    tiny = []
+    for k in range(40):
+        u = rng.normal(size=3)
+        u /= np.linalg.norm(u)
+        tiny.append(camera.make_camera(ctr + 1.2 * diag * u, ctr, fov, 16, 4))
+    cases.append((tiny, 16, 4))
+    for cams, W, H in cases:
+        out = voxrnd.render(G, cams, W, H, quality=quality, want_colors=False)
+        torch.cuda.synchronize()
+        vg = _np_u32(out["voxel"]).reshape(len(cams), -1)
+        hg = out["hit"].cpu().numpy().reshape(3, len(cams), -1)
+        for i, cam in enumerate(cams):
+            _, vox_o, hit_o, _ = T.render(cam, W, H, quality)
+            assert np.array_equal(vg[i], vox_o), (W, H, i)
+            m = vox_o != MISS
+            assert np.array_equal(hg[:, i, m].view(np.uint32),
+                                  hit_o[:, m].view(np.uint32))
+    # the away camera sees nothing; the inside one sees something
+    _, v_away, _, _ = T.render(away, 64, 48, quality)
+    _, v_in, _, _ = T.render(inside, 64, 48, quality)
+    assert (v_away == MISS).all() and (v_in != MISS).any()
