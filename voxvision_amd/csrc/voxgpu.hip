@@ -1647,12 +1647,21 @@ int get_ws(hipStream_t s, size_t spill_bytes, Workspace **out) {
     std::lock_guard<std::mutex> g(g_mu);
     auto &slot = g_ws[{devid, s}];
     if (!slot) {
-        slot.reset(new Workspace());
-        HIPCHK(hipMalloc(&slot->ctrl, sizeof(dev::Ctrl)));
-        HIPCHK(hipMemset(slot->ctrl, 0, sizeof(dev::Ctrl)));
+        // registered only once both buffers exist and are cleared
+        std::unique_ptr<Workspace> w(new Workspace());
         const size_t xb = dev::kXcds * dev::kXcdStride * sizeof(uint32_t);
-        HIPCHK(hipMalloc(&slot->xnext, xb));
-        HIPCHK(hipMemset(slot->xnext, 0, xb));
+        hipError_t e = hipMalloc(&w->ctrl, sizeof(dev::Ctrl));
+        if (e == hipSuccess) e = hipMemset(w->ctrl, 0, sizeof(dev::Ctrl));
+        if (e == hipSuccess) e = hipMalloc(&w->xnext, xb);
+        if (e == hipSuccess) e = hipMemset(w->xnext, 0, xb);
+        if (e != hipSuccess) {
+            if (w->ctrl) (void)hipFree(w->ctrl);
+            if (w->xnext) (void)hipFree(w->xnext);
+            g_ws.erase({devid, s});
+            return fail(VOXG_EIO, std::string("workspace: ") +
+                                      hipGetErrorString(e));
+        }
+        slot = std::move(w);
     }
     if (spill_bytes > slot->spill_bytes) {
         if (slot->spill) HIPCHK(hipFree(slot->spill));
