@@ -35,9 +35,17 @@ def _params(quality, texture, fmt):
 def render(tree, cams, W, H, quality=QUALITY_BEST, texture=None,
            fmt=ARGB8888, want_colors=True, want_voxels=True, want_hits=True,
            out=None, stream=None):
-    """Render len(cams) frames.  Returns dict of device tensors in the
-    square layout (index f*W*H + cs*16 + p): squares (uint32 colours, 0
-    where nothing was hit), voxel (int32, -1 = miss), hit (3, f*W*H)."""
+    """Render len(cams) frames (voxg_render).  Returns dict of device
+    tensors in the square layout (index f*W*H + cs*16 + p): squares (uint32
+    colours, 0 where nothing was hit), voxel (int32, -1 = miss), hit
+    (3, f*W*H; NaN where nothing was hit when allocated here).
+
+    Every quality gives the reference's results.  BEST traces one pixel per
+    work unit (launch kind "render_best"); FAST, ADAPTIVE and ADAPTIVE with
+    accurate merging root-search every pixel the same way, then replay each
+    4x4 block in the reference's order (leaf reuse, corners, merging),
+    using a per-stream scratch buffer of 20 bytes per pixel; ADAPTIVE with
+    plain ray merging traces block by block (launch kind "render")."""
     L = _lib.load()
     if not torch.is_tensor(cams):
         cams = cameras_tensor(cams)
