@@ -511,6 +511,54 @@ def test_large_ball_batch(gpu, oracle):
     assert 0.05 < want.mean() < 0.95
 
 
+def test_incoherent_rays_and_balls_at_full_size(gpu, oracle):
+    """The stress workload on the 256^3 scene at its full size: 16 M rays
+    with origins in the box grown by 10% per side and random directions,
+    and 4 M ball queries with radii in [0.5, 8].  2 M of the rays are
+    compared with the oracle bit for bit; of the full launches every 61st
+    ray and every 17th ball is, and each full launch gives the same
+    answers as separate launches of its own slices."""
+    from voxvision_amd import scenes, voxtrees as VT
+    c2 = scenes.config_c2(frames=1)
+    T = oracle.Tree(c2["dots"])
+    G = VT.make_tree(c2["dots"])
+    bmin, bmax = c2["bbox"]
+    o2, d2 = scenes.random_rays(1 << 21, bmin, bmax, seed=41)
+    vox_g, hit_g = VT.ray_tree_intersection(G, o2, d2)
+    torch.cuda.synchronize()
+    vox_o, hit_o = T.rays(o2, d2)
+    _assert_rays_equal(vox_g, hit_g, vox_o, hit_o)
+    assert 0.02 < (vox_o != MISS).mean() < 0.9
+
+    n = 1 << 24
+    o, d = scenes.random_rays(n, bmin, bmax, seed=11)
+    vox_f, hit_f = VT.ray_tree_intersection(G, o, d)
+    torch.cuda.synchronize()
+    pick = np.arange(0, n, 61)
+    vox_o, hit_o = T.rays(o[pick], d[pick])
+    _assert_rays_equal(vox_f[pick], hit_f[:, pick], vox_o, hit_o)
+    for a, b in ((0, 1000), (n // 2 - 777, n // 2 + 4096), (n - 65536, n)):
+        vs, hs = VT.ray_tree_intersection(G, o[a:b], d[a:b])
+        torch.cuda.synchronize()
+        assert torch.equal(vs, vox_f[a:b])
+        m = vs != MISS
+        assert torch.equal(hs[:, m], hit_f[:, a:b][:, m])
+
+    nb = 1 << 22
+    c, r = scenes.random_balls(nb, bmin, bmax, seed=13)
+    got = VT.tree_ball_collidep(G, c, r)
+    torch.cuda.synchronize()
+    got = got.cpu().numpy()
+    bpick = np.arange(0, nb, 17)
+    want = T.balls(c[bpick], r[bpick])
+    assert np.array_equal(got[bpick], want)
+    assert 0.05 < want.mean() < 0.95
+    for a, b in ((0, 999), (nb - 70000, nb)):
+        part = VT.tree_ball_collidep(G, c[a:b], r[a:b])
+        torch.cuda.synchronize()
+        assert np.array_equal(part.cpu().numpy(), got[a:b])
+
+
 @pytest.mark.parametrize("quality", [1, 2, 2 | 8])
 def test_render_three_pass_edge_cases(gpu, oracle, quality):
     """FAST, ADAPTIVE and ADAPTIVE with accurate merging take three passes
