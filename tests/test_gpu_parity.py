@@ -485,6 +485,17 @@ def test_render_rejects_oversized_frames(gpu):
     with pytest.raises(_lib.VoxgError, match="65536"):
         voxrnd.render(G, [c1["camera"]], 16, 65536, want_colors=False,
                       want_hits=False)
+    # frame counts whose pixel count reaches 2^31, also where the product
+    # n_frames * W * H would wrap around 2^64
+    L = _lib.load()
+    cams = voxrnd.cameras_tensor([c1["camera"]])
+    stream = C.c_void_p(torch.cuda.current_stream().cuda_stream)
+    for n_frames, W, H in ((1 << 25, 16, 4), (1 << 60, 16, 4),
+                           ((1 << 62) + 1, 64, 64)):
+        rc = L.voxg_render(G.handle, n_frames, cams.data_ptr(), W, H, None,
+                           None, None, None, stream)
+        assert rc == -34, L.voxg_last_error()  # VOXG_ERANGE
+    torch.cuda.synchronize()
 
 
 def test_large_ball_batch(gpu, oracle):

@@ -2235,8 +2235,10 @@ static int render_common(const voxg_tree *t, size_t n_frames,
     // block corners are kept in 16-bit fields
     if (W >= 65536 || H >= 65536)
         return fail(VOXG_ERANGE, "W and H must be < 65536");
-    size_t per = (size_t)W * (size_t)H;
-    if (n_frames * per >= 0x80000000ull)
+    // n_frames * per < 2^31, tested without forming a product that could
+    // wrap around
+    const size_t per = (size_t)W * (size_t)H;
+    if (n_frames > 0x7fffffffull / per)
         return fail(VOXG_ERANGE, "n_frames * W * H must be < 2^31");
     *npix = n_frames * per;
     return 0;
