@@ -91,3 +91,21 @@ def test_batch_graph_replay(gpu):
         assert torch.equal(vox, ref)
         m = ref != -1
         assert torch.equal(hit[:, m], refh[:, m])
+
+
+def test_batch_rejects_ray_ranges_that_would_wrap(gpu):
+    """An item's first ray and ray count must each stay below 2^31, so
+    that their sum cannot wrap around 2^64 into a small index."""
+    import ctypes as C
+    from voxvision_amd import _lib, voxtrees as VT
+    G = VT.make_tree(np.zeros((1, 3), np.float32))
+    L = _lib.load()
+    for first, count in (((1 << 64) - 1, 2), ((1 << 63), (1 << 63) + 5),
+                         (1 << 31, 0), (0, 1 << 31)):
+        items = (_lib.BatchItem * 1)()
+        items[0].tree = G.handle
+        items[0].first = first
+        items[0].count = count
+        out = C.c_void_p()
+        assert L.voxg_batch_create(1, items, C.byref(out)) == -34
+        assert not out.value

@@ -249,6 +249,21 @@ int voxg_redraw(uint32_t *squares, uint32_t *pixels, size_t n_frames, int W,
     return 0;
 }
 
+// dim[0] * dim[1] * dim[2] into *n, or false when that reaches `limit`:
+// tested factor by factor, so that no product wraps around
+static bool volume_samples(const unsigned dim[3], uint64_t limit,
+                           uint64_t *n) {
+    *n = 0;
+    if (dim[0] == 0 || dim[1] == 0 || dim[2] == 0) return true;
+    uint64_t p = 1;
+    for (int i = 0; i < 3; i++) {
+        if (p > (limit - 1) / dim[i]) return false;
+        p *= dim[i];
+    }
+    *n = p;
+    return true;
+}
+
 int voxg_raw_select(const void *samples, const unsigned dim[3],
                     unsigned samplesize, uint64_t lo, uint64_t hi,
                     const float voxel[3], float *xyz_out, size_t capacity,
@@ -256,8 +271,8 @@ int voxg_raw_select(const void *samples, const unsigned dim[3],
     if (!dim || !count_out) return fail(VOXG_EINVAL, "dim or count_out is NULL");
     if (samplesize < 1 || samplesize > 4)
         return fail(VOXG_EINVAL, "samplesize must be 1, 2, 3 or 4");
-    const uint64_t n = (uint64_t)dim[0] * dim[1] * dim[2];
-    if (n >= (1ull << 32) - kTile)
+    uint64_t n = 0;
+    if (!volume_samples(dim, (1ull << 32) - kTile, &n))
         return fail(VOXG_ERANGE, "volume must hold fewer than 2^32 samples");
     if (n && !samples) return fail(VOXG_EINVAL, "samples is NULL");
     const uint32_t ntiles = (uint32_t)((n + kTile - 1) / kTile);
@@ -296,7 +311,9 @@ int voxg_read_raw_data(const char *filename, const unsigned dim[3],
     if (!filename || !dim) return fail(VOXG_EINVAL, "filename or dim is NULL");
     if (samplesize < 1 || samplesize > 4)
         return fail(VOXG_EINVAL, "samplesize must be 1, 2, 3 or 4");
-    const uint64_t n = (uint64_t)dim[0] * dim[1] * dim[2];
+    uint64_t n = 0;
+    if (!volume_samples(dim, (1ull << 32) - kTile, &n))
+        return fail(VOXG_ERANGE, "volume must hold fewer than 2^32 samples");
     const int fd = open(filename, O_RDONLY);
     if (fd < 0) return fail(VOXG_EIO, std::string(filename) + ": " + strerror(errno));
     struct stat sb;

@@ -2445,8 +2445,9 @@ int voxg_batch_create(size_t n_items, const voxg_batch_item *items,
     if (n_items > (1u << 20)) return fail(VOXG_ERANGE, "too many items");
     int devid = 0;
     HIPCHK(hipGetDevice(&devid));
-    uint64_t total = 0;
-    for (size_t i = 0; i < n_items; i++) total += items[i].count;
+    uint64_t total = 0;  // only picks the chunk; each count is checked below
+    for (size_t i = 0; i < n_items; i++)
+        total += std::min<uint64_t>(items[i].count, 0x80000000ull);
     uint32_t chunk = effective_chunk((size_t)total, VOXG_LAUNCH_BATCH);
     std::vector<dev::BatchItem> h;
     uint64_t units = 0, rays_end = 0;
@@ -2455,7 +2456,9 @@ int voxg_batch_create(size_t n_items, const voxg_batch_item *items,
         const voxg_batch_item &it = items[i];
         int rc = check_tree(it.tree);
         if (rc) return rc;
-        if (it.first + it.count >= 0x80000000ull)
+        // each term below 2^31 first, so that the sum cannot wrap
+        if (it.first >= 0x80000000ull || it.count >= 0x80000000ull ||
+            it.first + it.count >= 0x80000000ull)
             return fail(VOXG_ERANGE, "ray indices must stay below 2^31");
         if (it.count == 0) continue;
         dev::BatchItem b;
