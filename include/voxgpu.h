@@ -162,7 +162,8 @@ int voxg_ball_batch(const voxg_tree *tree, size_t n, const float *cx,
  * earlier stay valid and unchanged until released.  Not thread-safe per
  * handle. */
 typedef struct voxg_edit_tree voxg_edit_tree;
-/* vox_make_tree over n points (stride 3 or 4 floats; voxel NULL = 1,1,1). */
+/* vox_make_tree over n points (stride 3 or 4 floats; voxel NULL = 1,1,1,
+ * else every component > 0, or VOXG_EINVAL). */
 int voxg_edit_create(const float *xyz, size_t n, size_t stride,
                      const float voxel[3], voxg_edit_tree **out);
 /* vox_insert_voxel: the point is aligned down to the voxel grid first;

@@ -139,6 +139,25 @@ def test_volume_sizes_that_would_wrap_are_refused(voxlib):
         assert not out.value
 
 
+def test_edit_create_rejects_bad_voxel_sizes(voxlib):
+    """Like voxg_tree_build, voxg_edit_create takes only positive voxel
+    sizes (a zero or NaN size has no grid to align points to)."""
+    pts = np.zeros((2, 3), np.float32)
+    pts[1] = 3
+    for bad in ((1, 0, 1), (1, 1, -2), (float("nan"), 1, 1)):
+        v = (C.c_float * 3)(*bad)
+        out = C.c_void_p()
+        assert voxlib.voxg_edit_create(pts.ctypes.data, 2, 3, v,
+                                       C.byref(out)) == -22
+        assert not out.value
+    out = C.c_void_p()
+    assert voxlib.voxg_edit_create(pts.ctypes.data, 2, 3,
+                                   (C.c_float * 3)(1, 2, 1),
+                                   C.byref(out)) == 0
+    assert voxlib.voxg_edit_voxels(out) == 2
+    voxlib.voxg_edit_release(out)
+
+
 def test_camera_setter_matches_reference_formula(voxlib):
     from voxvision_amd import _lib, camera
     for (w, h) in ((640, 480), (1920, 1080), (100, 100), (240, 320)):

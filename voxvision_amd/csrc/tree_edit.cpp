@@ -423,6 +423,8 @@ int voxg_edit_create(const float *xyz, size_t n, size_t stride,
     if (n && !xyz) return fail_edit("xyz is NULL", VOXG_EINVAL);
     if (stride != 3 && stride != 4)
         return fail_edit("stride must be 3 or 4", VOXG_EINVAL);
+    if (voxel && !(voxel[0] > 0.0f && voxel[1] > 0.0f && voxel[2] > 0.0f))
+        return fail_edit("voxel size must be positive", VOXG_EINVAL);
     auto *t = new (std::nothrow) voxg_edit_tree();
     if (!t) return fail_edit("host allocation failed", VOXG_ENOMEM);
     if (voxel)
