@@ -324,11 +324,11 @@ def test_render_fast_adaptive_merge_match_oracle(gpu, oracle, quality):
     assert cnt["voxel_tests"] == st["voxel_tests"]
 
 
-@pytest.mark.parametrize("q", [2, 2 | 8, 1])
+@pytest.mark.parametrize("q", [2, 2 | 4, 2 | 8, 1])
 def test_render_adaptive_ct_frames(gpu, oracle, q):
     """ADAPTIVE (the reference's default quality, renderer.c:78), ADAPTIVE
-    with accurate ray merging and FAST on two 1080p poses of the 256^3
-    scene: many blocks of both modes, long leaf-reuse chains."""
+    with plain or accurate ray merging and FAST on two 1080p poses of the
+    256^3 scene: many blocks of both modes, long leaf-reuse chains."""
     from voxvision_amd import scenes, voxrnd, voxtrees as VT
     c2 = scenes.config_c2(frames=64)
     T = oracle.Tree(c2["dots"])
@@ -570,13 +570,15 @@ def test_incoherent_rays_and_balls_at_full_size(gpu, oracle):
         assert np.array_equal(part.cpu().numpy(), got[a:b])
 
 
-@pytest.mark.parametrize("quality", [1, 2, 2 | 8])
+@pytest.mark.parametrize("quality", [1, 2, 2 | 4, 2 | 8])
 def test_render_three_pass_edge_cases(gpu, oracle, quality):
-    """FAST, ADAPTIVE and ADAPTIVE with accurate merging take three passes
-    (root searches, in-order replay, output); edge cases against the
-    oracle: a camera looking away (no block queued), a camera inside the
-    scene box, and 40 frames of 16x4 pixels (one block row per frame, so
-    a pre-pass tile spans many frames)."""
+    """FAST, ADAPTIVE and ADAPTIVE with either kind of ray merging take
+    the speculated passes (root searches, in-order replay, output; with
+    plain merging the odd pixels only for the blocks whose merging stops);
+    edge cases against the oracle: a camera looking away (no block
+    queued), a camera inside the scene box (with plain merging, blocks
+    that keep merging and blocks that stop), and 40 frames of 16x4 pixels
+    (one block row per frame, so a pre-pass tile spans many frames)."""
     from voxvision_amd import camera, voxrnd, voxtrees as VT
     c1 = _c1()
     T = oracle.Tree(c1["dots"])
@@ -612,5 +614,8 @@ def test_render_three_pass_edge_cases(gpu, oracle, quality):
                                   hit_o[:, m].view(np.uint32))
     # the away camera sees nothing; the inside one sees something
     _, v_away, _, _ = T.render(away, 64, 48, quality)
-    _, v_in, _, _ = T.render(inside, 64, 48, quality)
+    _, v_in, _, st_in = T.render(inside, 64, 48, quality)
     assert (v_away == MISS).all() and (v_in != MISS).any()
+    if quality == 2 | 4:
+        # plain merging starts in every block and stops in some
+        assert 0 < st_in["merge_blocks"] < st_in["blocks"]

@@ -324,6 +324,10 @@ struct Ctrl {
     // scene box (queued from the front); high word, the other entering
     // blocks (queued from the back)
     unsigned long long qpair;
+    // FAST/ADAPTIVE renders with plain ray merging: blocks whose merging
+    // stopped, so that the replay needs the odd pixels the first pass
+    // skipped (low word; the high word stays 0), listed by resolve_kernel
+    unsigned long long rpair;
 };
 // the counters cleared before a counting launch: visits .. spills
 constexpr size_t kCounterBytes =
@@ -543,7 +547,7 @@ struct BlockWork {
         else return group;
     }
     __device__ __forceinline__ uint32_t unit_shift() const { return 0; }
-    uint32_t unit_shift_host() const { return 0; }
+    uint32_t units_per_block_host() const { return 1u; }
     // reads once per thread what the block pre-pass counted on the device;
     // returns the number of units
     __device__ __forceinline__ uint32_t prepare(uint32_t n) {
@@ -773,16 +777,23 @@ struct BestWork {
     uint32_t qlen, qf;      // front, the rest from the back of qlen entries
     uint32_t split;         // a unit is 16 >> split pixels
     uint32_t group;         // units dealt to the XCDs in turns of `group`
+    // 0: all 16 pixels of a block; one pixel per unit (split 4) of
+    // 1: pixels 0, 2, ..., 14 and 15 of the block's order (9 units), or
+    // 2: pixels 1, 3, ..., 13 (7 units); pdiv divides by the units per block
+    uint32_t pset;
+    FastDiv pdiv;
     __device__ __forceinline__ uint32_t xcd_group() const { return group; }
     __device__ __forceinline__ uint32_t unit_shift() const { return split; }
-    uint32_t unit_shift_host() const { return split; }
+    uint32_t units_per_block_host() const {
+        return pset ? pdiv.d : 1u << split;
+    }
     // reads once per thread what the block pre-pass counted on the device;
     // returns the number of units
     __device__ __forceinline__ uint32_t prepare(uint32_t n) {
         if (!queue) return n;
         uint32_t qt;
         queue_counts(*qpair, qlen, qf, qt);
-        return qt << split;
+        return pset ? qt * pdiv.d : qt << split;
     }
     struct State {
         uint32_t frame, base;  // camera, output index of the block
@@ -803,8 +814,20 @@ struct BestWork {
     }
     __device__ __forceinline__ void begin(uint32_t unit, State &st,
                                           uint32_t &cur, f3 &org, f3 &d) const {
-        const uint32_t part = unit & ((1u << split) - 1u);
-        uint32_t b = unit >> split;
+        uint32_t b, i, len, own;
+        if (pset) {
+            b = pdiv.div(unit);
+            const uint32_t k = unit - b * pdiv.d;
+            i = pset == 1u ? (k < 8u ? 2u * k : 15u) : 2u * k + 1u;
+            len = 1u;
+            own = k == 0u;
+        } else {
+            const uint32_t part = unit & ((1u << split) - 1u);
+            b = unit >> split;
+            len = 16u >> split;
+            i = part * len;
+            own = part == 0u;
+        }
         if (queue) b = queue[b < qf ? b : qlen - 1u - (b - qf)];
         // frames are stored one after the other in the square layout
         st.frame = one.bpf.div(b);
@@ -812,8 +835,7 @@ struct BestWork {
         const uint32_t row = one.ws.div(cs);
         st.base = b * 16u;
         st.xy = ((cs - row * one.ws.d) * 4u) | ((row * 4u) << 16);
-        const uint32_t i = part * (16u >> split);
-        st.pos = i | ((i + (16u >> split)) << 8) | ((part ? 0u : 1u) << 16);
+        st.pos = i | ((i + len) << 8) | (own << 16);
         trace(st, cur, org, d);
     }
     __device__ __forceinline__ bool finish(State &st, bool hit, uint32_t vid,
@@ -1245,10 +1267,15 @@ block_cull_kernel(RenderItem it, uint32_t npix, Ctrl *ctrl,
     __syncthreads();
     uint32_t off = before + incl - mine;  // exclusive, both halves
     uint32_t of = gbase[0] + (off & 0xffffu), ob = gbase[1] + (off >> 16);
+    // the counts start at 0 unless a render failed after its pre-pass:
+    // entries beyond the queue are then dropped (the readers clamp the
+    // counts), never written out of bounds
     for (uint32_t j = 0; j < 4; ++j) {
         const uint32_t id = tile0 / 16 + 4 * tid + j;
-        if (cls[j] == 2) queue[of++] = id;
-        if (cls[j] == 1) queue[qlen - 1u - ob++] = id;
+        if (cls[j] == 2 && of < qlen) queue[of] = id;
+        if (cls[j] == 1 && ob < qlen) queue[qlen - 1u - ob] = id;
+        of += cls[j] == 2;
+        ob += cls[j] == 1;
     }
     if (COUNT && tid == 0 && missed) {
         const unsigned long long nb = missed;
@@ -1275,16 +1302,30 @@ block_cull_kernel(RenderItem it, uint32_t npix, Ctrl *ctrl,
 // are traced in vain.  A leaf try is the traversal from a leaf node: the
 // leaf's box test, then its voxels.  Results stay in SpecOut (a leaf try
 // that hits replaces the root search there); emit_kernel writes them out.
+//
+// With plain ray merging the first pass traced only the pixels a block
+// that keeps merging needs (0, 2, ..., 14 and 15: the odd ones are copies,
+// renderer.c:290).  A block whose merging stops (both corners hit within
+// 150 units of the camera, renderer.c:274-278) is listed in `redo` instead
+// of replayed (redo_pass 0); once its odd pixels are traced too, a second
+// launch (redo_pass 1) replays the listed blocks.
 __global__ void __launch_bounds__(256)
 resolve_kernel(RenderItem it, SpecOut spec, const uint32_t *__restrict__ queue,
-               uint32_t qlen, const Ctrl *ctrl) {
+               uint32_t qlen, Ctrl *ctrl, uint32_t *__restrict__ redo,
+               int redo_pass) {
     uint32_t qf, qt;
-    queue_counts(ctrl->qpair, qlen, qf, qt);
+    if (redo_pass) {
+        qf = qt = min((uint32_t)ctrl->rpair, qlen);
+    } else {
+        queue_counts(ctrl->qpair, qlen, qf, qt);
+    }
     const Tree &T = it.t;
     unsigned long long nt = 0;
     for (uint32_t b = blockIdx.x * blockDim.x + threadIdx.x; b < qt;
          b += gridDim.x * blockDim.x) {
-        const uint32_t id = queue[b < qf ? b : qlen - 1u - (b - qf)];
+        const uint32_t id = redo_pass
+                                ? redo[b]
+                                : queue[b < qf ? b : qlen - 1u - (b - qf)];
         const uint32_t f = it.bpf.div(id);
         const uint32_t cs = id - f * it.bpf.d;
         const uint32_t row = it.ws.div(cs);
@@ -1344,6 +1385,26 @@ resolve_kernel(RenderItem it, SpecOut spec, const uint32_t *__restrict__ queue,
                     }
                 }
             }
+        }
+        if (redo && !redo_pass) {
+            // the block's merging stopped: list it (one atomic per wave)
+            const bool wait = it.merge_mode == VOXG_QUALITY_RAY_MERGE && !merge;
+            const unsigned long long m = __ballot(wait);
+            if (m) {
+                const uint32_t lane = lane_id();
+                const uint32_t leader = (uint32_t)(__ffsll((long long)m) - 1);
+                uint32_t base = 0;
+                if (lane == leader)
+                    base = (uint32_t)atomicAdd(
+                        &ctrl->rpair, (unsigned long long)__popcll(m));
+                base = __shfl(base, (int)leader);
+                const uint32_t slot =
+                    base + (uint32_t)__popcll(m & ((1ull << lane) - 1ull));
+                // (a count left by a failed render cannot push a write
+                // past the list; the readers clamp the count)
+                if (wait && slot < qlen) redo[slot] = id;
+            }
+            if (wait) continue;
         }
         // BEST without merging: the root searches are the results
         if (mode != VOXG_QUALITY_FAST && !merge) continue;
@@ -1449,6 +1510,7 @@ emit_kernel(RenderItem it, SpecOut spec, const uint32_t *__restrict__ queue,
         __threadfence();
         if (atomicAdd(&ctrl->rdone, 1u) == gridDim.x - 1u) {
             atomicExch(&ctrl->qpair, 0ull);
+            atomicExch(&ctrl->rpair, 0ull);
             atomicExch(&ctrl->rdone, 0u);
         }
     }
@@ -2276,7 +2338,7 @@ static int render_run(const voxg_tree *t, Work &work, int kind,
                       const voxg_launch_config &cfg, size_t npix,
                       voxg_counts *counts, hipStream_t stream) {
     const size_t nblk = npix / 16;
-    const size_t units = nblk << work.unit_shift_host();
+    const size_t units = nblk * work.units_per_block_host();
     if (!cfg.render_cull || t->host.nodes.empty())
         return launch<Work, COUNT>(kind, t, work, units, stream, counts);
     // pre-pass: finish the blocks whose rays all miss the root box and
@@ -2302,18 +2364,24 @@ static int render_run(const voxg_tree *t, Work &work, int kind,
                                     units, stream, counts, false);
 }
 
-// FAST and ADAPTIVE renders (not ray merging), after the pre-pass: every
-// pixel of the blocks that the pre-pass queues is root-searched as in a
-// BEST render (coherent groups, BEST-render launch configuration) into a
-// per-stream scratch buffer of 20 bytes per pixel, resolve_kernel replays
-// each block's pixels in the reference's order, with its leaf tries, from
-// those results, and emit_kernel writes the outputs.
+// FAST and ADAPTIVE renders, after the pre-pass: every pixel of the
+// blocks that the pre-pass queues is root-searched as in a BEST render
+// (coherent groups, BEST-render launch configuration) into a per-stream
+// scratch buffer of 20 bytes per pixel, resolve_kernel replays each
+// block's pixels in the reference's order, with its leaf tries, from those
+// results, and emit_kernel writes the outputs.  With plain ray merging
+// (skip_odd) the first pass traces 9 of a block's 16 pixels, the ones a
+// block that keeps merging needs; the blocks whose merging stops get their
+// other 7 traced and are replayed by two more launches before the output
+// pass.
 static int render_speculative(const voxg_tree *t, const dev::RenderItem &one,
-                              size_t npix, hipStream_t stream) {
+                              size_t npix, bool skip_odd, hipStream_t stream) {
     voxg_launch_config cfg = current_config(VOXG_LAUNCH_RENDER_BEST);
     cfg.render_cull = 1;  // the resolve pass reads the pre-pass's queue
+    const size_t nblk = npix / 16;
     void *sp = nullptr;
-    int rc = voxg_internal::scratch(stream, npix * 20, &sp);
+    // root-search results, then the list of blocks whose merging stopped
+    int rc = voxg_internal::scratch(stream, npix * 20 + nblk * 4, &sp);
     if (rc) return rc;
     dev::SpecOut spec;
     spec.vid = (uint32_t *)sp;
@@ -2321,7 +2389,8 @@ static int render_speculative(const voxg_tree *t, const dev::RenderItem &one,
     spec.hx = (float *)(spec.node + npix);
     spec.hy = spec.hx + npix;
     spec.hz = spec.hy + npix;
-    spec.stride = (uint32_t)(npix / 16);
+    spec.stride = (uint32_t)nblk;
+    uint32_t *redo = skip_odd ? (uint32_t *)(spec.hz + npix) : nullptr;
     dev::BestWork work;
     std::memset(&work, 0, sizeof(work));
     work.one = one;
@@ -2329,6 +2398,11 @@ static int render_speculative(const voxg_tree *t, const dev::RenderItem &one,
     work.keep_queue = 1;
     work.split = 4;
     work.group = 256u << work.split;
+    if (skip_odd) {
+        work.pset = 1;
+        work.pdiv = make_fastdiv(9);
+        work.group = 256u * 9u;
+    }
     rc = render_run<dev::BestWork, false>(t, work, VOXG_LAUNCH_RENDER_BEST,
                                           cfg, npix, nullptr, stream);
     if (rc) return rc;
@@ -2337,13 +2411,30 @@ static int render_speculative(const voxg_tree *t, const dev::RenderItem &one,
     if (rc) return rc;
     int devid = 0;
     HIPCHK(hipGetDevice(&devid));
-    const size_t nblk = npix / 16;
     const size_t grid = std::max<size_t>(
         1, std::min<size_t>((nblk + 255) / 256, (size_t)cu_count(devid) * 8));
     hipLaunchKernelGGL(dev::resolve_kernel, dim3((unsigned)grid), dim3(256),
                        0, stream, one, spec, ws->queue, (uint32_t)nblk,
-                       ws->ctrl);
+                       ws->ctrl, redo, 0);
     HIPCHK(hipGetLastError());
+    if (skip_odd) {
+        // the odd pixels of the listed blocks, then their replay
+        dev::BestWork odd = work;
+        odd.pset = 2;
+        odd.pdiv = make_fastdiv(7);
+        odd.group = 256u * 7u;
+        odd.queue = redo;
+        odd.qpair = &ws->ctrl->rpair;
+        odd.qlen = (uint32_t)nblk;
+        rc = launch_tree<dev::BestWork, false>(
+            VOXG_LAUNCH_RENDER_BEST, dev_tree(t), t->stack_bound, odd,
+            nblk * 7, stream, nullptr, false);
+        if (rc) return rc;
+        hipLaunchKernelGGL(dev::resolve_kernel, dim3((unsigned)grid),
+                           dim3(256), 0, stream, one, spec, ws->queue,
+                           (uint32_t)nblk, ws->ctrl, redo, 1);
+        HIPCHK(hipGetLastError());
+    }
     hipLaunchKernelGGL(dev::emit_kernel, dim3((unsigned)grid), dim3(256), 0,
                        stream, one, spec, ws->queue, (uint32_t)nblk,
                        ws->ctrl);
@@ -2387,17 +2478,15 @@ static int render_launch(const voxg_tree *t, size_t n_frames,
             t, work, VOXG_LAUNCH_RENDER_BEST, cfg, npix, counts, stream);
     }
     const voxg_launch_config cfg = current_config(VOXG_LAUNCH_RENDER);
-    // three passes (render_speculative) except for the counting launch
-    // and for ray merging, where every other pixel of most blocks is a
-    // copy that the sequential path never traces (4.1 against 4.3 ms on
-    // the 256^3 benchmark frames; accurate merging, which merges far fewer
-    // blocks: 4.4 against 6.3 ms in three passes)
-    if (!COUNT && !(q & VOXG_QUALITY_RAY_MERGE) && cfg.render_cull &&
-        !t->host.nodes.empty())
-        return render_speculative(t, one, npix, stream);
+    // speculated passes (render_speculative) except for the counting
+    // launch; with plain ray merging the first pass skips the odd pixels,
+    // which most blocks copy
+    if (!COUNT && cfg.render_cull && !t->host.nodes.empty())
+        return render_speculative(t, one, npix,
+                                  (q & VOXG_QUALITY_RAY_MERGE) != 0, stream);
     // one 4x4 block per unit (renderer.c:214): the leaf of one pixel is
-    // tried first for the next (the counting launch and ray merging take
-    // this path)
+    // tried first for the next (the counting launch and renders without
+    // the pre-pass take this path)
     dev::BlockWork<false> work;
     std::memset(&work, 0, sizeof(work));
     work.one = one;
