@@ -233,9 +233,10 @@ typedef struct {
  * (the reference's square layout, renderer.h:14).  Outputs (each may be
  * NULL): squares = colours (pixels that miss are untouched), voxel_px =
  * voxel id or VOXG_MISS, hit_px = 3 planes of f*W*H floats (x, y, z),
- * untouched on a miss.  FAST and ADAPTIVE renders (not ray merging) use a
- * device scratch buffer of the stream, 20 bytes per pixel, allocated on the
- * first call that needs more and kept until the process ends. */
+ * untouched on a miss.  FAST and ADAPTIVE renders (with or without ray
+ * merging) use a device scratch buffer of the stream, 20 bytes per pixel +
+ * 4 per 4x4 block, allocated on the first call that needs more and kept
+ * until the process ends. */
 int voxg_render(const voxg_tree *tree, size_t n_frames,
                 const voxg_camera *cams, int W, int H,
                 const voxg_render_params *params, uint32_t *squares,

@@ -41,11 +41,13 @@ def render(tree, cams, W, H, quality=QUALITY_BEST, texture=None,
     (3, f*W*H; NaN where nothing was hit when allocated here).
 
     Every quality gives the reference's results.  BEST traces one pixel per
-    work unit (launch kind "render_best"); FAST, ADAPTIVE and ADAPTIVE with
-    accurate merging root-search every pixel the same way, then replay each
-    4x4 block in the reference's order (leaf reuse, corners, merging),
-    using a per-stream scratch buffer of 20 bytes per pixel; ADAPTIVE with
-    plain ray merging traces block by block (launch kind "render")."""
+    work unit (launch kind "render_best"); FAST and ADAPTIVE (with or
+    without ray merging) root-search the pixels the same way, then replay
+    each 4x4 block in the reference's order (leaf reuse, corners, merging),
+    using a per-stream scratch buffer of 20 bytes per pixel.  With plain
+    ray merging the odd pixels of a block, copies while it merges, are
+    traced only for the blocks whose merging stops.  Counting launches
+    trace block by block (launch kind "render")."""
     L = _lib.load()
     if not torch.is_tensor(cams):
         cams = cameras_tensor(cams)
