@@ -570,15 +570,17 @@ def test_incoherent_rays_and_balls_at_full_size(gpu, oracle):
         assert np.array_equal(part.cpu().numpy(), got[a:b])
 
 
-@pytest.mark.parametrize("quality", [1, 2, 2 | 4, 2 | 8])
+@pytest.mark.parametrize("quality", [0, 1, 2, 2 | 4, 2 | 8])
 def test_render_three_pass_edge_cases(gpu, oracle, quality):
     """FAST, ADAPTIVE and ADAPTIVE with either kind of ray merging take
     the speculated passes (root searches, in-order replay, output; with
     plain merging the odd pixels only for the blocks whose merging stops);
-    edge cases against the oracle: a camera looking away (no block
-    queued), a camera inside the scene box (with plain merging, blocks
-    that keep merging and blocks that stop), and 40 frames of 16x4 pixels
-    (one block row per frame, so a pre-pass tile spans many frames)."""
+    BEST the pre-pass and one pixel per unit.  Edge cases against the
+    oracle: a camera looking away (no block queued), a camera inside the
+    scene box (with plain merging, blocks that keep merging and blocks
+    that stop), frames with an odd number of block rows, and 40 frames of
+    16x4 pixels (one block row per frame, so a pre-pass tile spans many
+    frames)."""
     from voxvision_amd import camera, voxrnd, voxtrees as VT
     c1 = _c1()
     T = oracle.Tree(c1["dots"])
@@ -594,6 +596,7 @@ def test_render_three_pass_edge_cases(gpu, oracle, quality):
                                 fov, 64, 48)
     cases.append(([away], 64, 48))
     cases.append(([inside], 64, 48))
+    cases.append(([inside, away, inside], 48, 44))
     rng = np.random.default_rng(3)
     tiny = []
     for k in range(40):
