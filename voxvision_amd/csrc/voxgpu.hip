@@ -760,8 +760,12 @@ struct SpecOut {
 // one pixel per unit, the lanes that one refill starts trace neighbouring
 // pixels together: coherent rays, the same nodes at the same time.  Keeps
 // only the state BEST mode needs, which leaves registers for more waves
-// than BlockWork.
-struct BestWork {
+// than BlockWork.  SETS: one pixel per unit from a set of a block's pixels
+// (pset below), for the ray-merging passes of ADAPTIVE renders; a
+// separate instantiation, so that the other renders' units cost nothing
+// more.
+template <bool SETS>
+struct BestWorkT {
     // 7 waves per SIMD (at most 72 VGPRs): the compiler fits this kernel
     // without spilling, and the extra wave hides node-load latency
     static constexpr int kMinWavesPerSimd = 7;
@@ -777,15 +781,16 @@ struct BestWork {
     uint32_t qlen, qf;      // front, the rest from the back of qlen entries
     uint32_t split;         // a unit is 16 >> split pixels
     uint32_t group;         // units dealt to the XCDs in turns of `group`
-    // 0: all 16 pixels of a block; one pixel per unit (split 4) of
-    // 1: pixels 0, 2, ..., 14 and 15 of the block's order (9 units), or
-    // 2: pixels 1, 3, ..., 13 (7 units); pdiv divides by the units per block
+    // SETS only: one pixel per unit of 1: pixels 0, 2, ..., 14 and 15 of
+    // the block's order (9 units per block) or 2: pixels 1, 3, ..., 13 (7
+    // units); pdiv divides by the units per block
     uint32_t pset;
     FastDiv pdiv;
     __device__ __forceinline__ uint32_t xcd_group() const { return group; }
     __device__ __forceinline__ uint32_t unit_shift() const { return split; }
     uint32_t units_per_block_host() const {
-        return pset ? pdiv.d : 1u << split;
+        if constexpr (SETS) return pdiv.d;
+        else return 1u << split;
     }
     // reads once per thread what the block pre-pass counted on the device;
     // returns the number of units
@@ -793,7 +798,8 @@ struct BestWork {
         if (!queue) return n;
         uint32_t qt;
         queue_counts(*qpair, qlen, qf, qt);
-        return pset ? qt * pdiv.d : qt << split;
+        if constexpr (SETS) return qt * pdiv.d;
+        else return qt << split;
     }
     struct State {
         uint32_t frame, base;  // camera, output index of the block
@@ -815,7 +821,7 @@ struct BestWork {
     __device__ __forceinline__ void begin(uint32_t unit, State &st,
                                           uint32_t &cur, f3 &org, f3 &d) const {
         uint32_t b, i, len, own;
-        if (pset) {
+        if constexpr (SETS) {
             b = pdiv.div(unit);
             const uint32_t k = unit - b * pdiv.d;
             i = pset == 1u ? (k < 8u ? 2u * k : 15u) : 2u * k + 1u;
@@ -860,6 +866,8 @@ struct BestWork {
         atomicAdd(&ctrl->best_blocks, (unsigned long long)st.best);
     }
 };
+using BestWork = BestWorkT<false>;
+using BestSetWork = BestWorkT<true>;
 
 // ---------------------------------------------------------------------------
 // persistent traversal kernel
@@ -2391,20 +2399,30 @@ static int render_speculative(const voxg_tree *t, const dev::RenderItem &one,
     spec.hz = spec.hy + npix;
     spec.stride = (uint32_t)nblk;
     uint32_t *redo = skip_odd ? (uint32_t *)(spec.hz + npix) : nullptr;
-    dev::BestWork work;
-    std::memset(&work, 0, sizeof(work));
-    work.one = one;
-    work.spec = spec;
-    work.keep_queue = 1;
-    work.split = 4;
-    work.group = 256u << work.split;
+    // one pixel per unit, into the scratch; the queue counts stay for the
+    // passes below
+    auto first_pass = [&](auto &work) {
+        std::memset(&work, 0, sizeof(work));
+        work.one = one;
+        work.spec = spec;
+        work.keep_queue = 1;
+        work.split = 4;
+    };
     if (skip_odd) {
+        dev::BestSetWork work;
+        first_pass(work);
         work.pset = 1;
         work.pdiv = make_fastdiv(9);
         work.group = 256u * 9u;
+        rc = render_run<dev::BestSetWork, false>(
+            t, work, VOXG_LAUNCH_RENDER_BEST, cfg, npix, nullptr, stream);
+    } else {
+        dev::BestWork work;
+        first_pass(work);
+        work.group = 256u << work.split;
+        rc = render_run<dev::BestWork, false>(
+            t, work, VOXG_LAUNCH_RENDER_BEST, cfg, npix, nullptr, stream);
     }
-    rc = render_run<dev::BestWork, false>(t, work, VOXG_LAUNCH_RENDER_BEST,
-                                          cfg, npix, nullptr, stream);
     if (rc) return rc;
     Workspace *ws = nullptr;
     rc = get_ws(stream, 16, &ws);
@@ -2419,14 +2437,15 @@ static int render_speculative(const voxg_tree *t, const dev::RenderItem &one,
     HIPCHK(hipGetLastError());
     if (skip_odd) {
         // the odd pixels of the listed blocks, then their replay
-        dev::BestWork odd = work;
+        dev::BestSetWork odd;
+        first_pass(odd);
         odd.pset = 2;
         odd.pdiv = make_fastdiv(7);
         odd.group = 256u * 7u;
         odd.queue = redo;
         odd.qpair = &ws->ctrl->rpair;
         odd.qlen = (uint32_t)nblk;
-        rc = launch_tree<dev::BestWork, false>(
+        rc = launch_tree<dev::BestSetWork, false>(
             VOXG_LAUNCH_RENDER_BEST, dev_tree(t), t->stack_bound, odd,
             nblk * 7, stream, nullptr, false);
         if (rc) return rc;
