@@ -760,18 +760,21 @@ struct SpecOut {
 // one pixel per unit, the lanes that one refill starts trace neighbouring
 // pixels together: coherent rays, the same nodes at the same time.  Keeps
 // only the state BEST mode needs, which leaves registers for more waves
-// than BlockWork.  SETS: one pixel per unit from a set of a block's pixels
-// (pset below), for the ray-merging passes of ADAPTIVE renders; a
-// separate instantiation, so that the other renders' units cost nothing
-// more.
-template <bool SETS>
+// than BlockWork.  MODE 0: results to the job's outputs (BEST renders);
+// 1: root searches into SpecOut (FAST and ADAPTIVE renders); 2: the same
+// for one pixel per unit from a set of a block's pixels (pset below), the
+// ray-merging passes of ADAPTIVE renders.  Separate instantiations, so
+// that no unit pays for another mode's branches (a shared pixel-set branch
+// cost the BEST render 1%).
+template <int MODE>
 struct BestWorkT {
+    static constexpr bool SETS = MODE == 2;
     // 7 waves per SIMD (at most 72 VGPRs): the compiler fits this kernel
     // without spilling, and the extra wave hides node-load latency
     static constexpr int kMinWavesPerSimd = 7;
     RenderItem one;
-    SpecOut spec;            // set: root searches of all queued pixels for
-                             // resolve_kernel instead of the job's outputs
+    SpecOut spec;            // MODE 1, 2: the root searches, for
+                             // resolve_kernel, instead of the job's outputs
     uint32_t keep_queue;     // the queue counts are still needed afterwards
     __device__ __forceinline__ bool keeps_queue() const {
         return keep_queue != 0;
@@ -848,7 +851,7 @@ struct BestWorkT {
                                            uint32_t node, f3 e, uint32_t &cur,
                                            f3 &org, f3 &d) const {
         const int p = render_order((int)(st.pos & 0xffu));
-        if (spec.vid) spec.put(st.base >> 4, p, hit, vid, node, e);
+        if constexpr (MODE != 0) spec.put(st.base >> 4, p, hit, vid, node, e);
         else one.sink.put(st.base + (uint32_t)p, hit, vid, 0, e);
         st.pos++;
         if ((st.pos & 0xffu) < ((st.pos >> 8) & 0xffu)) {
@@ -866,8 +869,9 @@ struct BestWorkT {
         atomicAdd(&ctrl->best_blocks, (unsigned long long)st.best);
     }
 };
-using BestWork = BestWorkT<false>;
-using BestSetWork = BestWorkT<true>;
+using BestWork = BestWorkT<0>;
+using BestSpecWork = BestWorkT<1>;
+using BestSetWork = BestWorkT<2>;
 
 // ---------------------------------------------------------------------------
 // persistent traversal kernel
@@ -2417,10 +2421,10 @@ static int render_speculative(const voxg_tree *t, const dev::RenderItem &one,
         rc = render_run<dev::BestSetWork, false>(
             t, work, VOXG_LAUNCH_RENDER_BEST, cfg, npix, nullptr, stream);
     } else {
-        dev::BestWork work;
+        dev::BestSpecWork work;
         first_pass(work);
         work.group = 256u << work.split;
-        rc = render_run<dev::BestWork, false>(
+        rc = render_run<dev::BestSpecWork, false>(
             t, work, VOXG_LAUNCH_RENDER_BEST, cfg, npix, nullptr, stream);
     }
     if (rc) return rc;
