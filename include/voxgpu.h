@@ -4,7 +4,10 @@
  *
  * Plain C: pointers, sizes and status codes, no C++ or torch types.  Device
  * buffers are HIP device pointers (hipMalloc'd or torch CUDA tensors); all
- * batched calls are asynchronous on the given stream unless stated.
+ * batched calls are asynchronous on the given stream unless stated.  Calls
+ * on distinct streams may come from any threads at once; calls on one
+ * stream must come from one thread at a time (the library keeps per-stream
+ * scratch buffers, grown on demand after the stream's earlier work).
  *
  * Reference interfaces replaced (paths relative to the reference root):
  *   voxg_tree_upload / voxg_tree_build   vox_make_tree      src/voxtrees/tree.h:60
@@ -269,11 +272,15 @@ int voxg_render_count(const voxg_tree *tree, size_t n_frames,
 /* ---- batched form: many trees in one persistent launch ----
  * A batch is a device work queue of items, each a tree and a run of
  * consecutive rays [first, first+count) of the ray arrays passed at launch.
- * voxg_batch_create uploads the queue once (it allocates; call it outside
- * graph capture); voxg_batch_ray is a single kernel launch that needs no
- * memset, so it can be captured in a HIP graph and replayed.  Results are
- * those of voxg_ray_batch on each item's tree.  Trees must outlive the
- * batch; the ray chunk of the launch configuration is fixed at creation. */
+ * voxg_batch_create uploads the queue once and allocates the batch's own
+ * launch workspace (call it outside graph capture); voxg_batch_ray is a
+ * single kernel launch that needs no memset and uses only buffers owned
+ * by the batch, so it can be captured in a HIP graph and replayed for as
+ * long as the batch lives, whatever other calls run in between.  Results
+ * are those of voxg_ray_batch on each item's tree.  Trees must outlive the
+ * batch; the VOXG_LAUNCH_BATCH launch configuration is taken at creation.
+ * Launches of one batch must not overlap (one stream, or synchronise);
+ * voxg_batch_release waits for the device. */
 typedef struct voxg_batch voxg_batch;
 typedef struct {
     const voxg_tree *tree;
@@ -453,6 +460,21 @@ int voxg_get_kind_launch_config(int kind, voxg_launch_config *cfg);
 
 /* Human-readable text of the last error on this thread. */
 const char *voxg_last_error(void);
+
+/* Waits for the work queued on the stream, then reports VOXG_EIO if a
+ * traversal launched on it since the last check exceeded its stack bound
+ * (3 pending entries per inner level, which the search order never
+ * exceeds; the affected rays would be left as misses).  The flag is also
+ * checked, without waiting, at the start of every later traversal launch
+ * on the stream and by the counting calls.  Replaces the host-side wait a
+ * caller of the reference's synchronous searches does not need
+ * (src/voxtrees/search.h:24-37 return when done). */
+int voxg_synchronize(hipStream_t stream);
+
+/* Frees the buffers the library keeps for the stream (on the current
+ * device), after waiting for it.  Call before hipStreamDestroy on a stream
+ * the library has used; a later call on the stream allocates them again. */
+int voxg_release_stream(hipStream_t stream);
 
 #ifdef __cplusplus
 }

@@ -1,0 +1,31 @@
+/*
+ * voxgpu_testing.h -- test hooks of libvoxgpu.  No counterpart in the
+ * reference; for the library's own tests of its failure handling, which
+ * force an error from the host so that no kernel ever fails on purpose.
+ */
+#ifndef VOXGPU_TESTING_H
+#define VOXGPU_TESTING_H
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* Arms a one-shot failure on the calling thread: the checkpoint-th host
+ * step of the next render calls (1 = the first traversal launch after the
+ * block pre-pass, then the replay and output launches of FAST and ADAPTIVE
+ * renders) returns VOXG_EIO instead of launching, as a failed launch
+ * would.  0 disarms.  Later calls must give the results of a render that
+ * never failed. */
+int voxg_test_fail_at(int checkpoint);
+
+/* Caps the traversal stack bound of later launches (process-wide; 0
+ * removes the cap).  Below a tree's true bound, rays that need more
+ * pending entries lose the extra ones, finish as misses, and the stream's
+ * overflow flag is set for voxg_synchronize and the next launch to
+ * report.  Every access stays inside the buffers sized for the cap. */
+int voxg_test_stack_bound(unsigned bound);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

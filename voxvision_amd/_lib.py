@@ -16,7 +16,7 @@ QUALITY_ADAPTIVE = 0x2
 QUALITY_RAY_MERGE = 0x4
 QUALITY_RAY_MERGE_ACCURATE = 0x8
 
-# Every symbol include/voxgpu.h declares.
+# Every symbol include/voxgpu.h and include/voxgpu_testing.h declare.
 EXPORTS = [
     "voxg_init", "voxg_tree_build", "voxg_tree_upload", "voxg_tree_plan",
     "voxg_tree_plan_reference", "voxg_tree_release",
@@ -34,6 +34,8 @@ EXPORTS = [
     "voxg_edit_create", "voxg_edit_insert", "voxg_edit_delete",
     "voxg_edit_rebuild", "voxg_edit_root", "voxg_edit_voxels",
     "voxg_edit_commit", "voxg_edit_release", "voxg_server_replace_tree",
+    "voxg_synchronize", "voxg_release_stream", "voxg_test_fail_at",
+    "voxg_test_stack_bound",
 ]
 
 
@@ -151,6 +153,10 @@ def load():
     L.voxg_edit_voxels.restype = S
     L.voxg_edit_commit.argtypes = [P, C.POINTER(P)]
     L.voxg_edit_release.argtypes = [P]
+    L.voxg_synchronize.argtypes = [P]
+    L.voxg_release_stream.argtypes = [P]
+    L.voxg_test_fail_at.argtypes = [I]
+    L.voxg_test_stack_bound.argtypes = [C.c_uint]
     _lib = L
     return L
 

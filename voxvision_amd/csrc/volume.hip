@@ -334,7 +334,10 @@ int voxg_read_raw_data(const char *filename, const unsigned dim[3],
     hipStream_t s = nullptr;
     int rc = 0;
     auto cleanup = [&]() {  // best effort: the first error is reported
-        if (s) (void)hipStreamDestroy(s);
+        if (s) {
+            voxg_internal::release_stream(s);
+            (void)hipStreamDestroy(s);
+        }
         if (host) (void)hipHostFree(host);
         if (hxyz) (void)hipHostFree(hxyz);
         if (dev) (void)hipFree(dev);

@@ -18,6 +18,10 @@ int fail(int code, const std::string &msg);
 // queued on the same stream may use it; work on other streams gets its own.
 int scratch(hipStream_t stream, size_t bytes, void **out);
 
+// Frees every buffer the library keeps for (current device, stream); call
+// before destroying a stream the library has used.  Waits for the stream.
+void release_stream(hipStream_t stream);
+
 }  // namespace voxg_internal
 
 #define VOXG_HIPCHK(expr)                                                  \

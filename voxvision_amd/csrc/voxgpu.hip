@@ -25,6 +25,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <atomic>
 #include <cmath>
 #include <cstdio>
 #include <cstring>
@@ -314,7 +315,7 @@ struct PixSink {
 struct Ctrl {
     uint32_t rdone;    // workgroups of resolve_kernel that have finished
     uint32_t left;     // waves that have finished
-    uint32_t overflow; // set when a stack bound was exceeded (never expected)
+    uint32_t pad0;
     uint32_t pad;
     unsigned long long visits, tests, rays, leaf_trav;
     unsigned long long fast_blocks, best_blocks, merge_blocks;
@@ -349,7 +350,7 @@ struct ChunkWork {
     // no occupancy floor asked of the register allocator (it fits 7 waves
     // per SIMD as it is; 8 would spill)
     static constexpr int kMinWavesPerSimd = 1;
-    __device__ __forceinline__ bool keeps_queue() const { return false; }
+    __host__ __device__ __forceinline__ bool keeps_queue() const { return false; }
     // XCD hand-out: one contiguous range of the units per XCD
     __device__ __forceinline__ uint32_t xcd_group() const { return 0; }
     __device__ __forceinline__ uint32_t unit_shift() const { return 0; }
@@ -396,7 +397,7 @@ template <class Src, class Sink>
 struct MultiWork {
     // 6 waves per SIMD (at most 80 VGPRs) fit without spilling
     static constexpr int kMinWavesPerSimd = 6;
-    __device__ __forceinline__ bool keeps_queue() const { return false; }
+    __host__ __device__ __forceinline__ bool keeps_queue() const { return false; }
     __device__ __forceinline__ uint32_t xcd_group() const { return 0; }
     __device__ __forceinline__ uint32_t unit_shift() const { return 0; }
     Src src;
@@ -529,7 +530,7 @@ struct BlockWork {
     // 6 waves per SIMD (at most 80 VGPRs) fit without spilling for one
     // render; the multi-job form would spill at that floor
     static constexpr int kMinWavesPerSimd = MULTI ? 1 : 6;
-    __device__ __forceinline__ bool keeps_queue() const { return false; }
+    __host__ __device__ __forceinline__ bool keeps_queue() const { return false; }
     // one render's block queue is ordered longest-first (block_cull_kernel)
     RenderItem one;             // the job when !MULTI
     const RenderItem *items;    // the jobs when MULTI, by unit_begin
@@ -776,7 +777,7 @@ struct BestWorkT {
     SpecOut spec;            // MODE 1, 2: the root searches, for
                              // resolve_kernel, instead of the job's outputs
     uint32_t keep_queue;     // the queue counts are still needed afterwards
-    __device__ __forceinline__ bool keeps_queue() const {
+    __host__ __device__ __forceinline__ bool keeps_queue() const {
         return keep_queue != 0;
     }
     const uint32_t *queue;  // block queue of the pre-pass, or null: the
@@ -881,7 +882,8 @@ constexpr uint32_t kXcdStride = 64;  // counters 256 bytes apart
 template <int K, class Work, bool COUNT>
 __global__ void __launch_bounds__(256)
 __attribute__((amdgpu_waves_per_eu(COUNT ? 1 : Work::kMinWavesPerSimd, 8)))
-rti_kernel(Tree t, Work work_arg, uint32_t n, Ctrl *ctrl, uint32_t *xnext, uint4 *__restrict__ spill,
+rti_kernel(Tree t, Work work_arg, uint32_t n, Ctrl *ctrl, uint32_t *ovf,
+           uint32_t *xnext, uint4 *__restrict__ spill,
            uint32_t spill_slots, uint32_t refill, uint32_t fin_batch,
            uint32_t leaf_batch) {
     extern __shared__ uint4 ring[];  // [K][blockDim.x]
@@ -1052,7 +1054,7 @@ rti_kernel(Tree t, Work work_arg, uint32_t n, Ctrl *ctrl, uint32_t *xnext, uint4
     if ((CI) != kEnd) {                                                  \
         if (sp - lo == K) {                                              \
             if ((uint32_t)lo >= spill_slots) {                           \
-                atomicOr(&ctrl->overflow, 1u);                           \
+                *(volatile uint32_t *)ovf = 1u;                          \
             } else {                                                     \
                 spill[(size_t)lo * gthreads + gtid] =                    \
                     ring[(lo & (K - 1)) * bdim + tid];                   \
@@ -1651,11 +1653,20 @@ struct voxg_tree {
     uint32_t stack_bound = 0;
 };
 
+namespace {
+struct Workspace;
+}
+
 struct voxg_batch {
     int device = 0;
     dev::BatchItem *d_items = nullptr;
     uint32_t n_items = 0, units = 0, chunk = 1, stack_bound = 1;
     uint64_t rays_end = 0;  // one past the highest ray index used
+    // launch configuration and workspace fixed at creation, so that a graph
+    // captured around voxg_batch_ray never refers to buffers that another
+    // call could reallocate
+    voxg_launch_config cfg;
+    Workspace *ws = nullptr;
 };
 
 namespace {
@@ -1686,8 +1697,35 @@ struct Workspace {
     size_t queue_bytes = 0;
     void *aux = nullptr;        // scratch of the other translation units
     size_t aux_bytes = 0;
+    // stack-overflow flag of the traversals launched on this stream: pinned,
+    // mapped host memory that a kernel sets (it is never expected to); read
+    // and cleared by the host at the next launch or synchronising call
+    uint32_t *ovf_h = nullptr, *ovf_d = nullptr;
+    // a render enqueued its pre-pass but not the launch that clears the
+    // queue counts (a later step failed): the next render clears them first
+    bool dirty = false;
 };
 std::map<std::pair<int, hipStream_t>, std::unique_ptr<Workspace>> g_ws;
+
+void free_workspace(Workspace *w) {
+    if (w->ctrl) (void)hipFree(w->ctrl);
+    if (w->xnext) (void)hipFree(w->xnext);
+    if (w->spill) (void)hipFree(w->spill);
+    if (w->queue) (void)hipFree(w->queue);
+    if (w->aux) (void)hipFree(w->aux);
+    if (w->ovf_h) (void)hipHostFree(w->ovf_h);
+}
+
+// Test hook: the n-th host checkpoint of a render after this is armed
+// returns VOXG_EIO as if a launch had failed (0 = disarmed).
+thread_local int g_inject = 0;
+bool injected_failure() {
+    return g_inject > 0 && --g_inject == 0;
+}
+// Test hook: when nonzero, the stack bound given to traversal launches is
+// at most this (so that a bound violation can be forced from the host; the
+// kernel then drops the entries beyond it and sets the overflow flag).
+std::atomic<uint32_t> g_test_bound{0};
 
 int cu_count(int device) {
     static std::map<int, int> cache;
@@ -1715,29 +1753,48 @@ uint32_t effective_chunk(size_t n, int kind) {
     return (uint32_t)std::min(c, fit);
 }
 
+// The control record, the unit counters (cleared) and the overflow flag
+// of a new workspace; on failure everything allocated is freed.
+int init_workspace(Workspace *w) {
+    const size_t xb = dev::kXcds * dev::kXcdStride * sizeof(uint32_t);
+    hipError_t e = hipMalloc(&w->ctrl, sizeof(dev::Ctrl));
+    if (e == hipSuccess) e = hipMemset(w->ctrl, 0, sizeof(dev::Ctrl));
+    if (e == hipSuccess) e = hipMalloc(&w->xnext, xb);
+    if (e == hipSuccess) e = hipMemset(w->xnext, 0, xb);
+    if (e == hipSuccess)
+        e = hipHostMalloc((void **)&w->ovf_h, 256,
+                          hipHostMallocMapped | hipHostMallocCoherent);
+    if (e == hipSuccess) {
+        *(volatile uint32_t *)w->ovf_h = 0u;
+        e = hipHostGetDevicePointer((void **)&w->ovf_d, w->ovf_h, 0);
+    }
+    if (e != hipSuccess) {
+        free_workspace(w);
+        *w = Workspace();
+        return fail(VOXG_EIO, std::string("workspace: ") +
+                                  hipGetErrorString(e));
+    }
+    return 0;
+}
+
 int get_ws(hipStream_t s, size_t spill_bytes, Workspace **out) {
     int devid = 0;
     HIPCHK(hipGetDevice(&devid));
     std::lock_guard<std::mutex> g(g_mu);
     auto &slot = g_ws[{devid, s}];
     if (!slot) {
-        // registered only once both buffers exist and are cleared
+        // registered only once its buffers exist and are cleared
         std::unique_ptr<Workspace> w(new Workspace());
-        const size_t xb = dev::kXcds * dev::kXcdStride * sizeof(uint32_t);
-        hipError_t e = hipMalloc(&w->ctrl, sizeof(dev::Ctrl));
-        if (e == hipSuccess) e = hipMemset(w->ctrl, 0, sizeof(dev::Ctrl));
-        if (e == hipSuccess) e = hipMalloc(&w->xnext, xb);
-        if (e == hipSuccess) e = hipMemset(w->xnext, 0, xb);
-        if (e != hipSuccess) {
-            if (w->ctrl) (void)hipFree(w->ctrl);
-            if (w->xnext) (void)hipFree(w->xnext);
+        int rc = init_workspace(w.get());
+        if (rc) {
             g_ws.erase({devid, s});
-            return fail(VOXG_EIO, std::string("workspace: ") +
-                                      hipGetErrorString(e));
+            return rc;
         }
         slot = std::move(w);
     }
     if (spill_bytes > slot->spill_bytes) {
+        // kernels already queued on the stream may still use the old buffer
+        HIPCHK(hipStreamSynchronize(s));
         if (slot->spill) HIPCHK(hipFree(slot->spill));
         slot->spill = nullptr;
         slot->spill_bytes = 0;
@@ -1748,9 +1805,10 @@ int get_ws(hipStream_t s, size_t spill_bytes, Workspace **out) {
     return 0;
 }
 
-int get_queue(Workspace *ws, size_t bytes) {
+int get_queue(Workspace *ws, hipStream_t s, size_t bytes) {
     std::lock_guard<std::mutex> g(g_mu);
     if (bytes > ws->queue_bytes) {
+        HIPCHK(hipStreamSynchronize(s));
         if (ws->queue) HIPCHK(hipFree(ws->queue));
         ws->queue = nullptr;
         ws->queue_bytes = 0;
@@ -1771,6 +1829,7 @@ int scratch(hipStream_t stream, size_t bytes, void **out) {
     if (rc) return rc;
     std::lock_guard<std::mutex> g(g_mu);
     if (bytes > ws->aux_bytes) {
+        HIPCHK(hipStreamSynchronize(stream));
         if (ws->aux) HIPCHK(hipFree(ws->aux));
         ws->aux = nullptr;
         ws->aux_bytes = 0;
@@ -1780,7 +1839,52 @@ int scratch(hipStream_t stream, size_t bytes, void **out) {
     *out = ws->aux;
     return 0;
 }
+
+void release_stream(hipStream_t stream) {
+    int devid = 0;
+    if (hipGetDevice(&devid) != hipSuccess) return;
+    (void)hipStreamSynchronize(stream);
+    std::lock_guard<std::mutex> g(g_mu);
+    auto it = g_ws.find({devid, stream});
+    if (it == g_ws.end()) return;
+    free_workspace(it->second.get());
+    g_ws.erase(it);
+}
 }  // namespace voxg_internal
+
+namespace {
+
+// Reads and clears the stream's overflow flag: VOXG_EIO when a traversal
+// launched on it exceeded its stack bound (the ray of that lane was left
+// incomplete), else 0.
+int take_overflow(Workspace *ws) {
+    volatile uint32_t *f = ws->ovf_h;
+    if (f && *f) {
+        *f = 0u;
+        return fail(VOXG_EIO, "a traversal launched on this stream exceeded "
+                              "its stack bound; its results are incomplete");
+    }
+    return 0;
+}
+
+// Clears the block-queue counts of a render that failed after its
+// pre-pass (a fill kernel, so that the call can be graph-captured).
+__global__ void clear_queue_kernel(dev::Ctrl *c) {
+    c->qpair = 0ull;
+    c->rpair = 0ull;
+    c->rdone = 0u;
+}
+
+int clear_dirty_queue(Workspace *ws, hipStream_t stream) {
+    if (!ws->dirty) return 0;
+    hipLaunchKernelGGL(clear_queue_kernel, dim3(1), dim3(1), 0, stream,
+                       ws->ctrl);
+    HIPCHK(hipGetLastError());
+    ws->dirty = false;
+    return 0;
+}
+
+}  // namespace
 
 namespace {
 
@@ -1807,28 +1911,53 @@ dev::Tree dev_tree(const voxg_tree *t) {
     return d;
 }
 
-template <int K, class Work, bool COUNT>
-int launch_k(const dev::Tree &dt, uint32_t stack_bound, const Work &work,
-             size_t n, hipStream_t stream, const voxg_launch_config &cfg,
-             voxg_counts *counts, bool reset_counts) {
-    int devid = 0;
-    HIPCHK(hipGetDevice(&devid));
-    int cus = cu_count(devid);
-    int block = cfg.block;
-    int waves = std::max(1, cfg.waves_per_cu);
-    long long threads = (long long)cus * waves * 64;
+// Grid of a persistent traversal launch of n units: waves_per_cu waves per
+// CU, never more lanes than units (rounded up to whole workgroups; n is only
+// an upper bound when the count comes from the device).
+int traversal_grid(const voxg_launch_config &cfg, size_t n, int devid) {
+    const int cus = cu_count(devid);
+    const int block = cfg.block;
+    const int waves = std::max(1, cfg.waves_per_cu);
+    const long long threads = (long long)cus * waves * 64;
     int grid = (int)std::max<long long>(1, threads / block);
-    // never launch more lanes than units need (rounded up to whole blocks;
-    // n is only an upper bound when the count comes from the device)
     long long need_blocks = ((long long)n + block - 1) / block;
     if (need_blocks < 1) need_blocks = 1;
     if (grid > need_blocks) grid = (int)need_blocks;
+    return grid;
+}
+
+struct NoPre {
+    int operator()(Workspace *) const { return 0; }
+};
+
+// One traversal launch.  Everything that can fail on the host is reserved
+// first (workspace, spill buffer, LDS attribute); `pre` then enqueues what
+// must run before the traversal on the stream (the render pre-pass, which
+// may also fill in the work's queue), and the launch follows.
+template <int K, class Work, bool COUNT, class Pre>
+int launch_k(const dev::Tree &dt, uint32_t stack_bound, Work &work, size_t n,
+             hipStream_t stream, const voxg_launch_config &cfg,
+             voxg_counts *counts, bool reset_counts, Workspace *own,
+             Pre &&pre) {
+    int devid = 0;
+    HIPCHK(hipGetDevice(&devid));
+    const int block = cfg.block;
+    const int grid = traversal_grid(cfg, n, devid);
     uint32_t gthreads = (uint32_t)grid * (uint32_t)block;
+    if (const uint32_t tb = g_test_bound.load()) stack_bound = std::min(stack_bound, tb);
     uint32_t slots = stack_bound > (uint32_t)K ? stack_bound - K : 0u;
-    Workspace *ws = nullptr;
-    int rc = get_ws(stream, (size_t)std::max<uint32_t>(slots, 1u) * gthreads *
-                                sizeof(uint4),
-                    &ws);
+    const size_t spill_need =
+        (size_t)std::max<uint32_t>(slots, 1u) * gthreads * sizeof(uint4);
+    // the stream's workspace, or one the caller owns (sized in advance)
+    Workspace *ws = own;
+    int rc = 0;
+    if (!ws) {
+        rc = get_ws(stream, spill_need, &ws);
+        if (rc) return rc;
+    } else if (spill_need > ws->spill_bytes) {
+        return fail(VOXG_EINVAL, "workspace too small for this launch");
+    }
+    rc = take_overflow(ws);
     if (rc) return rc;
     size_t lds = (size_t)K * block * sizeof(uint4);
     auto kern = dev::rti_kernel<K, Work, COUNT>;
@@ -1841,19 +1970,25 @@ int launch_k(const dev::Tree &dt, uint32_t stack_bound, const Work &work,
         HIPCHK(hipMemsetAsync(&ws->ctrl->visits, 0,
                               dev::kCounterBytes, stream));
     }
+    rc = pre(ws);
+    if (rc) return rc;
+    if (injected_failure()) return fail(VOXG_EIO, "injected failure");
     hipLaunchKernelGGL(kern, dim3(grid), dim3(block), lds, stream, dt, work,
-                       (uint32_t)n, ws->ctrl, ws->xnext, ws->spill,
-                       slots,
+                       (uint32_t)n, ws->ctrl, ws->ovf_d, ws->xnext,
+                       ws->spill, slots,
                        (uint32_t)cfg.refill, (uint32_t)cfg.finish_batch,
                        (uint32_t)cfg.leaf_batch);
     HIPCHK(hipGetLastError());
+    // the traversal's last wave clears the queue counts unless a later
+    // pass still needs them
+    if (!work.keeps_queue()) ws->dirty = false;
     if (COUNT) {
         dev::Ctrl h;
         HIPCHK(hipMemcpyAsync(&h, ws->ctrl, sizeof(h), hipMemcpyDeviceToHost,
                               stream));
         HIPCHK(hipStreamSynchronize(stream));
-        if (h.overflow)
-            return fail(VOXG_EIO, "traversal stack bound exceeded");
+        rc = take_overflow(ws);
+        if (rc) return rc;
         counts->node_visits = h.visits;
         counts->voxel_tests = h.tests;
         counts->rays = h.rays;
@@ -1866,24 +2001,32 @@ int launch_k(const dev::Tree &dt, uint32_t stack_bound, const Work &work,
     return 0;
 }
 
-template <class Work, bool COUNT>
-int launch_tree(int kind, const dev::Tree &t, uint32_t stack_bound,
-                const Work &work, size_t n, hipStream_t stream,
-                voxg_counts *counts,
-                bool reset_counts = true) {
-    voxg_launch_config cfg = current_config(kind);
+template <class Work, bool COUNT, class Pre = NoPre>
+int launch_cfg(const voxg_launch_config &cfg, const dev::Tree &t,
+               uint32_t stack_bound, Work &work, size_t n,
+               hipStream_t stream, voxg_counts *counts,
+               bool reset_counts = true, Workspace *own = nullptr,
+               Pre &&pre = Pre()) {
     switch (cfg.stack_lds) {
-    case 2: return launch_k<2, Work, COUNT>(t, stack_bound, work, n, stream, cfg, counts, reset_counts);
-    case 4: return launch_k<4, Work, COUNT>(t, stack_bound, work, n, stream, cfg, counts, reset_counts);
-    case 8: return launch_k<8, Work, COUNT>(t, stack_bound, work, n, stream, cfg, counts, reset_counts);
-    case 16: return launch_k<16, Work, COUNT>(t, stack_bound, work, n, stream, cfg, counts, reset_counts);
-    case 32: return launch_k<32, Work, COUNT>(t, stack_bound, work, n, stream, cfg, counts, reset_counts);
+    case 2: return launch_k<2, Work, COUNT>(t, stack_bound, work, n, stream, cfg, counts, reset_counts, own, pre);
+    case 4: return launch_k<4, Work, COUNT>(t, stack_bound, work, n, stream, cfg, counts, reset_counts, own, pre);
+    case 8: return launch_k<8, Work, COUNT>(t, stack_bound, work, n, stream, cfg, counts, reset_counts, own, pre);
+    case 16: return launch_k<16, Work, COUNT>(t, stack_bound, work, n, stream, cfg, counts, reset_counts, own, pre);
+    case 32: return launch_k<32, Work, COUNT>(t, stack_bound, work, n, stream, cfg, counts, reset_counts, own, pre);
     default: return fail(VOXG_EINVAL, "stack_lds must be 2, 4, 8, 16 or 32");
     }
 }
 
 template <class Work, bool COUNT>
-int launch(int kind, const voxg_tree *t, const Work &work, size_t n,
+int launch_tree(int kind, const dev::Tree &t, uint32_t stack_bound,
+                Work &work, size_t n, hipStream_t stream,
+                voxg_counts *counts, bool reset_counts = true) {
+    return launch_cfg<Work, COUNT>(current_config(kind), t, stack_bound, work,
+                                   n, stream, counts, reset_counts);
+}
+
+template <class Work, bool COUNT>
+int launch(int kind, const voxg_tree *t, Work &work, size_t n,
            hipStream_t stream, voxg_counts *counts) {
     return launch_tree<Work, COUNT>(kind, dev_tree(t), t->stack_bound, work,
                                     n, stream, counts);
@@ -1931,6 +2074,35 @@ int finish_tree(voxg_tree *t, voxg_tree **out) {
 extern "C" {
 
 const char *voxg_last_error(void) { return g_err.c_str(); }
+
+int voxg_synchronize(hipStream_t stream) {
+    HIPCHK(hipStreamSynchronize(stream));
+    int devid = 0;
+    HIPCHK(hipGetDevice(&devid));
+    Workspace *ws = nullptr;
+    {
+        std::lock_guard<std::mutex> g(g_mu);
+        auto it = g_ws.find({devid, stream});
+        if (it != g_ws.end()) ws = it->second.get();
+    }
+    return ws ? take_overflow(ws) : 0;
+}
+
+int voxg_release_stream(hipStream_t stream) {
+    voxg_internal::release_stream(stream);
+    return 0;
+}
+
+int voxg_test_fail_at(int checkpoint) {
+    if (checkpoint < 0) return fail(VOXG_EINVAL, "checkpoint must be >= 0");
+    g_inject = checkpoint;
+    return 0;
+}
+
+int voxg_test_stack_bound(unsigned bound) {
+    g_test_bound.store(bound);
+    return 0;
+}
 
 int voxg_init(int device) {
     int n = 0;
@@ -2345,35 +2517,45 @@ static dev::PixSink make_sink(const voxg_tree *t,
 
 // One render: the block pre-pass (when enabled) and the traversal over the
 // blocks it queued, with units of `Work` (BlockWork<false> or BestWork).
+// Everything that can fail on the host is reserved before the pre-pass is
+// enqueued; a render that fails after its pre-pass leaves the stream's
+// workspace dirty, and the next render clears the queue counts first.
 template <class Work, bool COUNT>
-static int render_run(const voxg_tree *t, Work &work, int kind,
+static int render_run(const voxg_tree *t, Work &work,
                       const voxg_launch_config &cfg, size_t npix,
                       voxg_counts *counts, hipStream_t stream) {
     const size_t nblk = npix / 16;
     const size_t units = nblk * work.units_per_block_host();
     if (!cfg.render_cull || t->host.nodes.empty())
-        return launch<Work, COUNT>(kind, t, work, units, stream, counts);
+        return launch_cfg<Work, COUNT>(cfg, dev_tree(t), t->stack_bound, work,
+                                       units, stream, counts);
     // pre-pass: finish the blocks whose rays all miss the root box and
     // queue the others for the traversal kernel
     Workspace *ws = nullptr;
     int rc = get_ws(stream, 16, &ws);
     if (rc) return rc;
-    rc = get_queue(ws, nblk * sizeof(uint32_t));
+    rc = get_queue(ws, stream, nblk * sizeof(uint32_t));
     if (rc) return rc;
-    if (COUNT)
-        HIPCHK(hipMemsetAsync(&ws->ctrl->visits, 0,
-                              dev::kCounterBytes, stream));
-    hipLaunchKernelGGL((dev::block_cull_kernel<COUNT>),
-                       dim3((npix + dev::kCullTile - 1) / dev::kCullTile),
-                       dim3(256), 0, stream,
-                       work.one, (uint32_t)npix, ws->ctrl, ws->queue,
-                       (uint32_t)nblk);
-    HIPCHK(hipGetLastError());
-    work.queue = ws->queue;
-    work.qpair = &ws->ctrl->qpair;
-    work.qlen = (uint32_t)nblk;
-    return launch_tree<Work, COUNT>(kind, dev_tree(t), t->stack_bound, work,
-                                    units, stream, counts, false);
+    rc = clear_dirty_queue(ws, stream);
+    if (rc) return rc;
+    auto pre = [&](Workspace *w) -> int {
+        if (COUNT)
+            HIPCHK(hipMemsetAsync(&w->ctrl->visits, 0, dev::kCounterBytes,
+                                  stream));
+        w->dirty = true;
+        hipLaunchKernelGGL((dev::block_cull_kernel<COUNT>),
+                           dim3((npix + dev::kCullTile - 1) / dev::kCullTile),
+                           dim3(256), 0, stream, work.one, (uint32_t)npix,
+                           w->ctrl, w->queue, (uint32_t)nblk);
+        HIPCHK(hipGetLastError());
+        work.queue = w->queue;
+        work.qpair = &w->ctrl->qpair;
+        work.qlen = (uint32_t)nblk;
+        return 0;
+    };
+    return launch_cfg<Work, COUNT>(cfg, dev_tree(t), t->stack_bound, work,
+                                   units, stream, counts, false, nullptr,
+                                   pre);
 }
 
 // FAST and ADAPTIVE renders, after the pre-pass: every pixel of the
@@ -2385,7 +2567,8 @@ static int render_run(const voxg_tree *t, Work &work, int kind,
 // (skip_odd) the first pass traces 9 of a block's 16 pixels, the ones a
 // block that keeps merging needs; the blocks whose merging stops get their
 // other 7 traced and are replayed by two more launches before the output
-// pass.
+// pass.  emit_kernel clears the queue counts; until it is enqueued the
+// stream's workspace stays dirty.
 static int render_speculative(const voxg_tree *t, const dev::RenderItem &one,
                               size_t npix, bool skip_odd, hipStream_t stream) {
     voxg_launch_config cfg = current_config(VOXG_LAUNCH_RENDER_BEST);
@@ -2418,14 +2601,14 @@ static int render_speculative(const voxg_tree *t, const dev::RenderItem &one,
         work.pset = 1;
         work.pdiv = make_fastdiv(9);
         work.group = 256u * 9u;
-        rc = render_run<dev::BestSetWork, false>(
-            t, work, VOXG_LAUNCH_RENDER_BEST, cfg, npix, nullptr, stream);
+        rc = render_run<dev::BestSetWork, false>(t, work, cfg, npix, nullptr,
+                                                 stream);
     } else {
         dev::BestSpecWork work;
         first_pass(work);
         work.group = 256u << work.split;
-        rc = render_run<dev::BestSpecWork, false>(
-            t, work, VOXG_LAUNCH_RENDER_BEST, cfg, npix, nullptr, stream);
+        rc = render_run<dev::BestSpecWork, false>(t, work, cfg, npix, nullptr,
+                                                  stream);
     }
     if (rc) return rc;
     Workspace *ws = nullptr;
@@ -2435,12 +2618,14 @@ static int render_speculative(const voxg_tree *t, const dev::RenderItem &one,
     HIPCHK(hipGetDevice(&devid));
     const size_t grid = std::max<size_t>(
         1, std::min<size_t>((nblk + 255) / 256, (size_t)cu_count(devid) * 8));
+    if (injected_failure()) return fail(VOXG_EIO, "injected failure");
     hipLaunchKernelGGL(dev::resolve_kernel, dim3((unsigned)grid), dim3(256),
                        0, stream, one, spec, ws->queue, (uint32_t)nblk,
                        ws->ctrl, redo, 0);
     HIPCHK(hipGetLastError());
     if (skip_odd) {
-        // the odd pixels of the listed blocks, then their replay
+        // the odd pixels of the listed blocks, then their replay (the
+        // first pass reserved a spill buffer for at least as many lanes)
         dev::BestSetWork odd;
         first_pass(odd);
         odd.pset = 2;
@@ -2449,19 +2634,21 @@ static int render_speculative(const voxg_tree *t, const dev::RenderItem &one,
         odd.queue = redo;
         odd.qpair = &ws->ctrl->rpair;
         odd.qlen = (uint32_t)nblk;
-        rc = launch_tree<dev::BestSetWork, false>(
-            VOXG_LAUNCH_RENDER_BEST, dev_tree(t), t->stack_bound, odd,
-            nblk * 7, stream, nullptr, false);
+        rc = launch_cfg<dev::BestSetWork, false>(
+            cfg, dev_tree(t), t->stack_bound, odd, nblk * 7, stream, nullptr,
+            false);
         if (rc) return rc;
         hipLaunchKernelGGL(dev::resolve_kernel, dim3((unsigned)grid),
                            dim3(256), 0, stream, one, spec, ws->queue,
                            (uint32_t)nblk, ws->ctrl, redo, 1);
         HIPCHK(hipGetLastError());
     }
+    if (injected_failure()) return fail(VOXG_EIO, "injected failure");
     hipLaunchKernelGGL(dev::emit_kernel, dim3((unsigned)grid), dim3(256), 0,
                        stream, one, spec, ws->queue, (uint32_t)nblk,
                        ws->ctrl);
     HIPCHK(hipGetLastError());
+    ws->dirty = false;
     return 0;
 }
 
@@ -2497,8 +2684,8 @@ static int render_launch(const voxg_tree *t, size_t n_frames,
         // longest-first order of the block queue on every XCD, and spread
         // the refill atomics over the 8 per-XCD counters
         work.group = 256u << work.split;
-        return render_run<dev::BestWork, COUNT>(
-            t, work, VOXG_LAUNCH_RENDER_BEST, cfg, npix, counts, stream);
+        return render_run<dev::BestWork, COUNT>(t, work, cfg, npix, counts,
+                                                stream);
     }
     const voxg_launch_config cfg = current_config(VOXG_LAUNCH_RENDER);
     // speculated passes (render_speculative) except for the counting
@@ -2514,8 +2701,8 @@ static int render_launch(const voxg_tree *t, size_t n_frames,
     std::memset(&work, 0, sizeof(work));
     work.one = one;
     work.group = 256u;
-    return render_run<dev::BlockWork<false>, COUNT>(
-        t, work, VOXG_LAUNCH_RENDER, cfg, npix, counts, stream);
+    return render_run<dev::BlockWork<false>, COUNT>(t, work, cfg, npix,
+                                                    counts, stream);
 }
 
 }  // extern "C++"
@@ -2593,22 +2780,46 @@ int voxg_batch_create(size_t n_items, const voxg_batch_item *items,
     b->chunk = chunk;
     b->stack_bound = bound;
     b->rays_end = rays_end;
-    if (!h.empty()) {
-        HIPCHK(hipMalloc(&b->d_items, h.size() * sizeof(dev::BatchItem)));
-        hipError_t e = hipMemcpy(b->d_items, h.data(),
-                                 h.size() * sizeof(dev::BatchItem),
-                                 hipMemcpyHostToDevice);
-        if (e != hipSuccess) {
-            (void)hipFree(b->d_items);
-            return fail(VOXG_EIO, hipGetErrorString(e));
-        }
+    b->cfg = current_config(VOXG_LAUNCH_BATCH);
+    std::unique_ptr<Workspace> ws(new (std::nothrow) Workspace());
+    if (!ws) return fail(VOXG_ENOMEM, "host allocation failed");
+    int rc = init_workspace(ws.get());
+    if (rc) return rc;
+    // spill buffer for the launch's lanes at this configuration (the
+    // largest stack_lds ring is the configured one)
+    const uint32_t gthreads =
+        (uint32_t)traversal_grid(b->cfg, b->units, devid) *
+        (uint32_t)b->cfg.block;
+    const uint32_t k = (uint32_t)b->cfg.stack_lds;
+    const uint32_t slots = bound > k ? bound - k : 1u;
+    const size_t sb = (size_t)slots * gthreads * sizeof(uint4);
+    hipError_t e = hipMalloc(&ws->spill, sb);
+    if (e == hipSuccess) {
+        ws->spill_bytes = sb;
+        if (!h.empty())
+            e = hipMalloc(&b->d_items, h.size() * sizeof(dev::BatchItem));
     }
+    if (e == hipSuccess && !h.empty())
+        e = hipMemcpy(b->d_items, h.data(), h.size() * sizeof(dev::BatchItem),
+                      hipMemcpyHostToDevice);
+    if (e != hipSuccess) {
+        free_workspace(ws.get());
+        if (b->d_items) (void)hipFree(b->d_items);
+        return fail(VOXG_EIO, hipGetErrorString(e));
+    }
+    b->ws = ws.release();
     *out = b.release();
     return 0;
 }
 
 int voxg_batch_release(voxg_batch *b) {
     if (!b) return 0;
+    if (b->ws) {
+        // a launch of the batch may still be running on some stream
+        HIPCHK(hipDeviceSynchronize());
+        free_workspace(b->ws);
+        delete b->ws;
+    }
     if (b->d_items) HIPCHK(hipFree(b->d_items));
     delete b;
     return 0;
@@ -2634,9 +2845,9 @@ int voxg_batch_ray(const voxg_batch *b, const float *ox, const float *oy,
         b->n_items, b->chunk};
     dev::Tree none;
     std::memset(&none, 0, sizeof(none));
-    return launch_tree<decltype(work), false>(VOXG_LAUNCH_BATCH, none,
-                                              b->stack_bound, work, b->units,
-                                              stream, nullptr);
+    return launch_cfg<decltype(work), false>(b->cfg, none, b->stack_bound,
+                                             work, b->units, stream, nullptr,
+                                             true, b->ws);
 }
 
 }  // extern "C"
@@ -2717,6 +2928,7 @@ struct SrvRequest {
 struct SrvGroup {
     uint64_t id = 0;
     int stream = 0;
+    uint32_t stack_bound = 1;  // of the deepest tree, read at admission
     hipEvent_t done = nullptr;
     std::vector<SrvRequest *> reqs;
 };
@@ -2771,6 +2983,18 @@ size_t render_bytes(const voxg_render_request &r, bool color) {
     if (color && r.color_out) return np * 20 + 256;
     if (r.hit_out) return np * 16 + 256;
     return np * 4 + 256;
+}
+
+// Whether a traversal on server stream i flagged a stack overflow (read
+// and cleared).
+bool srv_stream_overflow(voxg_server *s, int i) {
+    std::lock_guard<std::mutex> g(g_mu);
+    auto it = g_ws.find({s->device, s->streams[i].s});
+    if (it == g_ws.end() || !it->second->ovf_h) return false;
+    volatile uint32_t *f = it->second->ovf_h;
+    if (!*f) return false;
+    *f = 0u;
+    return true;
 }
 
 // Launch one group on its stream (scheduler thread, no lock held).
@@ -2867,8 +3091,10 @@ int srv_launch(voxg_server *s, SrvGroup &g) {
         work.n_items = (uint32_t)ritems.size();
         dev::Tree none;
         std::memset(&none, 0, sizeof(none));
+        // (launch_k grows the stream's spill buffer here, on the
+        // scheduler thread, the only thread that launches on it)
         int rc = launch_tree<dev::BlockWork<true>, false>(
-            VOXG_LAUNCH_BATCH, none, s->stack_bound, work, units, ss.s,
+            VOXG_LAUNCH_BATCH, none, g.stack_bound, work, units, ss.s,
             nullptr);
         if (rc) return rc;
     }
@@ -2914,11 +3140,24 @@ void srv_loop(voxg_server *s) {
         // retire finished groups
         for (size_t i = 0; i < s->groups.size();) {
             SrvGroup &g = s->groups[i];
-            if (hipEventQuery(g.done) == hipSuccess) {
+            const hipError_t q = hipEventQuery(g.done);
+            if (q != hipErrorNotReady) {
                 double t = s->now();
+                int status = 0;
+                if (q != hipSuccess) {
+                    // the group's work failed: make sure nothing of it still
+                    // runs before its memory is handed out again
+                    (void)hipStreamSynchronize(s->streams[g.stream].s);
+                    status = VOXG_EIO;
+                    s->thread_error = std::string("group failed: ") +
+                                      hipGetErrorString(q);
+                } else if (srv_stream_overflow(s, g.stream)) {
+                    status = VOXG_EIO;
+                    s->thread_error = "a traversal exceeded its stack bound";
+                }
                 for (SrvRequest *r : g.reqs) {
                     r->t_done = t;
-                    r->status = 0;
+                    r->status = status;
                     s->arena.release(r->off, r->bytes);
                     s->inflight--;
                     s->st.completed++;
@@ -2946,6 +3185,7 @@ void srv_loop(voxg_server *s) {
         }
         if (!g.reqs.empty()) {
             g.id = s->next_group++;
+            g.stack_bound = s->stack_bound;
             g.stream = s->next_stream;
             s->next_stream = (s->next_stream + 1) % (int)s->streams.size();
             for (SrvRequest *r : g.reqs) {
@@ -2963,6 +3203,11 @@ void srv_loop(voxg_server *s) {
                       hipSuccess)
                          ? srv_launch(s, g)
                          : VOXG_EIO;
+            if (rc) {
+                // part of the group may be queued already (copies,
+                // kernels): let it drain before its memory is reused
+                (void)hipStreamSynchronize(s->streams[g.stream].s);
+            }
             lk.lock();
             if (rc) {
                 s->thread_error = g_err;
@@ -3038,20 +3283,6 @@ int voxg_server_create(const voxg_server_config *cfg, voxg_server **out) {
     return 0;
 }
 
-// Stream spill buffers sized for trees of stack bound `bound`.
-static int srv_fit_trees(voxg_server *s, uint32_t bound) {
-    voxg_launch_config c = current_config(VOXG_LAUNCH_BATCH);
-    uint32_t slots = bound > (uint32_t)c.stack_lds ? bound - c.stack_lds : 1;
-    int cus = cu_count(s->device);
-    size_t threads = (size_t)cus * std::max(1, c.waves_per_cu) * 64;
-    for (auto &ss : s->streams) {
-        Workspace *ws = nullptr;
-        int rc = get_ws(ss.s, (size_t)slots * threads * sizeof(uint4), &ws);
-        if (rc) return rc;
-    }
-    return 0;
-}
-
 // Requests submitted from now on use tree t under this id; requests
 // submitted before keep the version they were submitted with.  The
 // streams' spill buffers are not touched here (the scheduler thread may be
@@ -3068,16 +3299,14 @@ int voxg_server_replace_tree(voxg_server *s, int id, const voxg_tree *t) {
     return 0;
 }
 
+// The streams' spill buffers are grown by the scheduler thread when it
+// launches a group of a deeper tree, never from the caller's thread.
 int voxg_server_add_tree(voxg_server *s, const voxg_tree *t, int *id) {
     if (!s || !id) return fail(VOXG_EINVAL, "NULL argument");
     int rc = check_tree(t);
     if (rc) return rc;
-    // size every stream's spill buffer for the deepest tree
-    uint32_t bound = std::max(s->stack_bound, t->stack_bound);
-    rc = srv_fit_trees(s, bound);
-    if (rc) return rc;
     std::lock_guard<std::mutex> g(s->mu);
-    s->stack_bound = bound;
+    s->stack_bound = std::max(s->stack_bound, t->stack_bound);
     s->trees.push_back(dev_tree(t));
     *id = (int)s->trees.size() - 1;
     return 0;
@@ -3207,19 +3436,7 @@ int voxg_server_destroy(voxg_server *s) {
         if (ss.h_table) (void)hipHostFree(ss.h_table);
         if (ss.d_table) (void)hipFree(ss.d_table);
         if (ss.table_free) (void)hipEventDestroy(ss.table_free);
-        if (ss.s) {
-            std::lock_guard<std::mutex> g(g_mu);
-            auto it = g_ws.find({s->device, ss.s});
-            if (it != g_ws.end()) {
-                Workspace *w = it->second.get();
-                if (w->ctrl) (void)hipFree(w->ctrl);
-                if (w->xnext) (void)hipFree(w->xnext);
-                if (w->spill) (void)hipFree(w->spill);
-                if (w->queue) (void)hipFree(w->queue);
-                if (w->aux) (void)hipFree(w->aux);
-                g_ws.erase(it);
-            }
-        }
+        if (ss.s) voxg_internal::release_stream(ss.s);
     }
     if (s->d_arena) (void)hipFree(s->d_arena);
     if (s->d_texture) (void)hipFree(s->d_texture);
