@@ -150,6 +150,23 @@ int voxg_ray_batch(const voxg_tree *tree, size_t n, const float *ox,
                    uint32_t *leaf_out, float *hx, float *hy, float *hz,
                    hipStream_t stream);
 
+/* voxg_ray_batch for callers that cannot allocate device memory (code under
+ * HIP graph capture, tight serving loops, harnesses that forbid hipMalloc).
+ * No counterpart in the reference.  Same arguments and results as
+ * voxg_ray_batch, bit for bit; the launch's control record, counters and
+ * stack spill buffer live in a 32 MiB array that each GPU holds for the
+ * life of the process, and no allocating call is made.  The launch runs
+ * the VOXG_LAUNCH_RAYS configuration with fewer waves per CU when the
+ * array cannot hold the spill entries of all of them; a tree too deep for
+ * one wave per CU is rejected with VOXG_ERANGE.  Every call on a GPU shares
+ * that GPU's array: calls that overlap on one GPU (other streams, other
+ * threads) race and must be serialised by the caller. */
+int voxg_ray_batch_noalloc(const voxg_tree *tree, size_t n, const float *ox,
+                           const float *oy, const float *oz, const float *dx,
+                           const float *dy, const float *dz,
+                           uint32_t *voxel_out, uint32_t *leaf_out, float *hx,
+                           float *hy, float *hz, hipStream_t stream);
+
 /* Batched vox_tree_ball_collidep (search.c:155-183): out[i] = 1 when the
  * ball (c_i, r_i) touches a voxel (squared distance strictly below r^2). */
 int voxg_ball_batch(const voxg_tree *tree, size_t n, const float *cx,

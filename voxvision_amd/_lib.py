@@ -35,7 +35,7 @@ EXPORTS = [
     "voxg_edit_rebuild", "voxg_edit_root", "voxg_edit_voxels",
     "voxg_edit_commit", "voxg_edit_release", "voxg_server_replace_tree",
     "voxg_synchronize", "voxg_release_stream", "voxg_test_fail_at",
-    "voxg_test_stack_bound",
+    "voxg_test_stack_bound", "voxg_ray_batch_noalloc",
 ]
 
 
@@ -116,6 +116,7 @@ def load():
     L.voxg_tree_node_ptr.argtypes = [P, C.c_uint32]
     L.voxg_tree_node_ptr.restype = P
     L.voxg_ray_batch.argtypes = [P, S] + [P] * 6 + [P] * 5 + [P]
+    L.voxg_ray_batch_noalloc.argtypes = [P, S] + [P] * 6 + [P] * 5 + [P]
     L.voxg_ray_batch_count.argtypes = [P, S] + [P] * 6 + [
         C.POINTER(Counts), P]
     L.voxg_ball_batch.argtypes = [P, S, P, P, P, P, P, P]

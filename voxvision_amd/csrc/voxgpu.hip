@@ -1638,6 +1638,30 @@ collide_kernel(Tree t, uint32_t n, float *__restrict__ px,
     if (collided) collided[i] = moved_back;
 }
 
+// Scratch of voxg_ray_batch_noalloc: the control record, the per-XCD unit
+// counters, the overflow word and the stack spill buffer of one launch, in
+// a file-scope array that each GPU holds for the life of the process.  Host
+// code reaches it only through the address hipGetSymbolAddress returns on
+// each device (the array's name is not a device address in host code).
+constexpr size_t kNoAllocBytes = 32u << 20;
+constexpr size_t kNoAllocCtrl = 0;       // Ctrl
+constexpr size_t kNoAllocXnext = 256;    // kXcds counters, 256 B apart
+constexpr size_t kNoAllocOvf = 2304;     // overflow word
+constexpr size_t kNoAllocSpill = 4096;   // spill entries to the end
+static_assert(sizeof(Ctrl) <= kNoAllocXnext, "Ctrl fits its slot");
+static_assert(kNoAllocXnext + kXcds * kXcdStride * 4 <= kNoAllocOvf,
+              "counters fit their slot");
+__device__ __align__(256) uint8_t scratch[kNoAllocBytes];
+
+// Clears what a launch expects at zero: the control record, the counters
+// and the overflow word (a kernel rather than a memset, so that the call can
+// be captured in a graph).
+__global__ void __launch_bounds__(256) noalloc_clear_kernel(uint8_t *base) {
+    uint32_t *w = reinterpret_cast<uint32_t *>(base);
+    for (uint32_t i = threadIdx.x; i < kNoAllocSpill / 4; i += blockDim.x)
+        w[i] = 0u;
+}
+
 }  // namespace dev
 
 // ---------------------------------------------------------------------------
@@ -2071,6 +2095,21 @@ int finish_tree(voxg_tree *t, voxg_tree **out) {
 
 }  // namespace
 
+// Device address of dev::scratch on each device (hipGetSymbolAddress once
+// per device, with that device current); 0 until set up.
+std::mutex g_noalloc_mu;
+std::map<int, uint8_t *> g_noalloc;
+
+static int noalloc_setup(int device) {
+    std::lock_guard<std::mutex> g(g_noalloc_mu);
+    if (g_noalloc.count(device)) return 0;
+    void *ptr = nullptr;
+    HIPCHK(hipGetSymbolAddress(&ptr, HIP_SYMBOL(dev::scratch)));
+    if (!ptr) return fail(VOXG_EIO, "no address for the scratch array");
+    g_noalloc[device] = (uint8_t *)ptr;
+    return 0;
+}
+
 extern "C" {
 
 const char *voxg_last_error(void) { return g_err.c_str(); }
@@ -2116,7 +2155,7 @@ int voxg_init(int device) {
         return fail(VOXG_ENODEV, std::string("device is ") + p.gcnArchName +
                                      ", this library is built for gfx950");
     g_device = device;
-    return 0;
+    return noalloc_setup(device);
 }
 
 int voxg_tree_build(const float *xyz, size_t n, size_t stride,
@@ -2376,6 +2415,68 @@ int voxg_ray_batch_count(const voxg_tree *t, size_t n, const float *ox,
         (uint32_t)n, c};
     return launch<decltype(work), true>(VOXG_LAUNCH_RAYS, t, work,
                                         (n + c - 1) / c, stream, out);
+}
+
+int voxg_ray_batch_noalloc(const voxg_tree *t, size_t n, const float *ox,
+                           const float *oy, const float *oz, const float *dx,
+                           const float *dy, const float *dz,
+                           uint32_t *voxel_out, uint32_t *leaf_out, float *hx,
+                           float *hy, float *hz, hipStream_t stream) {
+    int rc = ray_common(t, n, ox, oy, oz, dx, dy, dz);
+    if (rc) return rc;
+    if ((hx || hy || hz) && !(hx && hy && hz))
+        return fail(VOXG_EINVAL, "hx, hy, hz must be all set or all NULL");
+    if (n == 0) return 0;
+    int devid = 0;
+    HIPCHK(hipGetDevice(&devid));
+    rc = noalloc_setup(devid);
+    if (rc) return rc;
+    uint8_t *base = nullptr;
+    {
+        std::lock_guard<std::mutex> g(g_noalloc_mu);
+        base = g_noalloc[devid];
+    }
+    voxg_launch_config cfg = current_config(VOXG_LAUNCH_RAYS);
+    uint32_t bound = t->stack_bound;
+    if (const uint32_t tb = g_test_bound.load()) bound = std::min(bound, tb);
+    const uint32_t slots =
+        std::max<uint32_t>(1u, bound > (uint32_t)cfg.stack_lds
+                                   ? bound - (uint32_t)cfg.stack_lds
+                                   : 0u);
+    // as many persistent lanes as the scratch holds spill entries for: the
+    // configured waves per CU when they fit, fewer otherwise
+    const size_t entries = (dev::kNoAllocBytes - dev::kNoAllocSpill) /
+                           sizeof(uint4);
+    const size_t lanes_fit = entries / slots;
+    const size_t wave_all = (size_t)cu_count(devid) * 64;  // a wave per CU
+    if (lanes_fit < wave_all)
+        return fail(VOXG_ERANGE,
+                    "tree too deep for the no-allocation scratch (" +
+                        std::to_string(dev::kNoAllocBytes >> 20) +
+                        " MiB holds " + std::to_string(lanes_fit) +
+                        " lanes' spill entries, fewer than one wave per CU)");
+    cfg.waves_per_cu = (int)std::min<size_t>(
+        (size_t)std::max(1, cfg.waves_per_cu), lanes_fit / wave_all);
+    uint32_t c = effective_chunk(n, VOXG_LAUNCH_RAYS);
+    dev::ChunkWork<dev::SoaSrc, dev::SoaSink> work{
+        dev::SoaSrc{ox, oy, oz, dx, dy, dz},
+        dev::SoaSink{voxel_out, leaf_out, hx, hy, hz}, (uint32_t)n, c};
+    // a workspace that points into the scratch array; nothing allocated
+    Workspace ws;
+    ws.ctrl = (dev::Ctrl *)(base + dev::kNoAllocCtrl);
+    ws.xnext = (uint32_t *)(base + dev::kNoAllocXnext);
+    ws.ovf_d = (uint32_t *)(base + dev::kNoAllocOvf);
+    ws.spill = (uint4 *)(base + dev::kNoAllocSpill);
+    ws.spill_bytes = entries * sizeof(uint4);
+    hipLaunchKernelGGL(dev::noalloc_clear_kernel, dim3(1), dim3(256), 0,
+                       stream, base);
+    HIPCHK(hipGetLastError());
+    dev::Tree dt = dev_tree(t);
+    rc = launch_cfg<decltype(work), false>(cfg, dt, bound, work,
+                                           (n + c - 1) / c, stream, nullptr,
+                                           true, &ws);
+    // the Workspace destructor frees nothing: its pointers are the array's
+    return rc;
 }
 
 static int ball_launch(const voxg_tree *t, size_t n, const float *cx,
