@@ -87,7 +87,8 @@ typedef struct voxg_tree voxg_tree;
 typedef struct {
     uint64_t nodes, inner, leaves, dense, overflow, dots, voxel_ids;
     uint32_t depth;          /* levels, root = 1 */
-    uint32_t stack_entries;  /* traversal stack bound: 3*(depth-1)+1 */
+    uint32_t stack_entries;  /* traversal stack bound: one entry per
+                                inner level, depth-1 (at least 1) */
     uint64_t device_bytes;   /* flattened tree size in HBM */
     float bbox_min[3], bbox_max[3];
     float voxel[3];          /* voxel size snapshot taken at build/upload */
@@ -426,8 +427,9 @@ int voxg_read_raw_data(const char *filename, const unsigned dim[3],
  *                            default, handed out to large groups of lanes)
  *   VOXG_LAUNCH_BATCH        voxg_batch_ray and the serving loop's launches
  * Fields:
- *   stack_lds     stack entries per lane kept in LDS (power of two 2..32;
- *                 deeper stacks spill to HBM and stay exact)
+ *   stack_lds     stack entries per lane kept in LDS (power of two 2..32,
+ *                 20 bytes each; at most 160 KiB per workgroup); deeper
+ *                 stacks spill to HBM and stay exact
  *   block         threads per workgroup (64, 128, 192 or 256)
  *   waves_per_cu  persistent waves launched per CU (1..64)
  *   refill        minimum idle lanes before a wave fetches new work (1..64)
@@ -480,8 +482,8 @@ const char *voxg_last_error(void);
 
 /* Waits for the work queued on the stream, then reports VOXG_EIO if a
  * traversal launched on it since the last check exceeded its stack bound
- * (3 pending entries per inner level, which the search order never
- * exceeds; the affected rays would be left as misses).  The flag is also
+ * (one entry per inner level, which the search order never exceeds; the
+ * affected rays would be left as misses).  The flag is also
  * checked, without waiting, at the start of every later traversal launch
  * on the stream and by the counting calls.  Replaces the host-side wait a
  * caller of the reference's synchronous searches does not need

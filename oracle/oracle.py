@@ -105,6 +105,19 @@ def lib():
                                  C.POINTER(C.c_uint32),
                                  C.POINTER(C.c_uint32), F, C.c_int,
                                  C.POINTER(_RenderStats)]
+        L.orc_render_trace.argtypes = [P, C.POINTER(_Camera), C.c_int,
+                                       C.c_int, C.c_uint,
+                                       C.POINTER(_Shading),
+                                       C.POINTER(C.c_uint32),
+                                       C.POINTER(C.c_uint32), F, C.c_int,
+                                       C.POINTER(_RenderStats),
+                                       C.POINTER(C.c_uint8),
+                                       C.POINTER(C.c_uint8), F]
+        L.orc_ray_max_pending.argtypes = [P, C.c_size_t] + [F] * 6 + [
+            C.POINTER(C.c_uint32)]
+        L.orc_tree_inner_nodes.argtypes = [P, F, C.POINTER(C.c_uint64),
+                                           C.c_size_t]
+        L.orc_tree_inner_nodes.restype = C.c_size_t
         L.orc_brute64_rays.argtypes = [F, C.c_size_t, F, C.c_size_t] + [
             F] * 6 + [C.c_double, C.POINTER(C.c_int32),
                       C.POINTER(C.c_double), C.POINTER(C.c_uint8), C.c_int]
@@ -312,6 +325,55 @@ class Tree:
         stats["node_visits"] = st.c.node_visits
         stats["voxel_tests"] = st.c.voxel_tests
         return squares, vox, hit, stats
+
+
+def render_trace(tree, cam, W, H, quality, nthreads=0):
+    """orc_render_trace: (voxel_px, hit_px, block_flags, px_src, try_box)
+    in the square layout (see vox_oracle.h)."""
+    set_voxel(tree.voxel)
+    c = camera_struct(cam)
+    sh = _Shading()
+    vox = np.full(W * H, MISS, np.uint32)
+    hit = np.full((3, W * H), np.nan, np.float32)
+    flags = np.full((W // 4) * (H // 4), 255, np.uint8)
+    src = np.full(W * H, 255, np.uint8)
+    box = np.full((6, W * H), np.nan, np.float32)
+    st = _RenderStats()
+    rc = lib().orc_render_trace(
+        tree.ptr, C.byref(c), W, H, quality, C.byref(sh), None,
+        vox.ctypes.data_as(C.POINTER(C.c_uint32)), _fp(hit), nthreads,
+        C.byref(st), flags.ctypes.data_as(C.POINTER(C.c_uint8)),
+        src.ctypes.data_as(C.POINTER(C.c_uint8)), _fp(box))
+    if rc != 0:
+        raise ValueError("orc_render_trace rejected the arguments (%d)" % rc)
+    return vox, hit, flags, src, box
+
+
+def ray_max_pending(tree, o, d):
+    """orc_ray_max_pending over (n, 3) rays."""
+    set_voxel(tree.voxel)
+    o = _f32(o).reshape(-1, 3)
+    d = _f32(d).reshape(-1, 3)
+    cols = [_f32(o[:, k]) for k in range(3)] + [_f32(d[:, k])
+                                                for k in range(3)]
+    out = np.zeros(len(o), np.uint32)
+    lib().orc_ray_max_pending(tree.ptr, len(o), *[_fp(c) for c in cols],
+                              out.ctypes.data_as(C.POINTER(C.c_uint32)))
+    return out
+
+
+def inner_nodes(tree):
+    """(boxes_min (k,3), boxes_max (k,3), centres (k,3), counts (k,)) of
+    the tree's inner nodes in depth-first preorder."""
+    if not tree.ptr:
+        z = np.zeros((0, 3), np.float32)
+        return z, z, z, np.zeros(0, np.uint64)
+    k = lib().orc_tree_inner_nodes(tree.ptr, None, None, 0)
+    out = np.zeros((max(k, 1), 9), np.float32)
+    cnt = np.zeros(max(k, 1), np.uint64)
+    lib().orc_tree_inner_nodes(tree.ptr, _fp(out),
+                               cnt.ctypes.data_as(C.POINTER(C.c_uint64)), k)
+    return out[:k, 0:3], out[:k, 3:6], out[:k, 6:9], cnt[:k]
 
 
 def camera_struct(cam):

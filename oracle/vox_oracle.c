@@ -507,6 +507,52 @@ uint64_t orc_tree_hash(const struct orc_node *tree)
 
 /* search.c:8-153.  Additionally reports the winning dot slot (-1 for a
  * dense leaf) and traversal counters. */
+/* Optional per-thread tracker for orc_ray_max_pending: for each inner node
+ * on the recursion path, how many of its children behind dividing planes
+ * (non-empty, not yet visited) are still pending; `max` records the most
+ * nodes with a pending child seen at any node visit.  Test-only
+ * instrumentation: the search's results and order are unchanged. */
+typedef struct {
+    int depth;
+    int rem[64];
+    int max;
+} pend_tracker;
+static __thread pend_tracker *g_pend;
+
+static void pend_visit(void)
+{
+    int k, cur = 0;
+    for (k = 0; k < g_pend->depth; k++) cur += g_pend->rem[k] > 0;
+    if (cur > g_pend->max) g_pend->max = cur;
+}
+
+/* Non-empty children of `tree` reached through its dividing planes in the
+ * order of search.c:105-147 (counted from the same primitive calls). */
+static int plane_children(const struct orc_node *tree, const orc_dot e,
+                          const orc_dot dir, int subspace)
+{
+    orc_dot pi[3];
+    float m[3];
+    int idx[3], cnt = 0, i, j, n = 0;
+    for (i = 0; i < 3; i++)
+        if (orc_hit_plane_within_box(e, dir, tree->data.inner.center, i,
+                                     pi[cnt], &tree->bounding_box)) {
+            m[cnt] = orc_abs_metric(e, pi[cnt]);
+            idx[cnt++] = i;
+        }
+    for (i = 0; i < cnt; i++) {
+        for (j = i + 1; j < cnt; j++)
+            if (m[j] < m[i]) {
+                float tm = m[j]; int ti = idx[j];
+                m[j] = m[i]; idx[j] = idx[i];
+                m[i] = tm; idx[i] = ti;
+            }
+        subspace ^= 1 << idx[i];
+        n += tree->data.inner.children[subspace] != NULL;
+    }
+    return n;
+}
+
 static const struct orc_node *rti(const struct orc_node *tree,
                                   const orc_dot origin, const orc_dot dir,
                                   orc_dot res, int *dot_slot,
@@ -521,6 +567,7 @@ static const struct orc_node *rti(const struct orc_node *tree,
 
     if (!tree) return NULL;
     c->node_visits++;
+    if (g_pend) pend_visit();
     if (!orc_hit_box(&tree->bounding_box, origin, dir, bb_inter)) return NULL;
 
     /* search.c:31-37 */
@@ -562,9 +609,17 @@ static const struct orc_node *rti(const struct orc_node *tree,
     /* search.c:84-97 */
     subspace = orc_get_corrected_subspace_idx(tree->data.inner.center,
                                               bb_inter, dir);
-    if ((leaf = rti(tree->data.inner.children[subspace], bb_inter, dir, res,
-                    dot_slot, c)))
+    if (g_pend) {
+        g_pend->rem[g_pend->depth] =
+            plane_children(tree, bb_inter, dir, subspace);
+        g_pend->depth++;
+    }
+    leaf = rti(tree->data.inner.children[subspace], bb_inter, dir, res,
+               dot_slot, c);
+    if (leaf) {
+        if (g_pend) g_pend->depth--;
         return leaf;
+    }
 
     /* search.c:105-113 */
     for (i = 0; i < 3; i++) {
@@ -591,11 +646,40 @@ static const struct orc_node *rti(const struct orc_node *tree,
             }
         }
         subspace = subspace ^ (1 << plane_idx[i]);
+        if (g_pend && tree->data.inner.children[subspace])
+            g_pend->rem[g_pend->depth - 1]--;
         if ((leaf = rti(tree->data.inner.children[subspace], plane_inter[i],
-                        dir, res, dot_slot, c)))
+                        dir, res, dot_slot, c))) {
+            if (g_pend) g_pend->depth--;
             return leaf;
+        }
     }
+    if (g_pend) g_pend->depth--;
     return NULL;
+}
+
+/* For each ray, the most inner nodes on the search path that still had a
+ * child behind a dividing plane to visit, over all node visits of the
+ * search: the entries a stack holding one entry per such node needs. */
+void orc_ray_max_pending(const struct orc_node *tree, size_t n,
+                         const float *ox, const float *oy, const float *oz,
+                         const float *dx, const float *dy, const float *dz,
+                         uint32_t *out)
+{
+    size_t r;
+    pend_tracker t;
+    orc_counters c = {0, 0};
+    g_pend = &t;
+    for (r = 0; r < n; r++) {
+        orc_dot o = {ox[r], oy[r], oz[r], 0.0f};
+        orc_dot d = {dx[r], dy[r], dz[r], 0.0f};
+        orc_dot res;
+        int slot;
+        memset(&t, 0, sizeof(t));
+        (void)rti(tree, o, d, res, &slot, &c);
+        out[r] = (uint32_t)t.max;
+    }
+    g_pend = NULL;
 }
 
 const struct orc_node *orc_ray_tree_intersection(const struct orc_node *tree,
@@ -918,8 +1002,31 @@ typedef struct {
     uint32_t *squares;
     uint32_t *voxel_px;
     float *hit_px;
+    /* optional trace of the loop's decisions (orc_render_trace) */
+    uint8_t *block_flags;
+    uint8_t *px_src;
+    float *try_box;
     atomic_ullong fast_blocks, best_blocks, merge_blocks, root_tr, leaf_tr;
 } render_ctx;
+
+/* Trace of pixel (px, py): where its result came from, and for a leaf try
+ * that answered it, the box of the leaf that was tried. */
+static void px_trace(render_ctx *x, int px, int py, int src,
+                     const struct orc_node *tried)
+{
+    size_t cs, i, np;
+    int l;
+    if (!x->px_src) return;
+    cs = (size_t)(py >> 2) * x->ws + (size_t)(px >> 2);
+    i = cs * 16 + (size_t)((py & 3) * 4 + (px & 3));
+    np = (size_t)x->W * x->H;
+    x->px_src[i] = (uint8_t)src;
+    if (x->try_box && tried)
+        for (l = 0; l < 3; l++) {
+            x->try_box[l * np + i] = tried->bounding_box.min[l];
+            x->try_box[(3 + l) * np + i] = tried->bounding_box.max[l];
+        }
+}
 
 /* Per-pixel outputs use the square layout: index cs*16 + p. */
 static void px_out(render_ctx *x, int px, int py, const struct orc_node *leaf,
@@ -968,6 +1075,7 @@ static void render_task(void *p, size_t b, size_t e, orc_counters *c)
             leaf = corner1;
             vid1 = voxel_id_of(corner1, inter1, slot);
             px_out(x, xstart, ystart, corner1, inter1, vid1);
+            px_trace(x, xstart, ystart, 0, NULL);
             vid = vid1;
             if (corner1 != NULL) {
                 iend = 15;
@@ -975,6 +1083,7 @@ static void render_task(void *p, size_t b, size_t e, orc_counters *c)
                 orc_screen2world(x->cam, xstart + 3, ystart + 3, dir2);
                 corner2 = rti(x->scene, origin, dir2, inter2, &slot, c);
                 nroot++;
+                px_trace(x, xstart + 3, ystart + 3, 0, NULL);
                 px_out(x, xstart + 3, ystart + 3, corner2, inter2,
                        voxel_id_of(corner2, inter2, slot));
                 if (corner2 != NULL) {
@@ -1001,6 +1110,9 @@ static void render_task(void *p, size_t b, size_t e, orc_counters *c)
         if (block_rnd_mode == ORC_QUALITY_FAST) atomic_fetch_add(&x->fast_blocks, 1);
         else atomic_fetch_add(&x->best_blocks, 1);
         if (block_merge_mode) atomic_fetch_add(&x->merge_blocks, 1);
+        if (x->block_flags)
+            x->block_flags[cs] = (uint8_t)((block_rnd_mode == ORC_QUALITY_BEST) |
+                                           (block_merge_mode ? 2 : 0));
 
         for (i = istart; i < iend; i++) {
             int pp = rendering_order[i];
@@ -1008,12 +1120,15 @@ static void render_task(void *p, size_t b, size_t e, orc_counters *c)
             int merge = block_merge_mode && (i & 1);
             orc_screen2world(x->cam, xx + xstart, y + ystart, dir1);
             if (!merge) {
+                const struct orc_node *tried = NULL;
                 if (block_rnd_mode == ORC_QUALITY_FAST) {
                     if (leaf != NULL) {
+                        tried = leaf;
                         leaf = rti(leaf, origin, dir1, inter1, &slot, c);
                         nleaf++;
                     }
                     if (leaf == NULL) {
+                        tried = NULL;
                         leaf = rti(x->scene, origin, dir1, inter1, &slot, c);
                         nroot++;
                     }
@@ -1022,6 +1137,9 @@ static void render_task(void *p, size_t b, size_t e, orc_counters *c)
                     nroot++;
                 }
                 vid = voxel_id_of(leaf, inter1, slot);
+                px_trace(x, xx + xstart, y + ystart, tried ? 1 : 0, tried);
+            } else {
+                px_trace(x, xx + xstart, y + ystart, 2, NULL);
             }
             if (leaf != NULL) {
                 uint32_t color = 0;
@@ -1043,6 +1161,18 @@ int orc_render(const struct orc_node *scene, const orc_camera *cam, int W,
                uint32_t *squares, uint32_t *voxel_px, float *hit_px,
                int nthreads, orc_render_stats *stats)
 {
+    return orc_render_trace(scene, cam, W, H, quality, shade, squares,
+                            voxel_px, hit_px, nthreads, stats, NULL, NULL,
+                            NULL);
+}
+
+int orc_render_trace(const struct orc_node *scene, const orc_camera *cam,
+                     int W, int H, unsigned quality,
+                     const orc_shading *shade, uint32_t *squares,
+                     uint32_t *voxel_px, float *hit_px, int nthreads,
+                     orc_render_stats *stats, uint8_t *block_flags,
+                     uint8_t *px_src, float *try_box)
+{
     render_ctx x;
     orc_counters tot;
     size_t nsq;
@@ -1062,6 +1192,9 @@ int orc_render(const struct orc_node *scene, const orc_camera *cam, int W,
     x.squares = squares;
     x.voxel_px = voxel_px;
     x.hit_px = hit_px;
+    x.block_flags = block_flags;
+    x.px_src = px_src;
+    x.try_box = try_box;
     nsq = (size_t)(W >> 2) * (size_t)(H >> 2);
     orc_parallel(render_task, &x, nsq, 64, nthreads, &tot);
     if (stats) {
@@ -1634,4 +1767,34 @@ struct orc_node *orc_make_dense_leaf_box(const float mn[3], const float mx[3])
 int orc_tree_flags(const struct orc_node *tree)
 {
     return tree ? (int)tree->flags : -1;
+}
+
+/* ------------------------------------------------------------------ */
+/* Inner nodes for the tests: box, centre and voxel count of each inner */
+/* node, in depth-first preorder over children 0..7.                    */
+/* ------------------------------------------------------------------ */
+static void inner_walk(const struct orc_node *t, float *out, uint64_t *cnt,
+                       size_t cap, size_t *k)
+{
+    int i;
+    if (!t || (t->flags & ORC_LEAF_MASK)) return;
+    if (*k < cap) {
+        for (i = 0; i < 3; i++) {
+            out[9 * *k + i] = t->bounding_box.min[i];
+            out[9 * *k + 3 + i] = t->bounding_box.max[i];
+            out[9 * *k + 6 + i] = t->data.inner.center[i];
+        }
+        cnt[*k] = t->dots_num;
+    }
+    (*k)++;
+    for (i = 0; i < 8; i++) inner_walk(t->data.inner.children[i], out, cnt,
+                                       cap, k);
+}
+
+size_t orc_tree_inner_nodes(const struct orc_node *tree, float *out,
+                            uint64_t *counts, size_t cap)
+{
+    size_t k = 0;
+    inner_walk(tree, out, counts, cap, &k);
+    return k;
 }

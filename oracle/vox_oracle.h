@@ -173,6 +173,33 @@ int orc_render(const struct orc_node *scene, const orc_camera *cam, int W,
                uint32_t *squares, uint32_t *voxel_px, float *hit_px,
                int nthreads, orc_render_stats *stats);
 
+/* orc_render with a trace of the loop's decisions (each output may be
+ * NULL): block_flags[cs] = bit 0 block traced in BEST mode, bit 1 every
+ * other pixel merged; px_src[i] (square layout) = 0 root search (corners
+ * included), 1 answered by a try of the previous pixel's leaf, 2 a merged
+ * copy of the previous pixel; try_box = 6 planes of W*H floats (min xyz,
+ * max xyz) holding the tried leaf's box where px_src is 1. */
+int orc_render_trace(const struct orc_node *scene, const orc_camera *cam,
+                     int W, int H, unsigned quality,
+                     const orc_shading *shade, uint32_t *squares,
+                     uint32_t *voxel_px, float *hit_px, int nthreads,
+                     orc_render_stats *stats, uint8_t *block_flags,
+                     uint8_t *px_src, float *try_box);
+
+/* Inner nodes in depth-first preorder: 9 floats each (box min xyz, box
+ * max xyz, centre xyz) and the voxel count (dots_num, tree.c:270).  Writes
+ * at most cap of them; returns how many there are. */
+size_t orc_tree_inner_nodes(const struct orc_node *tree, float *out,
+                            uint64_t *counts, size_t cap);
+
+/* For each ray, the most inner nodes on the search path of
+ * vox_ray_tree_intersection that still had a child behind a dividing plane
+ * to visit, over the search's node visits (single thread). */
+void orc_ray_max_pending(const struct orc_node *tree, size_t n,
+                         const float *ox, const float *oy, const float *oz,
+                         const float *dx, const float *dy, const float *dz,
+                         uint32_t *out);
+
 /* ---- FP64 brute force (independent of the tree) ---- */
 /* For each ray: first voxel hit along o + t d, t >= 0 (closed boxes, an
  * origin inside a box hits at t=0).  voxel_idx: index into dots or -1.

@@ -303,3 +303,56 @@ def test_launch_configs_per_kind():
         assert L.voxg_get_launch_config(C.byref(cfg)) == 0
         assert cfg.leaf_batch == 7 and cfg.refill == saved["rays"]["refill"]
     assert VT.get_launch_configs() == saved
+
+
+def _deep_scenes():
+    from voxvision_amd import scenes
+    rng = np.random.default_rng(17)
+    ii = np.arange(-50, 50)
+    I, J, K = np.meshgrid(ii, ii, ii, indexing="ij")
+    m = (I * I + J * J + K * K) < 2500
+    yield "ball50", np.stack([I[m], J[m], K[m]], 1).astype(np.float32)
+    yield "menger81", scenes.volume_to_dots(scenes.menger_volume(81))
+    # sparse points in a large box: many inner levels
+    yield "sparse", np.unique(rng.integers(0, 4096, (3000, 3)),
+                              axis=0).astype(np.float32)
+    # a diagonal chain of voxels with clusters along it
+    t = np.arange(0, 2000, 3)
+    chain = np.stack([t, t // 2, t // 3], 1)
+    blobs = (chain[::25, None, :] + rng.integers(-2, 3, (len(chain[::25]),
+                                                         20, 3))).reshape(-1, 3)
+    yield "chain", np.unique(np.concatenate([chain, blobs]),
+                             axis=0).astype(np.float32)
+    yield "ct256", scenes.config_c2(frames=1)["dots"]
+
+
+@pytest.mark.parametrize("name", ["ball50", "menger81", "sparse", "chain",
+                                  "ct256"])
+def test_stack_bound_covers_the_search(voxlib, oracle, name):
+    """The traversal stack holds one entry per inner node on the current
+    path that still has a child behind a dividing plane to visit; the bound
+    the library sizes its stack for (voxg_tree_plan's stack_entries,
+    depth - 1) must cover the most such nodes of any search, measured by
+    the oracle's instrumented vox_ray_tree_intersection (search.c:84-148)
+    on random rays and rays from outside the box on the deepest scenes."""
+    from voxvision_amd import scenes, voxtrees as VT
+    dots = dict(_deep_scenes())[name]
+    info, _ = VT.plan(dots)
+    T = oracle.Tree(dots)
+    assert info["depth"] == T.stats["depth"]
+    assert info["stack_entries"] == max(1, info["depth"] - 1)
+    bmin, bmax = scenes.bbox_of(dots)
+    o, d = scenes.random_rays(60000, bmin, bmax, seed=3)
+    # rays from far outside, aimed at random voxels (long searches)
+    rng = np.random.default_rng(4)
+    tgt = dots[rng.integers(0, len(dots), 20000)] + 0.5
+    src = (bmin + bmax) / 2 + rng.standard_normal((20000, 3)) * \
+        float(np.max(bmax - bmin))
+    o = np.concatenate([o, src.astype(np.float32)])
+    d = np.concatenate([d, (tgt - src).astype(np.float32)])
+    mp = oracle.ray_max_pending(T, o, d)
+    print(name, "depth", info["depth"], "bound", info["stack_entries"],
+          "max pending", int(mp.max()), "mean", float(mp.mean()))
+    assert int(mp.max()) <= info["stack_entries"]
+    assert int(mp.max()) >= 2
+    T.close()

@@ -12,8 +12,8 @@ coordinate descent from the kind's current configuration: each parameter in turn
 is swept over its candidate values with the others fixed, keeping the
 fastest, until a full pass changes nothing.  Timing uses HIP events on the
 current stream; each candidate is warmed up once and timed over `reps`
-runs.  Configurations whose LDS use (stack_lds * block * 16 B) exceeds
-64 KiB are skipped.
+runs.  Configurations whose stack ring (stack_lds * block * 20 B of LDS)
+exceeds the 160 KiB of LDS of a gfx950 CU are skipped.
 """
 import itertools
 import json
@@ -23,7 +23,7 @@ import torch
 from . import voxtrees as VT
 
 SPACE = {
-    "stack_lds": [2, 4, 8, 16],
+    "stack_lds": [2, 4, 8, 16, 32],
     "block": [64, 128, 256],
     "waves_per_cu": [8, 12, 16, 20, 24, 32],
     "refill": [1, 4, 8, 16, 32, 48, 64],
@@ -35,8 +35,12 @@ SPACE = {
 }
 
 
+ENTRY_BYTES = 20  # one stack entry: node, entry point, pending list
+LDS_BYTES = 160 * 1024
+
+
 def _valid(cfg):
-    return cfg["stack_lds"] * cfg["block"] * 16 <= 64 * 1024
+    return cfg["stack_lds"] * cfg["block"] * ENTRY_BYTES <= LDS_BYTES
 
 
 def time_config(make_runner, cfg, reps=3, kind=None):

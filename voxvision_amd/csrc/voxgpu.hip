@@ -47,11 +47,16 @@
 
 using namespace voxg;
 
-// Pending-entry bound of the traversal stack: at most three pending
-// children per inner level on the current path.
+// Entry bound of the traversal stack: one entry per inner node on the
+// current path that still has a child to visit, so at most one per inner
+// level.
 static uint32_t stack_bound_of(const HostTree &h) {
-    return h.depth > 1 ? 3u * (h.depth - 1) + 1u : 1u;
+    return h.depth > 1 ? h.depth - 1u : 1u;
 }
+
+// Bytes of one stack entry (ring and spill buffer): node index and entry
+// point (16 B), then the node's pending list (4 B) in a second array.
+constexpr size_t kEntryBytes = sizeof(uint4) + sizeof(uint32_t);
 
 static size_t device_bytes_of(const HostTree &h) {
     size_t nb = h.nodes.size() * sizeof(NodeRec);
@@ -886,7 +891,9 @@ rti_kernel(Tree t, Work work_arg, uint32_t n, Ctrl *ctrl, uint32_t *ovf,
            uint32_t *xnext, uint4 *__restrict__ spill,
            uint32_t spill_slots, uint32_t refill, uint32_t fin_batch,
            uint32_t leaf_batch) {
-    extern __shared__ uint4 ring[];  // [K][blockDim.x]
+    // stack ring: entries [K][blockDim.x], then their pending lists
+    extern __shared__ uint4 ring[];
+    uint32_t *const ring_st = reinterpret_cast<uint32_t *>(ring + K * blockDim.x);
     Work work = work_arg;
     // the unit count may come from the device (the render's block queue)
     n = work.prepare(n);
@@ -900,6 +907,9 @@ rti_kernel(Tree t, Work work_arg, uint32_t n, Ctrl *ctrl, uint32_t *ovf,
     const uint32_t gtid = blockIdx.x * bdim + tid;
     const uint32_t gthreads = gridDim.x * bdim;
     const uint32_t lane = lane_id();
+    // spill buffer: entries [spill_slots][gthreads], then their lists
+    uint32_t *const spill_st =
+        reinterpret_cast<uint32_t *>(spill + (size_t)spill_slots * gthreads);
 
     // XCD-aware hand-out: the units are cut into chunks of xch units and
     // chunk j goes to range j % kXcds, each range with a counter of its own
@@ -1046,31 +1056,45 @@ rti_kernel(Tree t, Work work_arg, uint32_t n, Ctrl *ctrl, uint32_t *ovf,
                 int s2 = s1 ^ (1 << i1);
                 int s3 = s2 ^ (1 << i2);
                 uint32_t cs0 = child_of(nd.ref, mask, s);
-                uint32_t cs1 = cntp >= 1 ? child_of(nd.ref, mask, s1) : kEnd;
-                uint32_t cs2 = cntp >= 2 ? child_of(nd.ref, mask, s2) : kEnd;
-                uint32_t cs3 = cntp >= 3 ? child_of(nd.ref, mask, s3) : kEnd;
-                // push in reverse visiting order (:137-147)
-#define VOXG_PUSH(CI, Q)                                                  \
-    if ((CI) != kEnd) {                                                  \
-        if (sp - lo == K) {                                              \
-            if ((uint32_t)lo >= spill_slots) {                           \
-                *(volatile uint32_t *)ovf = 1u;                          \
-            } else {                                                     \
-                spill[(size_t)lo * gthreads + gtid] =                    \
-                    ring[(lo & (K - 1)) * bdim + tid];                   \
-                if (COUNT) ns++;                                         \
-            }                                                            \
-            lo++;                                                        \
-        }                                                                \
-        ring[(sp & (K - 1)) * bdim + tid] = make_uint4(                  \
-            (CI), __float_as_uint((Q).x), __float_as_uint((Q).y),        \
-            __float_as_uint((Q).z));                                     \
-        sp++;                                                            \
-    }
-                VOXG_PUSH(cs3, Q2)
-                VOXG_PUSH(cs2, Q1)
-                VOXG_PUSH(cs1, Q0)
-#undef VOXG_PUSH
+                // the children behind the planes that exist, in visiting
+                // order (:137-147), as (plane, child offset) pairs of a
+                // pending list: count in bits 0-1, pair j in bits 2+5j..
+                uint32_t lst = 0u, np = 0u;
+                auto pend = [&](int sub, int plane) {
+                    if ((mask >> sub) & 1u) {
+                        const uint32_t off =
+                            (uint32_t)__popc(mask & ((1u << sub) - 1u));
+                        lst |= ((uint32_t)plane | (off << 2)) << (2u + 5u * np);
+                        np++;
+                    }
+                };
+                if (cntp >= 1) pend(s1, i0);
+                if (cntp >= 2) pend(s2, i1);
+                if (cntp >= 3) pend(s3, i2);
+                // one stack entry for the node: its index and entry point,
+                // the plane points are recomputed when the entry is popped
+                if (np) {
+                    if (sp - lo == K) {
+                        // ring full: its oldest entry goes to the spill
+                        // buffer (or is dropped past the bound: reported)
+                        if ((uint32_t)lo >= spill_slots) {
+                            *(volatile uint32_t *)ovf = 1u;
+                        } else {
+                            const size_t si = (size_t)lo * gthreads + gtid;
+                            const uint32_t ri = (lo & (K - 1)) * bdim + tid;
+                            spill[si] = ring[ri];
+                            spill_st[si] = ring_st[ri];
+                            if (COUNT) ns++;
+                        }
+                        lo++;
+                    }
+                    const uint32_t ri = (sp & (K - 1)) * bdim + tid;
+                    ring[ri] = make_uint4(cur, __float_as_uint(e.x),
+                                          __float_as_uint(e.y),
+                                          __float_as_uint(e.z));
+                    ring_st[ri] = lst | np;
+                    sp++;
+                }
                 if (cs0 != kEnd) {
                     cur = cs0;
                     org = e;
@@ -1100,25 +1124,46 @@ rti_kernel(Tree t, Work work_arg, uint32_t n, Ctrl *ctrl, uint32_t *ovf,
             }
         }
 
-        // phase 3: continue with the next pending (node, origin) pair
+        // phase 3: continue with the next pending child of the nearest
+        // node that has one: the child behind the entry's next plane, with
+        // the plane point as origin, computed from the node's entry point as
+        // hit_plane computes it (geom-sse.c:118-142)
         if (popme) {
-            if (sp == 0) {
-                cur = kEnd;
-            } else {
-                uint4 v;
-                if (sp == lo) {
-                    sp--;
-                    lo--;
-                    v = ((uint32_t)lo < spill_slots)
-                            ? spill[(size_t)lo * gthreads + gtid]
-                            : make_uint4(kEnd, 0, 0, 0);
+            cur = kEnd;
+            bool have = sp > 0;
+            if (have && sp == lo) {
+                // the ring is empty: the top entry comes back from the
+                // spill buffer (unless it was dropped past the bound)
+                lo--;
+                if ((uint32_t)lo < spill_slots) {
+                    const size_t si = (size_t)lo * gthreads + gtid;
+                    const uint32_t ri = (lo & (K - 1)) * bdim + tid;
+                    ring[ri] = spill[si];
+                    ring_st[ri] = spill_st[si];
                 } else {
-                    sp--;
-                    v = ring[(sp & (K - 1)) * bdim + tid];
+                    have = false;
                 }
-                cur = v.x;  // kEnd only after a reported overflow
-                org = f3{__uint_as_float(v.y), __uint_as_float(v.z),
-                         __uint_as_float(v.w)};
+            }
+            if (have) {
+                const uint32_t ri = ((sp - 1) & (K - 1)) * bdim + tid;
+                const uint4 v = ring[ri];
+                const uint32_t lst = ring_st[ri];
+                const uint32_t np = lst & 3u, pair = (lst >> 2) & 31u;
+                if (np == 1u) sp--;
+                else ring_st[ri] = (np - 1u) | ((lst >> 5) & ~3u);
+                const int p = (int)(pair & 3u);
+                const f3 e = f3{__uint_as_float(v.y), __uint_as_float(v.z),
+                                __uint_as_float(v.w)};
+                const uint32_t *rec =
+                    reinterpret_cast<const uint32_t *>(T.nodes + v.x);
+                const uint32_t ref = rec[offsetof(NodeRec, ref) / 4];
+                const float cp =
+                    __uint_as_float(rec[offsetof(NodeRec, c) / 4 + p]);
+                const float k = (cp - comp(e, p)) / comp(d, p);
+                org = f3{__builtin_fmaf(k, d.x, e.x),
+                         __builtin_fmaf(k, d.y, e.y),
+                         __builtin_fmaf(k, d.z, e.z)};
+                cur = ref + (pair >> 2);
             }
             if (cur == kEnd) {
                 ph = PH_DONE;
@@ -1971,7 +2016,7 @@ int launch_k(const dev::Tree &dt, uint32_t stack_bound, Work &work, size_t n,
     if (const uint32_t tb = g_test_bound.load()) stack_bound = std::min(stack_bound, tb);
     uint32_t slots = stack_bound > (uint32_t)K ? stack_bound - K : 0u;
     const size_t spill_need =
-        (size_t)std::max<uint32_t>(slots, 1u) * gthreads * sizeof(uint4);
+        (size_t)std::max<uint32_t>(slots, 1u) * gthreads * kEntryBytes;
     // the stream's workspace, or one the caller owns (sized in advance)
     Workspace *ws = own;
     int rc = 0;
@@ -1983,7 +2028,7 @@ int launch_k(const dev::Tree &dt, uint32_t stack_bound, Work &work, size_t n,
     }
     rc = take_overflow(ws);
     if (rc) return rc;
-    size_t lds = (size_t)K * block * sizeof(uint4);
+    size_t lds = (size_t)K * block * kEntryBytes;
     auto kern = dev::rti_kernel<K, Work, COUNT>;
     if (lds > 65536) {
         HIPCHK(hipFuncSetAttribute((const void *)kern,
@@ -2315,8 +2360,9 @@ static int check_launch_config(const voxg_launch_config *cfg) {
         return fail(VOXG_EINVAL, "stack_lds must be 2, 4, 8, 16 or 32");
     if (cfg->block < 64 || cfg->block > 256 || cfg->block % 64)
         return fail(VOXG_EINVAL, "block must be 64, 128, 192 or 256");
-    if ((size_t)k * cfg->block * 16 > 160 * 1024)
-        return fail(VOXG_EINVAL, "stack_lds * block * 16 exceeds 160 KiB of LDS");
+    if ((size_t)k * cfg->block * kEntryBytes > 160 * 1024)
+        return fail(VOXG_EINVAL,
+                    "stack_lds * block * 20 exceeds 160 KiB of LDS");
     if (cfg->waves_per_cu < 1 || cfg->waves_per_cu > 64)
         return fail(VOXG_EINVAL, "waves_per_cu must be in 1..64");
     if (cfg->refill < 1 || cfg->refill > 64)
@@ -2446,7 +2492,7 @@ int voxg_ray_batch_noalloc(const voxg_tree *t, size_t n, const float *ox,
     // as many persistent lanes as the scratch holds spill entries for: the
     // configured waves per CU when they fit, fewer otherwise
     const size_t entries = (dev::kNoAllocBytes - dev::kNoAllocSpill) /
-                           sizeof(uint4);
+                           kEntryBytes;
     const size_t lanes_fit = entries / slots;
     const size_t wave_all = (size_t)cu_count(devid) * 64;  // a wave per CU
     if (lanes_fit < wave_all)
@@ -2467,7 +2513,7 @@ int voxg_ray_batch_noalloc(const voxg_tree *t, size_t n, const float *ox,
     ws.xnext = (uint32_t *)(base + dev::kNoAllocXnext);
     ws.ovf_d = (uint32_t *)(base + dev::kNoAllocOvf);
     ws.spill = (uint4 *)(base + dev::kNoAllocSpill);
-    ws.spill_bytes = entries * sizeof(uint4);
+    ws.spill_bytes = entries * kEntryBytes;
     hipLaunchKernelGGL(dev::noalloc_clear_kernel, dim3(1), dim3(256), 0,
                        stream, base);
     HIPCHK(hipGetLastError());
@@ -2893,7 +2939,7 @@ int voxg_batch_create(size_t n_items, const voxg_batch_item *items,
         (uint32_t)b->cfg.block;
     const uint32_t k = (uint32_t)b->cfg.stack_lds;
     const uint32_t slots = bound > k ? bound - k : 1u;
-    const size_t sb = (size_t)slots * gthreads * sizeof(uint4);
+    const size_t sb = (size_t)slots * gthreads * kEntryBytes;
     hipError_t e = hipMalloc(&ws->spill, sb);
     if (e == hipSuccess) {
         ws->spill_bytes = sb;
