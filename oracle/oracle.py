@@ -121,6 +121,10 @@ def lib():
         L.orc_brute64_rays.argtypes = [F, C.c_size_t, F, C.c_size_t] + [
             F] * 6 + [C.c_double, C.POINTER(C.c_int32),
                       C.POINTER(C.c_double), C.POINTER(C.c_uint8), C.c_int]
+        L.orc_camera_rays.argtypes = [C.POINTER(_Camera), C.c_int, C.c_int,
+                                      F]
+        L.orc_dda64_rays.argtypes = L.orc_brute64_rays.argtypes
+        L.orc_dda64_rays.restype = C.c_int
         L.orc_brute_balls.argtypes = [F, C.c_size_t, F, C.c_size_t] + [
             F] * 4 + [C.POINTER(C.c_uint8), C.c_int]
         L.orc_cpu_count.restype = C.c_int
@@ -349,6 +353,14 @@ def render_trace(tree, cam, W, H, quality, nthreads=0):
     return vox, hit, flags, src, box
 
 
+def camera_rays(cam, W, H):
+    """(W*H, 3) fp32 ray directions of a frame in the square layout."""
+    c = camera_struct(cam)
+    out = np.zeros((3, W * H), np.float32)
+    lib().orc_camera_rays(C.byref(c), W, H, _fp(out))
+    return np.ascontiguousarray(out.T)
+
+
 def ray_max_pending(tree, o, d):
     """orc_ray_max_pending over (n, 3) rays."""
     set_voxel(tree.voxel)
@@ -517,6 +529,31 @@ def brute64_rays(dots_xyz, o, d, voxel=(1.0, 1.0, 1.0), eps=1e-6,
                            hit.ctypes.data_as(C.POINTER(C.c_double)),
                            tie.ctypes.data_as(C.POINTER(C.c_uint8)),
                            nthreads)
+    return idx, hit, tie.astype(bool)
+
+
+def dda64_rays(dots_xyz, o, d, voxel=(1.0, 1.0, 1.0), eps=1e-6,
+               nthreads=0):
+    """FP64 grid walk with the brute force's answers (see vox_oracle.h):
+    (voxel index into dots or -1, hit xyz f64, tie)."""
+    dots = _f32(dots_xyz).reshape(-1, 3)
+    o = _f32(o).reshape(-1, 3)
+    d = _f32(d).reshape(-1, 3)
+    n = len(o)
+    V = _f32(voxel)
+    cols = [_f32(o[:, k]) for k in range(3)] + [_f32(d[:, k])
+                                                for k in range(3)]
+    idx = np.full(n, -1, np.int32)
+    hit = np.full((n, 3), np.nan, np.float64)
+    tie = np.zeros(n, np.uint8)
+    rc = lib().orc_dda64_rays(_fp(dots), len(dots), _fp(V), n,
+                              *[_fp(c) for c in cols], float(eps),
+                              idx.ctypes.data_as(C.POINTER(C.c_int32)),
+                              hit.ctypes.data_as(C.POINTER(C.c_double)),
+                              tie.ctypes.data_as(C.POINTER(C.c_uint8)),
+                              nthreads)
+    if rc != 0:
+        raise ValueError("orc_dda64_rays failed (%d)" % rc)
     return idx, hit, tie.astype(bool)
 
 

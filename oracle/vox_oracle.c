@@ -1798,3 +1798,21 @@ size_t orc_tree_inner_nodes(const struct orc_node *tree, float *out,
     inner_walk(tree, out, counts, cap, &k);
     return k;
 }
+
+/* The fp32 ray directions of a frame, in the square layout (index
+ * cs*16 + p): orc_screen2world of every pixel (simple-camera.c:21-31). */
+void orc_camera_rays(const orc_camera *cam, int W, int H, float *dxyz)
+{
+    size_t np = (size_t)W * H, cs, ws = (size_t)(W >> 2);
+    int p;
+    for (cs = 0; cs < np / 16; cs++)
+        for (p = 0; p < 16; p++) {
+            orc_dot r;
+            size_t i = cs * 16 + (size_t)p;
+            orc_screen2world(cam, (int)(cs % ws) * 4 + (p & 3),
+                             (int)(cs / ws) * 4 + (p >> 2), r);
+            dxyz[i] = r[0];
+            dxyz[np + i] = r[1];
+            dxyz[2 * np + i] = r[2];
+        }
+}

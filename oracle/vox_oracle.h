@@ -200,6 +200,10 @@ void orc_ray_max_pending(const struct orc_node *tree, size_t n,
                          const float *dx, const float *dy, const float *dz,
                          uint32_t *out);
 
+/* Ray directions of every pixel of a W x H frame (orc_screen2world), as 3
+ * planes of W*H floats in the square layout. */
+void orc_camera_rays(const orc_camera *cam, int W, int H, float *dxyz);
+
 /* ---- FP64 brute force (independent of the tree) ---- */
 /* For each ray: first voxel hit along o + t d, t >= 0 (closed boxes, an
  * origin inside a box hits at t=0).  voxel_idx: index into dots or -1.
@@ -211,6 +215,15 @@ void orc_brute64_rays(const float *dots_xyz, size_t nd, const float voxel[3],
                       const float *oz, const float *dx, const float *dy,
                       const float *dz, double eps, int32_t *voxel_idx,
                       double *hit_xyz, uint8_t *tie, int nthreads);
+/* orc_brute64_rays for voxels on the grid (dots multiples of voxel), by an
+ * FP64 grid walk: the same voxel and hit point; near misses more than two
+ * cells past the first hit do not count as ties.  Returns 0, or -22 when a
+ * dot is off the grid, -34 / -12 when the grid is too large. */
+int orc_dda64_rays(const float *dots_xyz, size_t nd, const float voxel[3],
+                   size_t n, const float *ox, const float *oy,
+                   const float *oz, const float *dx, const float *dy,
+                   const float *dz, double eps, int32_t *voxel_idx,
+                   double *hit_xyz, uint8_t *tie, int nthreads);
 /* Brute-force ball test with the reference fp32 formula over all voxels. */
 void orc_brute_balls(const float *dots_xyz, size_t nd, const float voxel[3],
                      size_t n, const float *cx, const float *cy,

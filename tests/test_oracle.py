@@ -828,3 +828,56 @@ def test_inner_centres_are_rounded_means(oracle, scene):
           "excluded (inexact sums)", int((~exact).sum()))
     assert checked >= 0.5 * 3 * len(cnt)
     T.close()
+
+
+def test_dda64_matches_brute_force(oracle):
+    """The FP64 grid walk (used on scenes too large for the brute force)
+    gives the brute force's voxel and hit point on every ray, and flags a
+    subset of its ties (it ignores near misses two cells past the first
+    hit), on random scenes with random and axis-aligned rays."""
+    from voxvision_amd import scenes
+    for seed in (1, 2, 3):
+        rng = np.random.default_rng(seed)
+        occ = rng.random((24, 20, 18)) < 0.2
+        dots = scenes.volume_to_dots(occ)
+        bmin, bmax = scenes.bbox_of(dots)
+        o, d = scenes.random_rays(8000, bmin, bmax, seed=seed)
+        # integer origins (on faces and edges) with zero direction terms
+        oi = rng.integers(-2, 26, (4000, 3)).astype(np.float32)
+        di = rng.integers(-2, 3, (4000, 3)).astype(np.float32)
+        di[(di == 0).all(1), 0] = 1
+        o = np.concatenate([o, oi])
+        d = np.concatenate([d, di])
+        i1, h1, t1 = oracle.brute64_rays(dots, o, d, eps=1e-5)
+        i2, h2, t2 = oracle.dda64_rays(dots, o, d, eps=1e-5)
+        # the same voxel, except on exact ties (equal entries, where each
+        # keeps the voxel it tested first), which both flag
+        diff = i1 != i2
+        assert (t1[diff] & t2[diff]).all()
+        assert np.array_equal(i1 >= 0, i2 >= 0)
+        m = i1 >= 0
+        assert np.array_equal(h1[m], h2[m])
+        assert not (t2 & ~t1).any()
+        assert diff.mean() < 0.02
+
+
+def test_fp64_accuracy_on_the_benchmark_workloads(oracle):
+    """Hit/miss and hit voxel agree with FP64 outside the tie set on the
+    full C1 frame, 120,000 pixels of two 1080p poses of the 256^3 scene and
+    200,000 of its incoherent rays; hit points within 1e-5 of the scene
+    scale; the numbers recorded in tests/golden/fp64_accuracy.json are
+    reproduced."""
+    import json
+    import sys
+    sys.path.insert(0, os.path.join(HERE, "golden"))
+    import make_fp64_accuracy as M
+    got = M.measure()
+    want = json.load(open(M.OUT))
+    for name in ("c1_frame_640x480", "c2_two_poses_1080p_120k",
+                 "c3_incoherent_200k"):
+        r = got[name]
+        assert r["hit_miss_agree_outside_ties"], name
+        assert r["voxel_agree_outside_ties"], name
+        assert r["tie_fraction"] < 0.01, name
+        assert r["hit_error_rel_max"] <= 1e-5, name
+        assert r == want[name], name
