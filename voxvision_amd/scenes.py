@@ -236,14 +236,48 @@ def camera_rays(cam, W, H):
     return np.ascontiguousarray(o, np.float32), d.astype(np.float32)
 
 
+def _c4_scene(kind, edge, occ, nrays, a, seed, s):
+    if kind == "noise":
+        vol = noise_volume(edge, occ, seed * 1000 + s)
+    elif kind == "shell":
+        vol = shell_volume(edge, occ)
+    else:
+        vol = menger_volume(edge)
+    dots = volume_to_dots(vol)
+    occ_real = float(vol.mean())
+    del vol
+    bmin, bmax = bbox_of(dots) if len(dots) else (np.zeros(3),
+                                                  np.ones(3) * edge)
+    ncam = nrays // 2
+    W = max(16, int(np.sqrt(ncam * 16 / 9)))
+    H = max(1, ncam // W)
+    center = (bmin + bmax) / 2.0
+    dist = 1.5 * np.sqrt(3.0) * edge
+    pos = center + dist * np.array([np.cos(a) * 0.94, np.sin(a) * 0.94,
+                                    0.34])
+    cam = _camera.make_camera(pos, center, np.tan(np.radians(30)), W, H)
+    oc, dc = camera_rays(cam, W, H)
+    orr, drr = random_rays(nrays - len(oc), bmin, bmax,
+                           seed=seed * 1000 + 500 + s)
+    return dict(kind=kind, edge=edge, occupancy=occ_real, dots=dots,
+                o=np.concatenate([oc, orr]), d=np.concatenate([dc, drr]),
+                camera_pixels=len(oc))
+
+
 def config_c4(seed=23, n_scenes=32, edge_range=(32, 512),
-              ray_range=(1 << 16, 1 << 23), max_edge=None, ray_scale=1.0):
+              ray_range=(1 << 16, 1 << 23), max_edge=None, ray_scale=1.0,
+              workers=None):
     """32 scenes of uneven size: edges log-uniform in edge_range, a mix of
     value-noise volumes (1-40% occupancy), hollow sphere shells (1-40%)
     and level-4 Menger sponges; rays per scene log-uniform in ray_range,
-    half from a pinhole camera looking at the scene, half random."""
+    half from a pinhole camera looking at the scene, half random.  The
+    scenes' parameters are drawn in order from one generator; the volumes
+    are then built on `workers` threads (default: the usable cores, at most
+    16), which does not change them."""
+    import os
+    from concurrent.futures import ThreadPoolExecutor
     rng = np.random.default_rng(seed)
-    out = []
+    params = []
     for s in range(n_scenes):
         edge = int(round(np.exp(rng.uniform(np.log(edge_range[0]),
                                             np.log(edge_range[1])))))
@@ -254,30 +288,15 @@ def config_c4(seed=23, n_scenes=32, edge_range=(32, 512),
         nrays = int(np.exp(rng.uniform(np.log(ray_range[0]),
                                        np.log(ray_range[1]))) * ray_scale)
         nrays = max(64, nrays)
-        if kind == "noise":
-            vol = noise_volume(edge, occ, seed * 1000 + s)
-        elif kind == "shell":
-            vol = shell_volume(edge, occ)
-        else:
-            vol = menger_volume(edge)
-        dots = volume_to_dots(vol)
-        occ_real = float(vol.mean())
-        del vol
-        bmin, bmax = bbox_of(dots) if len(dots) else (np.zeros(3),
-                                                      np.ones(3) * edge)
-        ncam = nrays // 2
-        W = max(16, int(np.sqrt(ncam * 16 / 9)))
-        H = max(1, ncam // W)
-        center = (bmin + bmax) / 2.0
         a = rng.uniform(0, 2 * np.pi)
-        dist = 1.5 * np.sqrt(3.0) * edge
-        pos = center + dist * np.array([np.cos(a) * 0.94, np.sin(a) * 0.94,
-                                        0.34])
-        cam = _camera.make_camera(pos, center, np.tan(np.radians(30)), W, H)
-        oc, dc = camera_rays(cam, W, H)
-        orr, drr = random_rays(nrays - len(oc), bmin, bmax,
-                               seed=seed * 1000 + 500 + s)
-        out.append(dict(kind=kind, edge=edge, occupancy=occ_real, dots=dots,
-                        o=np.concatenate([oc, orr]),
-                        d=np.concatenate([dc, drr]), camera_pixels=len(oc)))
+        params.append((kind, edge, occ, nrays, a, seed, s))
+    if workers is None:
+        workers = min(16, len(os.sched_getaffinity(0)))
+    # the largest volumes first, so that the pool finishes together
+    order = sorted(range(n_scenes), key=lambda i: -params[i][1])
+    out = [None] * n_scenes
+    with ThreadPoolExecutor(max_workers=max(1, workers)) as ex:
+        futs = {i: ex.submit(_c4_scene, *params[i]) for i in order}
+        for i, f in futs.items():
+            out[i] = f.result()
     return out
