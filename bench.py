@@ -27,6 +27,7 @@ METRIC = ("Mrays/s coherent+incoherent on 256³ scene @1080p; "
           "batched vs plain; p99 latency")
 HBM_PEAK_GBS = 8000.0
 BYTES_PER_RAY, BYTES_PER_VISIT, BYTES_PER_TEST = 40, 32, 12
+BYTES_PER_BALL = 17  # centre 12 B + radius 4 B in, 1 B out
 QUALITY_NAMES = {0: "best", 1: "fast", 2: "adaptive",
                  6: "adaptive_ray_merge", 10: "adaptive_ray_merge_accurate"}
 
@@ -84,6 +85,10 @@ def cpu_baseline(dots, cams, W, H, budget_s):
     T.close()
     return {"value": rays / dt / 1e6, "unit": "Mrays/s", "cores": cores,
             "kind": "port",
+            "build": "gcc -O3 -march=x86-64-v3 -ffp-contract=off "
+                     "(oracle/Makefile: portable AVX2+FMA, so the library "
+                     "built here runs on the GPU host; the reference's "
+                     "RELEASE build is clang -O3 -march=native)",
             "sample": "%d of the 64 poses at %dx%d (%.1f s, %d threads, "
                       "oracle vox_render loop, BEST); 1 core: %.2f Mrays/s"
                       % (frames, W, H, dt, cores, W * H / dt1 / 1e6),
@@ -376,6 +381,124 @@ def serving_leg(log, seconds, rate, seed=31):
     return res
 
 
+def c3_leg(G, bmin, bmax, ev0, ev1, stream, log, tune=False):
+    """The incoherent workload on the headline scene: 16 M rays with
+    origins uniform in the box grown by 10% of its extent per side and
+    N(0, I) directions (voxg_ray_batch), and 4 M ball queries with radii
+    U[0.5, 8] (voxg_ball_batch).  Each leg gets the roofline of its kernel:
+    algorithmic bytes = 40 B per ray (17 B per ball) + 32 B per node visit
+    + 12 B per voxel test, visits and tests counted by the counting launch
+    on the same queries, over the launch's event time."""
+    import ctypes as C
+    import numpy as np
+    import torch
+    from voxvision_amd import _lib, scenes
+    L = _lib.load()
+    sp = C.c_void_p(stream.cuda_stream)
+    res = {}
+    n = 1 << 24
+    o, d = scenes.random_rays(n, bmin, bmax, seed=11)
+    so = [torch.from_numpy(np.ascontiguousarray(o[:, k])).cuda()
+          for k in range(3)]
+    sd = [torch.from_numpy(np.ascontiguousarray(d[:, k])).cuda()
+          for k in range(3)]
+    vo = torch.empty(n, dtype=torch.int32, device="cuda")
+    ho = [torch.empty(n, device="cuda") for _ in range(3)]
+
+    def inc():
+        _lib.check(L.voxg_ray_batch(
+            G.handle, n, *[t.data_ptr() for t in so + sd], vo.data_ptr(),
+            None, *[t.data_ptr() for t in ho], sp), "voxg_ray_batch")
+    if tune:
+        from voxvision_amd import autotune
+        icfg, _, _ = autotune.autotune(
+            lambda cfg: inc, params=["leaf_batch", "finish_batch", "chunk",
+                                     "refill"], reps=2, kind="rays")
+        res["incoherent_launch"] = icfg
+    inc()
+    torch.cuda.synchronize()
+    ev0.record(stream)
+    for _ in range(5):
+        inc()
+    ev1.record(stream)
+    torch.cuda.synchronize()
+    ims = ev0.elapsed_time(ev1) / 5
+    cnt = _lib.Counts()
+    _lib.check(L.voxg_ray_batch_count(
+        G.handle, n, *[t.data_ptr() for t in so + sd], C.byref(cnt), sp),
+        "voxg_ray_batch_count")
+    alg = (BYTES_PER_RAY * n + BYTES_PER_VISIT * cnt.node_visits +
+           BYTES_PER_TEST * cnt.voxel_tests)
+    tr = load_traffic("traffic_ct256_c3.json")
+    res["incoherent_16M"] = {
+        "ms": round(ims, 4), "mrays_s": round(n / ims / 1e3, 1),
+        "roofline": {
+            "bound": "hbm", "kernel": "rti_kernel<4, ChunkWork>",
+            "achieved": round(alg / (ims / 1e3) / 1e9, 2),
+            "peak": HBM_PEAK_GBS, "unit": "GB/s",
+            "frac": round(alg / (ims / 1e3) / 1e9 / HBM_PEAK_GBS, 5),
+            "traffic": tr.get("rays_hbm_bytes_per_launch") if tr else None,
+            "algorithmic_bytes_per_launch": alg,
+            "node_visits_per_ray": round(cnt.node_visits / n, 4),
+            "voxel_tests_per_ray": round(cnt.voxel_tests / n, 4),
+            "stack_spills": int(cnt.stack_spills)}}
+    del so, sd, vo, ho
+    nb = 1 << 22
+    cb, rb = scenes.random_balls(nb, bmin, bmax, seed=13)
+    cbt = [torch.from_numpy(np.ascontiguousarray(cb[:, k])).cuda()
+           for k in range(3)]
+    rbt = torch.from_numpy(rb).cuda()
+    bo = torch.empty(nb, dtype=torch.uint8, device="cuda")
+
+    def balls():
+        _lib.check(L.voxg_ball_batch(
+            G.handle, nb, *[t.data_ptr() for t in cbt], rbt.data_ptr(),
+            bo.data_ptr(), sp), "voxg_ball_batch")
+    balls()
+    torch.cuda.synchronize()
+    ev0.record(stream)
+    for _ in range(5):
+        balls()
+    ev1.record(stream)
+    torch.cuda.synchronize()
+    bms = ev0.elapsed_time(ev1) / 5
+    _lib.check(L.voxg_ball_batch_count(
+        G.handle, nb, *[t.data_ptr() for t in cbt], rbt.data_ptr(),
+        C.byref(cnt), sp), "voxg_ball_batch_count")
+    alg = (BYTES_PER_BALL * nb + BYTES_PER_VISIT * cnt.node_visits +
+           BYTES_PER_TEST * cnt.voxel_tests)
+    res["balls_4M"] = {
+        "ms": round(bms, 4), "mqueries_s": round(nb / bms / 1e3, 1),
+        "hit_rate": round(float(bo.float().mean()), 4),
+        "roofline": {
+            "bound": "hbm", "kernel": "ball_kernel",
+            "achieved": round(alg / (bms / 1e3) / 1e9, 2),
+            "peak": HBM_PEAK_GBS, "unit": "GB/s",
+            "frac": round(alg / (bms / 1e3) / 1e9 / HBM_PEAK_GBS, 5),
+            "traffic": tr.get("balls_hbm_bytes_per_launch") if tr else None,
+            "algorithmic_bytes_per_launch": alg,
+            "node_visits_per_ball": round(cnt.node_visits / nb, 4),
+            "voxel_tests_per_ball": round(cnt.voxel_tests / nb, 4)}}
+    log("[bench] incoherent leg: %s" % json.dumps(res))
+    return res
+
+
+def fp64_accuracy():
+    """The hit accuracy of the fp32 search against FP64, as recorded by
+    tests/golden/make_fp64_accuracy.py (the GPU's results equal the CPU
+    restatement's bit for bit, which the GPU tests check)."""
+    p = os.path.join(ROOT, "tests", "golden", "fp64_accuracy.json")
+    if not os.path.exists(p):
+        return None
+    r = json.load(open(p))
+    return {k: {f: v[f] for f in ("rays", "tie_fraction",
+                                  "voxel_agree_outside_ties",
+                                  "hit_error_voxels_median",
+                                  "hit_error_voxels_max",
+                                  "hit_error_rel_max")}
+            for k, v in r.items() if isinstance(v, dict)}
+
+
 def load_traffic(name):
     p = os.path.join(ROOT, "profiles", name)
     if not os.path.exists(p):
@@ -406,8 +529,18 @@ def main():
                     help="render quality of the timed step (0 BEST, the "
                          "headline; 1 FAST, 2 ADAPTIVE, 6 / 10 with ray "
                          "merging), for profiling the other modes")
+    ap.add_argument("--tune", action="store_true",
+                    help="autotune the headline render (and the incoherent "
+                         "rays) before timing; by default the library's "
+                         "default launch configurations, which the "
+                         "committed profiles were taken at, are timed")
     ap.add_argument("--no-tune", action="store_true",
-                    help="use the library's default launch configuration")
+                    help="the default (kept for older command lines)")
+    ap.add_argument("--legs", default="modes,c3,aux",
+                    help="extra legs besides the headline step, comma "
+                         "separated: modes (other render qualities), c3 "
+                         "(16 M incoherent rays, 4 M balls), aux (calls "
+                         "around the path); --quick runs none")
     ap.add_argument("--copy-calib-mib", type=int, default=0,
                     help="run a plain device copy of this size first (the "
                          "reference kernel for HBM counter calibration)")
@@ -443,7 +576,7 @@ def main():
         voxrnd.render(G, cams, W, H, quality=args.quality, out=out)
 
     tuned = None
-    if not args.no_tune:
+    if args.tune:
         from voxvision_amd import autotune
         cfg, best_ms, hist = autotune.autotune(
             lambda cfg: step, params=["stack_lds", "waves_per_cu", "refill",
@@ -504,7 +637,8 @@ def main():
                 "voxel_tests_per_ray": round(cnt["voxel_tests"] / n, 4)}
 
     extra = {"hit_rate": round(hits / n, 4)}
-    if not args.quick:
+    legs = set() if args.quick else set(x for x in args.legs.split(",") if x)
+    if "modes" in legs:
         # the other quality modes on the same frames and outputs (default
         # configurations of their kinds); ADAPTIVE is the reference's
         # default (renderer.c:78)
@@ -524,60 +658,18 @@ def main():
                            "mrays_s": round(n / qms / 1e3, 1)}
         extra["render_modes"] = modes
         log("[bench] other quality modes: %s" % json.dumps(modes))
+    if "c3" in legs:
         bmin, bmax = c2["bbox"]
-        o, d = scenes.random_rays(1 << 24, bmin, bmax, seed=11)
-        so = [torch.from_numpy(np.ascontiguousarray(o[:, k])).cuda()
-              for k in range(3)]
-        sd = [torch.from_numpy(np.ascontiguousarray(d[:, k])).cuda()
-              for k in range(3)]
-        vo = torch.empty(1 << 24, dtype=torch.int32, device="cuda")
-        ho = [torch.empty(1 << 24, device="cuda") for _ in range(3)]
-        import ctypes as C
-
-        def inc():
-            _lib.check(L.voxg_ray_batch(
-                G.handle, 1 << 24, *[t.data_ptr() for t in so + sd],
-                vo.data_ptr(), None, *[t.data_ptr() for t in ho],
-                C.c_void_p(stream.cuda_stream)), "voxg_ray_batch")
-        if not args.no_tune:
-            from voxvision_amd import autotune
-            icfg, _, _ = autotune.autotune(
-                lambda cfg: inc, params=["leaf_batch", "finish_batch", "chunk",
-                                         "refill"], reps=2, kind="rays")
-            extra["incoherent_launch"] = icfg
-        inc()
-        torch.cuda.synchronize()
-        ev0.record(stream)
-        for _ in range(5):
-            inc()
-        ev1.record(stream)
-        torch.cuda.synchronize()
-        ims = ev0.elapsed_time(ev1) / 5
-        extra["incoherent_16M_mrays_s"] = round((1 << 24) / ims / 1e3, 1)
-        cb, rb = scenes.random_balls(1 << 22, bmin, bmax, seed=13)
-        cbt = [torch.from_numpy(np.ascontiguousarray(cb[:, k])).cuda()
-               for k in range(3)]
-        rbt = torch.from_numpy(rb).cuda()
-        bo = torch.empty(1 << 22, dtype=torch.uint8, device="cuda")
-
-        def balls():
-            _lib.check(L.voxg_ball_batch(
-                G.handle, 1 << 22, *[t.data_ptr() for t in cbt],
-                rbt.data_ptr(), bo.data_ptr(),
-                C.c_void_p(stream.cuda_stream)), "voxg_ball_batch")
-        balls()
-        torch.cuda.synchronize()
-        ev0.record(stream)
-        for _ in range(5):
-            balls()
-        ev1.record(stream)
-        torch.cuda.synchronize()
-        bms = ev0.elapsed_time(ev1) / 5
-        extra["balls_4M_mqueries_s"] = round((1 << 22) / bms / 1e3, 1)
-
-    if not args.quick:
+        c3 = c3_leg(G, bmin, bmax, ev0, ev1, stream, log, tune=args.tune)
+        extra.update(c3)
+        extra["incoherent_16M_mrays_s"] = c3["incoherent_16M"]["mrays_s"]
+        extra["balls_4M_mqueries_s"] = c3["balls_4M"]["mqueries_s"]
+    if "aux" in legs:
         extra["around_the_path"] = aux_leg(G, dots, W, H, nf, ev0, ev1,
                                            stream, log)
+    acc = fp64_accuracy()
+    if acc:
+        extra["fp64_accuracy"] = acc
     if not args.no_batched:
         extra["batched"] = batched_leg(ev0, ev1, stream, log)
     if not args.no_serving:
