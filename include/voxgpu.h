@@ -235,14 +235,52 @@ void voxg_camera_set_window_size(voxg_camera *cam, int w, int h);
 #define VOXG_QUALITY_RAY_MERGE 0x4
 #define VOXG_QUALITY_RAY_MERGE_ACCURATE 0x8
 
+/* ---- light manager (src/voxrnd/lights.h, lights.c; src/voxtrees/mtree.c)
+ * An ambient light and shadowless light spheres, kept in the reference's
+ * M-tree so that the lights containing a point are summed in the
+ * reference's order.  Host object; a render that uses it uploads its
+ * current state to the device first (synchronously, when it changed since
+ * the last upload), so change lights outside graph capture.  Not
+ * thread-safe per handle. */
+typedef struct voxg_lights voxg_lights;
+/* vox_make_light_manager: no lights, ambient light (1, 1, 1). */
+int voxg_lights_create(voxg_lights **out);
+/* vox_destroy_light_manager (waits for the device). */
+int voxg_lights_release(voxg_lights *lights);
+/* vox_set_ambient_light: *set = 1 when every component is in [0, 1] (the
+ * light changed), else 0; set may be NULL. */
+int voxg_lights_set_ambient(voxg_lights *lights, const float color[3],
+                            int *set);
+/* vox_insert_shadowless_light: *inserted = 1 when added, 0 when the
+ * reference refuses it (a colour component outside [0, 1], radius not
+ * positive, the same sphere present).  VOXG_EINVAL for what the
+ * reference's M-tree cannot hold (a light with this centre present,
+ * src/voxtrees/mtree.h) or a non-finite centre or radius. */
+int voxg_lights_insert(voxg_lights *lights, const float center[3],
+                       float radius, const float color[3], int *inserted);
+/* vox_delete_shadowless_light: *deleted = 1 when the sphere was there. */
+int voxg_lights_delete(voxg_lights *lights, const float center[3],
+                       float radius, int *deleted);
+/* vox_delete_shadowless_lights */
+int voxg_lights_clear(voxg_lights *lights);
+/* vox_shadowless_lights_number */
+size_t voxg_lights_count(const voxg_lights *lights);
+/* vox_get_light at n points (host): rgb[3i..3i+2] for xyz[3i..3i+2]. */
+int voxg_lights_get(const voxg_lights *lights, size_t n, const float *xyz,
+                    float *rgb);
+
 typedef struct {
     unsigned quality;
-    /* Colour (renderer.c:27-52 without lights): device pointer to a 32^3
-     * byte texture indexed x*1024 + y*32 + z, mapped to a 32-bit pixel as
-     * (c<<rshift)|(c<<gshift)|(c<<bshift)|amask.  texture may be NULL when
-     * squares is NULL. */
+    /* Colour (renderer.c:27-52): device pointer to a 32^3 byte texture
+     * indexed x*1024 + y*32 + z; without lights a texel c maps to the
+     * 32-bit pixel (c<<rshift)|(c<<gshift)|(c<<bshift)|amask, with lights
+     * each channel is min(light, 1) * c truncated to 8 bits (the light of
+     * voxg_lights_get at the hit point).  texture may be NULL when squares
+     * is NULL. */
     const uint8_t *texture;
     uint32_t rshift, gshift, bshift, amask;
+    /* vox_context_set_light_manager (renderer.h:184): NULL = no lights */
+    const voxg_lights *lights;
 } voxg_render_params;
 
 /* The vox_render per-pixel loop for n_frames camera poses at W x H

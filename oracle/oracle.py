@@ -51,7 +51,12 @@ class _Camera(C.Structure):
 class _Shading(C.Structure):
     _fields_ = [("texture", C.c_void_p), ("rshift", C.c_uint32),
                 ("gshift", C.c_uint32), ("bshift", C.c_uint32),
-                ("amask", C.c_uint32)]
+                ("amask", C.c_uint32), ("lights", C.c_void_p)]
+
+
+class _Sphere(C.Structure):
+    _fields_ = [("center", C.c_float * 3), ("color", C.c_float * 3),
+                ("radius", C.c_float), ("sqr_radius", C.c_float)]
 
 
 class _RenderStats(C.Structure):
@@ -115,6 +120,19 @@ def lib():
                                        C.POINTER(C.c_uint8), F]
         L.orc_ray_max_pending.argtypes = [P, C.c_size_t] + [F] * 6 + [
             C.POINTER(C.c_uint32)]
+        L.orc_lights_create.restype = P
+        L.orc_lights_destroy.argtypes = [P]
+        L.orc_lights_set_ambient.argtypes = [P, F]
+        L.orc_lights_insert.argtypes = [P, F, C.c_float, F]
+        L.orc_lights_delete.argtypes = [P, F, C.c_float]
+        L.orc_lights_clear.argtypes = [P]
+        L.orc_lights_count.argtypes = [P]
+        L.orc_lights_count.restype = C.c_size_t
+        L.orc_get_light.argtypes = [P, F, F]
+        L.orc_lights_spheres.argtypes = [P, C.POINTER(_Sphere), C.c_size_t]
+        L.orc_lights_spheres.restype = C.c_size_t
+        L.orc_lights_height.argtypes = [P]
+        L.orc_lights_height.restype = C.c_uint
         L.orc_tree_inner_nodes.argtypes = [P, F, C.POINTER(C.c_uint64),
                                            C.c_size_t]
         L.orc_tree_inner_nodes.restype = C.c_size_t
@@ -302,11 +320,14 @@ class Tree:
         return out
 
     def render(self, cam, W, H, quality=QUALITY_BEST, texture=None,
-               fmt=(16, 8, 0, 0xFF000000), nthreads=0, squares=None):
-        """Returns (squares, voxel_px, hit_px, stats) in the square layout."""
+               fmt=(16, 8, 0, 0xFF000000), nthreads=0, squares=None,
+               lights=None):
+        """Returns (squares, voxel_px, hit_px, stats) in the square layout;
+        lights: a Lights object (the context's light manager) or None."""
         set_voxel(self.voxel)
         c = camera_struct(cam)
         sh = _Shading()
+        sh.lights = lights.ptr if lights is not None else None
         tex = None
         if texture is not None:
             tex = np.ascontiguousarray(texture, dtype=np.uint8)
@@ -329,6 +350,64 @@ class Tree:
         stats["node_visits"] = st.c.node_visits
         stats["voxel_tests"] = st.c.voxel_tests
         return squares, vox, hit, stats
+
+
+class Lights:
+    """The oracle's light manager (src/voxrnd/lights.c): an M-tree of
+    shadowless light spheres and an ambient light."""
+
+    def __init__(self):
+        self.ptr = lib().orc_lights_create()
+
+    def set_ambient(self, color):
+        return int(lib().orc_lights_set_ambient(self.ptr, _fp(_f32(color))))
+
+    def insert(self, center, radius, color):
+        return int(lib().orc_lights_insert(self.ptr, _fp(_f32(center)),
+                                           float(radius), _fp(_f32(color))))
+
+    def delete(self, center, radius):
+        return int(lib().orc_lights_delete(self.ptr, _fp(_f32(center)),
+                                           float(radius)))
+
+    def clear(self):
+        lib().orc_lights_clear(self.ptr)
+
+    def __len__(self):
+        return int(lib().orc_lights_count(self.ptr))
+
+    def height(self):
+        return int(lib().orc_lights_height(self.ptr))
+
+    def light(self, p):
+        """vox_get_light at each of the (n, 3) points."""
+        p = _f32(p).reshape(-1, 3)
+        out = np.zeros((len(p), 3), np.float32)
+        for i in range(len(p)):
+            lib().orc_get_light(self.ptr, _fp(np.ascontiguousarray(p[i])),
+                                out[i].ctypes.data_as(C.POINTER(C.c_float)))
+        return out
+
+    def spheres(self):
+        """(centers (k, 3), colors (k, 3), radii (k,)) in walk order."""
+        k = lib().orc_lights_spheres(self.ptr, None, 0)
+        arr = (_Sphere * max(1, k))()
+        lib().orc_lights_spheres(self.ptr, arr, k)
+        c = np.array([list(a.center) for a in arr[:k]], np.float32)
+        col = np.array([list(a.color) for a in arr[:k]], np.float32)
+        r = np.array([a.radius for a in arr[:k]], np.float32)
+        return c.reshape(-1, 3), col.reshape(-1, 3), r
+
+    def close(self):
+        if self.ptr:
+            lib().orc_lights_destroy(self.ptr)
+            self.ptr = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
 
 
 def render_trace(tree, cam, W, H, quality, nthreads=0):

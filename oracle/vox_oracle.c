@@ -989,6 +989,18 @@ static uint32_t get_color(const orc_shading *sh, const orc_dot inter)
     z = abs((int)(inter[2] / orc_voxel[2])) & 31;
     idx = x * 32 * 32 + y * 32 + z;
     c = sh->texture[idx];
+    if (sh->lights) {
+        /* renderer.c:40-49: each channel is min(light, 1) * colour,
+         * truncated to 8 bits */
+        float light[3];
+        uint32_t r, g, b;
+        orc_get_light(sh->lights, inter, light);
+        r = (uint32_t)(uint8_t)(fminf(light[0], 1.0f) * (float)c);
+        g = (uint32_t)(uint8_t)(fminf(light[1], 1.0f) * (float)c);
+        b = (uint32_t)(uint8_t)(fminf(light[2], 1.0f) * (float)c);
+        return (r << sh->rshift) | (g << sh->gshift) | (b << sh->bshift) |
+               sh->amask;
+    }
     return (c << sh->rshift) | (c << sh->gshift) | (c << sh->bshift) |
            sh->amask;
 }

@@ -153,9 +153,11 @@ void orc_screen2world(const orc_camera *cam, int sx, int sy, orc_dot ray);
 #define ORC_QUALITY_RAY_MERGE 4
 #define ORC_QUALITY_RAY_MERGE_ACCURATE 8
 
+typedef struct orc_lights orc_lights;
 typedef struct {
     const uint8_t *texture; /* 32^3 bytes, index x*1024 + y*32 + z */
     uint32_t rshift, gshift, bshift, amask; /* SDL_MapRGB for a 32-bit format */
+    const orc_lights *lights; /* NULL: no light manager (renderer.c:38) */
 } orc_shading;
 
 typedef struct {
@@ -203,6 +205,27 @@ void orc_ray_max_pending(const struct orc_node *tree, size_t n,
 /* Ray directions of every pixel of a W x H frame (orc_screen2world), as 3
  * planes of W*H floats in the square layout. */
 void orc_camera_rays(const orc_camera *cam, int W, int H, float *dxyz);
+
+/* ---- lights: src/voxrnd/lights.c, src/voxtrees/mtree.c ---- */
+typedef struct {
+    float center[3];
+    float color[3];
+    float radius, sqr_radius;
+} orc_sphere;
+orc_lights *orc_lights_create(void);                 /* ambient 1,1,1 */
+void orc_lights_destroy(orc_lights *l);
+int orc_lights_set_ambient(orc_lights *l, const float color[3]);
+int orc_lights_insert(orc_lights *l, const float center[3], float radius,
+                      const float color[3]);
+int orc_lights_delete(orc_lights *l, const float center[3], float radius);
+void orc_lights_clear(orc_lights *l);
+size_t orc_lights_count(const orc_lights *l);
+/* vox_get_light: ambient plus, for each sphere containing p in the
+ * M-tree's walk order, (1 - min(r, |p - c|) / r) * colour. */
+void orc_get_light(const orc_lights *l, const float p[3], float light[3]);
+/* Spheres in the M-tree's depth-first order; returns the count. */
+size_t orc_lights_spheres(const orc_lights *l, orc_sphere *out, size_t cap);
+unsigned orc_lights_height(const orc_lights *l);
 
 /* ---- FP64 brute force (independent of the tree) ---- */
 /* For each ray: first voxel hit along o + t d, t >= 0 (closed boxes, an

@@ -356,3 +356,66 @@ def test_stack_bound_covers_the_search(voxlib, oracle, name):
     assert int(mp.max()) <= info["stack_entries"]
     assert int(mp.max()) >= 2
     T.close()
+
+
+def _light_ops(seed, n_ins=300):
+    rng = np.random.default_rng(seed)
+    ops = []
+    live = []
+    for k in range(n_ins):
+        c = rng.uniform(0, 64, 3).astype(np.float32)
+        if k % 7 == 3 and live:       # a duplicate sphere: refused (0)
+            c, r = live[int(rng.integers(0, len(live)))]
+            ops.append(("ins", c, r, rng.uniform(0, 1, 3)))
+            continue
+        r = np.float32(rng.uniform(1, 20))
+        col = rng.uniform(0, 1, 3)
+        if k % 11 == 5:
+            col[1] = 1.5               # colour outside [0, 1]: refused
+        ops.append(("ins", c, r, col))
+        if col[1] <= 1:
+            live.append((c, r))
+        if k % 3 == 2 and live:        # delete one present, one absent
+            i = int(rng.integers(0, len(live)))
+            ops.append(("del", live[i][0], live[i][1]))
+            del live[i]
+            ops.append(("del", c + 100, r))
+    return ops
+
+
+@pytest.mark.parametrize("seed", [1, 2])
+def test_lights_match_the_oracle(voxlib, oracle, seed):
+    """The library's light manager keeps the reference's M-tree
+    (src/voxtrees/mtree.c) node for node: after every step of a long
+    sequence of inserts (with node splits) and deletes (with merges of
+    parents left with one child), the light at random points
+    (vox_get_light, src/voxrnd/lights.c:103-116) equals the oracle's bit
+    for bit, and both refuse the same inserts and deletes."""
+    from voxvision_amd.lights import Lights
+    G, O = Lights(), oracle.Lights()
+    assert G.set_ambient((0.2, 0.3, 0.4)) == O.set_ambient((0.2, 0.3, 0.4))
+    assert G.set_ambient((0.2, 1.3, 0.4)) == O.set_ambient((0.2, 1.3, 0.4))
+    rng = np.random.default_rng(seed + 10)
+    pts = rng.uniform(-5, 70, (400, 3)).astype(np.float32)
+    for k, op in enumerate(_light_ops(seed)):
+        if op[0] == "ins":
+            assert G.insert(op[1], op[2], op[3]) == O.insert(op[1], op[2],
+                                                             op[3])
+        else:
+            assert G.delete(op[1], op[2]) == O.delete(op[1], op[2])
+        assert len(G) == len(O)
+        if k % 25 == 0:
+            assert np.array_equal(G.light(pts).view(np.uint32),
+                                  O.light(pts).view(np.uint32)), k
+    assert O.height() >= 3
+    assert np.array_equal(G.light(pts).view(np.uint32),
+                          O.light(pts).view(np.uint32))
+    # what the reference's M-tree cannot hold is refused loudly
+    c, _, r = O.spheres()
+    with pytest.raises(Exception):
+        G.insert(c[0], float(r[0]) + 1.0, (1, 1, 1))
+    assert O.insert(c[0], float(r[0]) + 1.0, (1, 1, 1)) == -1
+    G.clear()
+    O.clear()
+    assert len(G) == 0 and len(O) == 0
+    assert np.array_equal(G.light(pts), O.light(pts))

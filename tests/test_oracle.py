@@ -881,3 +881,35 @@ def test_fp64_accuracy_on_the_benchmark_workloads(oracle):
         assert r["tie_fraction"] < 0.01, name
         assert r["hit_error_rel_max"] <= 1e-5, name
         assert r == want[name], name
+
+
+def test_lights_against_fp64(oracle):
+    """vox_get_light (src/voxrnd/lights.c:91-116) in the oracle against an
+    FP64 sum over every sphere containing the point, found without the
+    M-tree: ambient + sum of (1 - min(r, |p - c|) / r) * colour.  Agreement
+    within 2e-6 (fp32 rounding of a few terms) outside a 1e-5 band around
+    the spheres' surfaces, and the M-tree walk finds every containing
+    sphere (a missed or extra sphere changes the light by a whole term)."""
+    rng = np.random.default_rng(3)
+    L = oracle.Lights()
+    L.set_ambient((0.1, 0.2, 0.3))
+    for _ in range(150):
+        assert L.insert(rng.uniform(0, 64, 3), rng.uniform(2, 18),
+                        rng.uniform(0, 1, 3)) == 1
+    assert L.height() >= 2
+    c, col, r = L.spheres()
+    assert len(r) == 150
+    p = rng.uniform(-5, 70, (5000, 3)).astype(np.float32)
+    got = L.light(p).astype(np.float64)
+    d = np.sqrt(((p[:, None, :].astype(np.float64) -
+                  c[None].astype(np.float64)) ** 2).sum(2))
+    r64 = r[None].astype(np.float64)
+    inside = d < r64
+    near = np.abs(d - r64) < 1e-5 * 64
+    add = np.where(inside, 1 - np.minimum(r64, d) / r64, 0.0)
+    want = np.array([0.1, 0.2, 0.3]) + (add[:, :, None] *
+                                        col[None].astype(np.float64)).sum(1)
+    ok = ~near.any(1)
+    assert ok.mean() > 0.99
+    assert np.abs(got[ok] - want[ok]).max() < 2e-6
+    assert (inside.sum(1) >= 2).mean() > 0.3  # overlapping lights exercised

@@ -36,6 +36,9 @@ EXPORTS = [
     "voxg_edit_commit", "voxg_edit_release", "voxg_server_replace_tree",
     "voxg_synchronize", "voxg_release_stream", "voxg_test_fail_at",
     "voxg_test_stack_bound", "voxg_ray_batch_noalloc",
+    "voxg_lights_create", "voxg_lights_release", "voxg_lights_set_ambient",
+    "voxg_lights_insert", "voxg_lights_delete", "voxg_lights_clear",
+    "voxg_lights_count", "voxg_lights_get",
 ]
 
 
@@ -69,7 +72,8 @@ class Camera(C.Structure):
 class RenderParams(C.Structure):
     _fields_ = [("quality", C.c_uint), ("texture", C.c_void_p),
                 ("rshift", C.c_uint32), ("gshift", C.c_uint32),
-                ("bshift", C.c_uint32), ("amask", C.c_uint32)]
+                ("bshift", C.c_uint32), ("amask", C.c_uint32),
+                ("lights", C.c_void_p)]
 
 
 class BatchItem(C.Structure):
@@ -156,6 +160,15 @@ def load():
     L.voxg_edit_release.argtypes = [P]
     L.voxg_synchronize.argtypes = [P]
     L.voxg_release_stream.argtypes = [P]
+    L.voxg_lights_create.argtypes = [C.POINTER(P)]
+    L.voxg_lights_release.argtypes = [P]
+    L.voxg_lights_set_ambient.argtypes = [P, P, C.POINTER(C.c_int)]
+    L.voxg_lights_insert.argtypes = [P, P, C.c_float, P, C.POINTER(C.c_int)]
+    L.voxg_lights_delete.argtypes = [P, P, C.c_float, C.POINTER(C.c_int)]
+    L.voxg_lights_clear.argtypes = [P]
+    L.voxg_lights_count.argtypes = [P]
+    L.voxg_lights_count.restype = S
+    L.voxg_lights_get.argtypes = [P, S, P, P]
     L.voxg_test_fail_at.argtypes = [I]
     L.voxg_test_stack_bound.argtypes = [C.c_uint]
     _lib = L

@@ -19,7 +19,7 @@ BUILD = os.path.join(PKG, "_build")
 
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 HIP_SOURCES = ["voxgpu.hip", "volume.hip"]
-CXX_SOURCES = ["tree_build.cpp", "tree_edit.cpp"]
+CXX_SOURCES = ["tree_build.cpp", "tree_edit.cpp", "lights.cpp"]
 ARCH = "gfx950"
 
 HIP_FLAGS = [

@@ -41,6 +41,7 @@
 
 #include "cull.h"
 #include "fastdiv.h"
+#include "lights.h"
 #include "tree_build.h"
 #include "voxg_internal.h"
 #include "voxg_layout.h"
@@ -283,18 +284,73 @@ __device__ __forceinline__ f3 screen2world(const CamDev &c, int sx, int sy) {
               __builtin_fmaf(c.qw, t3, vz) + c3};
 }
 
+__device__ __forceinline__ float dist2(const float *c, f3 p) {
+    float x = c[0] - p.x, y = c[1] - p.y, z = c[2] - p.z;
+    x = x * x;
+    y = y * y;
+    z = z * z;
+    return (x + y) + z;
+}
+
 struct Shade {
     const uint8_t *tex;
     uint32_t rshift, gshift, bshift, amask;
     float vx, vy, vz;
     float ivx, ivy, ivz;
-    // renderer.c:27-52 without a light manager
+    // light manager (renderer.c:38-49), or lnodes null: none
+    const LightNode *lnodes;
+    const LightRec *lsph;
+    uint32_t nlnodes;
+    float amb[3];
+    // vox_get_light (lights.c:103-116): the ambient light plus, for each
+    // light sphere containing p in the order of the M-tree walk
+    // (mtree.c:463-491: depth first, a one-sphere leaf reporting its sphere
+    // on the leaf's test), (1 - min(r, |p - c|) / r) * colour, summed with
+    // one fused multiply-add a channel (lights.c:91-101)
+    __device__ __forceinline__ void light(f3 p, float L[3]) const {
+        L[0] = amb[0];
+        L[1] = amb[1];
+        L[2] = amb[2];
+        uint32_t n = 0;
+        while (n < nlnodes) {
+            const LightNode nd = lnodes[n];
+            if (!(dist2(nd.c, p) < nd.sqr_radius)) {
+                n = nd.escape;
+                continue;
+            }
+            if (!(nd.info & 1u)) {  // inner: its first child is next
+                n++;
+                continue;
+            }
+            const uint32_t cnt = nd.info >> 8;
+            for (uint32_t k = 0; k < cnt; k++) {
+                const LightRec sp = lsph[nd.first + k];
+                if (cnt == 1u || dist2(sp.c, p) < sp.sqr_radius) {
+                    const float d = fminf(sp.radius, sqrtf(dist2(sp.c, p)));
+                    const float a = 1.0f - d / sp.radius;
+                    L[0] = __builtin_fmaf(a, sp.color[0], L[0]);
+                    L[1] = __builtin_fmaf(a, sp.color[1], L[1]);
+                    L[2] = __builtin_fmaf(a, sp.color[2], L[2]);
+                }
+            }
+            n = nd.escape;
+        }
+    }
+    // renderer.c:27-52: texture texel, scaled per channel by the light
+    // (min(light, 1) * texel, truncated to 8 bits) when there is a manager
     __device__ __forceinline__ uint32_t color(f3 p) const {
         int x = abs((int)div_v(p.x, vx, ivx)) & 31;
         int y = abs((int)div_v(p.y, vy, ivy)) & 31;
         int z = abs((int)div_v(p.z, vz, ivz)) & 31;
         uint32_t c = tex[x * 1024 + y * 32 + z];
-        return (c << rshift) | (c << gshift) | (c << bshift) | amask;
+        if (!lnodes) return (c << rshift) | (c << gshift) | (c << bshift) | amask;
+        float L[3];
+        light(p, L);
+        const float cf = (float)c;
+        const uint32_t r = (uint32_t)(fminf(L[0], 1.0f) * cf);
+        const uint32_t g = (uint32_t)(fminf(L[1], 1.0f) * cf);
+        const uint32_t b = (uint32_t)(fminf(L[2], 1.0f) * cf);
+        return (r << rshift) | (g << gshift) | (b << bshift) | amask;
     }
 };
 
@@ -1726,6 +1782,21 @@ namespace {
 struct Workspace;
 }
 
+// Light manager (src/voxrnd/lights.c): host M-tree and ambient light, and
+// a device copy of the flattened tree refreshed when a render needs it
+// after a change.
+struct voxg_lights {
+    voxg::LightTree tree;
+    float ambient[3] = {1.0f, 1.0f, 1.0f};  // lights.c:20
+    bool dirty = true;
+    int device = -1;
+    void *d_mem = nullptr;
+    size_t d_bytes = 0;
+    uint32_t nnodes = 0;
+    const voxg::LightNode *d_nodes = nullptr;
+    const voxg::LightRec *d_sph = nullptr;
+};
+
 struct voxg_batch {
     int device = 0;
     dev::BatchItem *d_items = nullptr;
@@ -2854,6 +2925,125 @@ static int render_launch(const voxg_tree *t, size_t n_frames,
 
 }  // extern "C++"
 
+// Device copy of the manager's flattened M-tree on the current device,
+// made again after a change (synchronous; a manager with no lights keeps
+// no device memory, and its walk ends at once).
+static int lights_upload(voxg_lights *l) {
+    int devid = 0;
+    HIPCHK(hipGetDevice(&devid));
+    if (!l->dirty && l->device == devid) return 0;
+    std::vector<voxg::LightNode> nodes;
+    std::vector<voxg::LightRec> sph;
+    try {
+        l->tree.flatten(nodes, sph);
+    } catch (const std::bad_alloc &) {
+        return fail(VOXG_ENOMEM, "host allocation failed");
+    }
+    const size_t nb = nodes.size() * sizeof(voxg::LightNode);
+    const size_t sb = sph.size() * sizeof(voxg::LightRec);
+    if (l->d_mem && (l->device != devid || nb + sb > l->d_bytes)) {
+        HIPCHK(hipDeviceSynchronize());  // renders may still read it
+        HIPCHK(hipFree(l->d_mem));
+        l->d_mem = nullptr;
+        l->d_bytes = 0;
+    }
+    if (nb && !l->d_mem) {
+        HIPCHK(hipMalloc(&l->d_mem, nb + sb));
+        l->d_bytes = nb + sb;
+    } else if (nb) {
+        HIPCHK(hipDeviceSynchronize());  // earlier renders read the old copy
+    }
+    if (nb) {
+        HIPCHK(hipMemcpy(l->d_mem, nodes.data(), nb, hipMemcpyHostToDevice));
+        if (sb)
+            HIPCHK(hipMemcpy((char *)l->d_mem + nb, sph.data(), sb,
+                             hipMemcpyHostToDevice));
+    }
+    l->d_nodes = (const voxg::LightNode *)l->d_mem;
+    l->d_sph = (const voxg::LightRec *)((char *)l->d_mem + nb);
+    l->nnodes = (uint32_t)nodes.size();
+    l->device = devid;
+    l->dirty = false;
+    return 0;
+}
+
+static bool light_color_ok(const float c[3]) {  // lights.c:81-89
+    for (int i = 0; i < 3; i++)
+        if (!(c[i] >= 0.0f && c[i] <= 1.0f)) return false;
+    return true;
+}
+
+int voxg_lights_create(voxg_lights **out) {
+    if (!out) return fail(VOXG_EINVAL, "out is NULL");
+    *out = new (std::nothrow) voxg_lights();
+    return *out ? 0 : fail(VOXG_ENOMEM, "host allocation failed");
+}
+
+int voxg_lights_release(voxg_lights *l) {
+    if (!l) return 0;
+    if (l->d_mem) {
+        HIPCHK(hipDeviceSynchronize());
+        HIPCHK(hipFree(l->d_mem));
+    }
+    delete l;
+    return 0;
+}
+
+int voxg_lights_set_ambient(voxg_lights *l, const float color[3], int *set) {
+    if (!l || !color) return fail(VOXG_EINVAL, "NULL argument");
+    const bool ok = light_color_ok(color);
+    if (ok) std::memcpy(l->ambient, color, sizeof(l->ambient));
+    if (set) *set = ok ? 1 : 0;
+    return 0;
+}
+
+int voxg_lights_insert(voxg_lights *l, const float center[3], float radius,
+                       const float color[3], int *inserted) {
+    if (!l || !center || !color) return fail(VOXG_EINVAL, "NULL argument");
+    int r = 0;
+    if (light_color_ok(color)) {
+        try {
+            r = l->tree.insert(center, radius, color);
+        } catch (const std::bad_alloc &) {
+            return fail(VOXG_ENOMEM, "host allocation failed");
+        }
+    }
+    if (r < 0)
+        return fail(VOXG_EINVAL, "a light with this centre exists, or the "
+                                 "centre or radius is not finite");
+    if (r) l->dirty = true;
+    if (inserted) *inserted = r;
+    return 0;
+}
+
+int voxg_lights_delete(voxg_lights *l, const float center[3], float radius,
+                       int *deleted) {
+    if (!l || !center) return fail(VOXG_EINVAL, "NULL argument");
+    const int r = l->tree.remove(center, radius);
+    if (r) l->dirty = true;
+    if (deleted) *deleted = r;
+    return 0;
+}
+
+int voxg_lights_clear(voxg_lights *l) {
+    if (!l) return fail(VOXG_EINVAL, "NULL argument");
+    l->tree.clear();
+    l->dirty = true;
+    return 0;
+}
+
+size_t voxg_lights_count(const voxg_lights *l) {
+    return l ? l->tree.count() : 0;
+}
+
+int voxg_lights_get(const voxg_lights *l, size_t n, const float *xyz,
+                    float *rgb) {
+    if (!l || (n && (!xyz || !rgb))) return fail(VOXG_EINVAL, "NULL argument");
+    for (size_t i = 0; i < n; i++)
+        l->tree.light(l->ambient, xyz + 3 * i, rgb + 3 * i);
+    return 0;
+}
+
 int voxg_render(const voxg_tree *t, size_t n_frames, const voxg_camera *cams,
                 int W, int H, const voxg_render_params *params,
                 uint32_t *squares, uint32_t *voxel_px, float *hit_px,
@@ -2863,6 +3053,15 @@ int voxg_render(const voxg_tree *t, size_t n_frames, const voxg_camera *cams,
     if (rc) return rc;
     if (n == 0) return 0;
     dev::PixSink sink = make_sink(t, params, squares, voxel_px, hit_px, n);
+    if (squares && params->lights) {
+        rc = lights_upload(const_cast<voxg_lights *>(params->lights));
+        if (rc) return rc;
+        const voxg_lights *l = params->lights;
+        sink.sh.lnodes = l->d_nodes;
+        sink.sh.lsph = l->d_sph;
+        sink.sh.nlnodes = l->nnodes;
+        for (int k = 0; k < 3; k++) sink.sh.amb[k] = l->ambient[k];
+    }
     return render_launch<false>(t, n_frames, cams, W, H, params, sink, n,
                                 nullptr, stream);
 }

@@ -24,17 +24,18 @@ def cameras_tensor(cams):
     return torch.from_numpy(camera_array(cams)).to("cuda")
 
 
-def _params(quality, texture, fmt):
+def _params(quality, texture, fmt, lights=None):
     p = _lib.RenderParams()
     p.quality = quality
     p.texture = texture.data_ptr() if texture is not None else None
     p.rshift, p.gshift, p.bshift, p.amask = fmt
+    p.lights = lights.handle if lights is not None else None
     return p
 
 
 def render(tree, cams, W, H, quality=QUALITY_BEST, texture=None,
            fmt=ARGB8888, want_colors=True, want_voxels=True, want_hits=True,
-           out=None, stream=None):
+           out=None, stream=None, lights=None):
     """Render len(cams) frames (voxg_render).  Returns dict of device
     tensors in the square layout (index f*W*H + cs*16 + p): squares (uint32
     colours, 0 where nothing was hit), voxel (int32, -1 = miss), hit
@@ -47,7 +48,9 @@ def render(tree, cams, W, H, quality=QUALITY_BEST, texture=None,
     using a per-stream scratch buffer of 20 bytes per pixel.  With plain
     ray merging the odd pixels of a block, copies while it merges, are
     traced only for the blocks whose merging stops.  Counting launches
-    trace block by block (launch kind "render")."""
+    trace block by block (launch kind "render").  `lights` (a
+    voxvision_amd.lights.Lights, vox_context_set_light_manager) scales the
+    colours by the light at each hit point."""
     L = _lib.load()
     if not torch.is_tensor(cams):
         cams = cameras_tensor(cams)
@@ -65,7 +68,7 @@ def render(tree, cams, W, H, quality=QUALITY_BEST, texture=None,
     tex = None
     if texture is not None:
         tex = torch.as_tensor(texture, dtype=torch.uint8).to("cuda")
-    p = _params(quality, tex, fmt)
+    p = _params(quality, tex, fmt, lights)
     sq = out.get("squares")
     vx = out.get("voxel")
     hv = out.get("hit")
