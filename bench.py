@@ -318,65 +318,74 @@ def serving_leg(log, seconds, rate, seed=31):
             plan.append((t, "b", ids[k], n, off))
     log("[bench] serving leg: %d requests planned over %.1f s, setup %.1f s"
         % (len(plan), seconds, time.perf_counter() - t0))
-    tickets = queue.Queue()
+    def run(paced):
+        """Submit the plan (at its arrival times when paced, else as fast
+        as the result buffers come back) and collect every result."""
+        tickets = queue.Queue()
 
-    def collect(out):
-        while True:
-            item = tickets.get()
-            if item is None:
-                return
-            tk, kind, px, buf, sz = item
-            info = S.wait(tk, 120.0)
-            if buf is not None:
-                pools[sz].put(buf)
-            out.append((kind, px, info))
+        def collect(out):
+            while True:
+                item = tickets.get()
+                if item is None:
+                    return
+                tk, kind, px, buf, sz = item
+                info = S.wait(tk, 120.0)
+                if buf is not None:
+                    pools[sz].put(buf)
+                out.append((kind, px, info))
 
-    done = []
-    th = threading.Thread(target=collect, args=(done,))
-    th.start()
-    start = time.perf_counter()
-    for item in plan:
-        delay = item[0] - (time.perf_counter() - start)
-        if delay > 0:
-            time.sleep(delay)
-        if item[1] == "r":
-            _, _, tid, cam, sz, q = item
-            buf = pools[sz].get()
-            tk, _ = S.submit_render(tid, cam, sz[0], sz[1], q, out=buf)
-            tickets.put((tk, "r", sz[0] * sz[1], buf, sz))
-        else:
-            _, _, tid, n, off = item
-            tk, _ = S.submit_balls(tid, bc[off:off + n], br[off:off + n])
-            tickets.put((tk, "b", n, None, None))
-    tickets.put(None)
-    th.join()
+        done = []
+        th = threading.Thread(target=collect, args=(done,))
+        th.start()
+        start = time.perf_counter()
+        for item in plan:
+            delay = item[0] - (time.perf_counter() - start)
+            if paced and delay > 0:
+                time.sleep(delay)
+            if item[1] == "r":
+                _, _, tid, cam, sz, q = item
+                buf = pools[sz].get()
+                tk, _ = S.submit_render(tid, cam, sz[0], sz[1], q, out=buf)
+                tickets.put((tk, "r", sz[0] * sz[1], buf, sz))
+            else:
+                _, _, tid, n, off = item
+                tk, _ = S.submit_balls(tid, bc[off:off + n], br[off:off + n])
+                tickets.put((tk, "b", n, None, None))
+        tickets.put(None)
+        th.join()
+        lat = np.array([i["t_done"] - i["t_submit"] for _, _, i in done]) * 1e3
+        lr = np.array([i["t_done"] - i["t_submit"] for k, _, i in done
+                       if k == "r"]) * 1e3
+        lb = np.array([i["t_done"] - i["t_submit"] for k, _, i in done
+                       if k == "b"]) * 1e3
+        first = min(i["t_submit"] for _, _, i in done)
+        last = max(i["t_done"] for _, _, i in done)
+        span = last - first
+        px = sum(p for k, p, _ in done if k == "r")
+        nb = sum(p for k, p, _ in done if k == "b")
+        sizes_g = [i["group_size"] for _, _, i in done]
+        return {"requests": len(done), "span_s": round(span, 3),
+                "p50_ms": round(float(np.percentile(lat, 50)), 3),
+                "p99_ms": round(float(np.percentile(lat, 99)), 3),
+                "render_p50_ms": round(float(np.percentile(lr, 50)), 3),
+                "render_p99_ms": round(float(np.percentile(lr, 99)), 3),
+                "balls_p50_ms": round(float(np.percentile(lb, 50)), 3),
+                "balls_p99_ms": round(float(np.percentile(lb, 99)), 3),
+                "sustained_mrays_s": round(px / span / 1e6, 1),
+                "sustained_mballs_s": round(nb / span / 1e6, 1),
+                "requests_per_s": round(len(done) / span, 1),
+                "mean_group_size": round(float(np.mean(sizes_g)), 2)}
+
+    res = run(paced=True)
+    res["offered_rate_per_s"] = rate
     st = S.stats()
+    res["groups"] = st["groups"]
+    res["peak_groups_inflight"] = st["peak_groups_inflight"]
+    res["peak_inflight"] = st["peak_inflight"]
+    # capacity: the same requests submitted without pacing (in flight at
+    # most as many as there are result buffers, 200)
+    res["saturation"] = run(paced=False)
     S.close()
-    lat = np.array([i["t_done"] - i["t_submit"] for _, _, i in done]) * 1e3
-    lr = np.array([i["t_done"] - i["t_submit"] for k, _, i in done
-                   if k == "r"]) * 1e3
-    lb = np.array([i["t_done"] - i["t_submit"] for k, _, i in done
-                   if k == "b"]) * 1e3
-    first = min(i["t_submit"] for _, _, i in done)
-    last = max(i["t_done"] for _, _, i in done)
-    span = last - first
-    px = sum(p for k, p, _ in done if k == "r")
-    nb = sum(p for k, p, _ in done if k == "b")
-    sizes_g = [i["group_size"] for _, _, i in done]
-    res = {"requests": len(done), "offered_rate_per_s": rate,
-           "span_s": round(span, 3),
-           "p50_ms": round(float(np.percentile(lat, 50)), 3),
-           "p99_ms": round(float(np.percentile(lat, 99)), 3),
-           "render_p50_ms": round(float(np.percentile(lr, 50)), 3),
-           "render_p99_ms": round(float(np.percentile(lr, 99)), 3),
-           "balls_p50_ms": round(float(np.percentile(lb, 50)), 3),
-           "balls_p99_ms": round(float(np.percentile(lb, 99)), 3),
-           "sustained_mrays_s": round(px / span / 1e6, 1),
-           "sustained_mballs_s": round(nb / span / 1e6, 1),
-           "groups": st["groups"],
-           "mean_group_size": round(float(np.mean(sizes_g)), 2),
-           "peak_groups_inflight": st["peak_groups_inflight"],
-           "peak_inflight": st["peak_inflight"]}
     log("[bench] serving leg: %s" % json.dumps(res))
     return res
 

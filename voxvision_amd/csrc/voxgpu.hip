@@ -3260,6 +3260,7 @@ struct SrvRequest {
     int kind = 0;  // 0 render, 1 balls
     int tree = 0;
     dev::Tree t;   // the tree version current when the request was submitted
+    const voxg_tree *th = nullptr;  // and its host handle
     voxg_render_request rr;
     size_t n = 0;
     std::vector<float> balls;  // cx | cy | cz | r
@@ -3296,6 +3297,7 @@ struct voxg_server {
     voxg_server_config cfg;
     Clock::time_point t0;
     std::vector<dev::Tree> trees;
+    std::vector<const voxg_tree *> htrees;  // the same versions, host handles
     uint32_t stack_bound = 1;
     void *d_arena = nullptr;
     uint8_t *d_texture = nullptr;
@@ -3343,6 +3345,14 @@ bool srv_stream_overflow(voxg_server *s, int i) {
     return true;
 }
 
+// Frames at least this large (pixels) are rendered through the render
+// path of voxg_render; smaller ones share a multi-job launch.
+constexpr size_t kSrvBigPixels = (size_t)1 << 19;
+
+bool srv_big(const SrvRequest *r) {
+    return (size_t)r->rr.W * (size_t)r->rr.H >= kSrvBigPixels;
+}
+
 // Launch one group on its stream (scheduler thread, no lock held).
 int srv_launch(voxg_server *s, SrvGroup &g) {
     SrvStream &ss = s->streams[g.stream];
@@ -3356,7 +3366,21 @@ int srv_launch(voxg_server *s, SrvGroup &g) {
     std::vector<dev::CamDev> cams;
     uint32_t units = 0, balls = 0;
     bool color = s->d_texture != nullptr;
-    for (SrvRequest *r : g.reqs) {
+    // render requests in table order: the small frames (one multi-job
+    // launch), then the large ones (the render path, one at a time)
+    std::vector<SrvRequest *> order, big;
+    for (SrvRequest *r : g.reqs)
+        if (r->kind == 0 && !srv_big(r)) order.push_back(r);
+    const size_t n_small = order.size();
+    for (SrvRequest *r : g.reqs)
+        if (r->kind == 0 && srv_big(r)) {
+            order.push_back(r);
+            big.push_back(r);
+        }
+    for (SrvRequest *r : g.reqs)
+        if (r->kind != 0) order.push_back(r);
+    uint32_t small_units = 0;
+    for (SrvRequest *r : order) {
         if (r->kind == 0) {
             size_t np = (size_t)r->rr.W * r->rr.H;
             char *base = arena + r->off;
@@ -3390,6 +3414,7 @@ int srv_launch(voxg_server *s, SrvGroup &g) {
             it.sink.sh.ivy = it.t.ivy;
             it.sink.sh.ivz = it.t.ivz;
             units += it.bpf.d;
+            if (ritems.size() < n_small) small_units = units;
             ritems.push_back(it);
             dev::CamDev c;
             std::memcpy(&c, &r->rr.cam, sizeof(c));
@@ -3430,18 +3455,36 @@ int srv_launch(voxg_server *s, SrvGroup &g) {
     HIPCHK(hipMemcpyAsync(dt, ht, o_c + cb, hipMemcpyHostToDevice, ss.s));
     HIPCHK(hipEventRecord(ss.table_free, ss.s));
     ss.table_pending = true;
-    if (units) {
+    // Large frames go through the render path of voxg_render (block
+    // pre-pass, coherent one-pixel units, the speculated FAST / ADAPTIVE
+    // passes), one after the other on the group's stream; the small ones,
+    // which would each be a few short launches, share one multi-job launch
+    // of one 4x4 block per lane (their units were numbered first).  Both
+    // give the reference's results.  (launch_k and the render path grow
+    // the stream's buffers here, on the scheduler thread, the only thread
+    // that launches on the server's streams.)
+    if (small_units) {
         dev::BlockWork<true> work;
         std::memset(&work, 0, sizeof(work));
         work.items = (const dev::RenderItem *)dt;
-        work.n_items = (uint32_t)ritems.size();
+        work.n_items = (uint32_t)n_small;
         dev::Tree none;
         std::memset(&none, 0, sizeof(none));
-        // (launch_k grows the stream's spill buffer here, on the
-        // scheduler thread, the only thread that launches on it)
         int rc = launch_tree<dev::BlockWork<true>, false>(
-            VOXG_LAUNCH_BATCH, none, g.stack_bound, work, units, ss.s,
+            VOXG_LAUNCH_BATCH, none, g.stack_bound, work, small_units, ss.s,
             nullptr);
+        if (rc) return rc;
+    }
+    for (size_t i = n_small; i < ritems.size(); i++) {
+        SrvRequest *r = big[i - n_small];
+        voxg_render_params p;
+        std::memset(&p, 0, sizeof(p));
+        p.quality = r->rr.quality;
+        const size_t np = (size_t)r->rr.W * (size_t)r->rr.H;
+        int rc = render_launch<false>(r->th, 1,
+                                      (const voxg_camera *)ritems[i].cams,
+                                      r->rr.W, r->rr.H, &p, ritems[i].sink,
+                                      np, nullptr, ss.s);
         if (rc) return rc;
     }
     if (balls) {
@@ -3642,6 +3685,7 @@ int voxg_server_replace_tree(voxg_server *s, int id, const voxg_tree *t) {
         return fail(VOXG_EINVAL, "unknown tree id");
     s->stack_bound = std::max(s->stack_bound, t->stack_bound);
     s->trees[id] = dev_tree(t);
+    s->htrees[id] = t;
     return 0;
 }
 
@@ -3654,6 +3698,7 @@ int voxg_server_add_tree(voxg_server *s, const voxg_tree *t, int *id) {
     std::lock_guard<std::mutex> g(s->mu);
     s->stack_bound = std::max(s->stack_bound, t->stack_bound);
     s->trees.push_back(dev_tree(t));
+    s->htrees.push_back(t);
     *id = (int)s->trees.size() - 1;
     return 0;
 }
@@ -3665,6 +3710,7 @@ static int srv_submit(voxg_server *s, SrvRequest *r, uint64_t *ticket) {
         return fail(VOXG_EINVAL, "unknown tree id");
     }
     r->t = s->trees[r->tree];
+    r->th = s->htrees[r->tree];
     if (r->bytes > s->arena.size()) {
         s->st.rejected++;
         delete r;
