@@ -95,12 +95,14 @@ def cpu_baseline(dots, cams, W, H, budget_s):
             "other": other}
 
 
-def batched_leg(ev0, ev1, stream, log):
+def batched_leg(ev0, ev1, stream, log, fixed=False, split=False):
     """32 uneven scenes (edges 32..512, noise / shells / Menger sponges,
     64K..8M rays each, half camera rays, half random): one plain
     voxg_ray_batch launch per scene versus one persistent batched launch
     (and its HIP-graph replay), each form at its own autotuned
-    configuration."""
+    configuration (fixed: at the library's default configurations, for
+    profiling).  split: the camera halves of the 32 scenes and their random
+    halves also timed alone, as batched launches."""
     import ctypes as C
     import numpy as np
     import torch
@@ -137,10 +139,15 @@ def batched_leg(ev0, ev1, stream, log):
 
     space = ["chunk", "waves_per_cu", "stack_lds", "refill", "leaf_batch",
              "finish_batch"]
-    pcfg, pms, _ = autotune.autotune(plain_runner, params=space, reps=2,
-                                     kind="rays")
-    bcfg, bms, _ = autotune.autotune(batch_runner, params=space, reps=2,
-                                     kind="batch")
+    if fixed:
+        pcfg, bcfg = VT.get_launch_config("rays"), VT.get_launch_config("batch")
+        pms = autotune.time_config(plain_runner, pcfg, 3, "rays")
+        bms = autotune.time_config(batch_runner, bcfg, 3, "batch")
+    else:
+        pcfg, pms, _ = autotune.autotune(plain_runner, params=space, reps=2,
+                                         kind="rays")
+        bcfg, bms, _ = autotune.autotune(batch_runner, params=space, reps=2,
+                                         kind="batch")
     B = batch.Batch(list(zip(trees, first, counts)))
     g = torch.cuda.CUDAGraph()
     gs = torch.cuda.Stream()
@@ -159,13 +166,30 @@ def batched_leg(ev0, ev1, stream, log):
     gms = ev0.elapsed_time(ev1) / 5
     log("[bench] batched leg: plain %.2f ms, batched %.2f ms, graph %.2f ms"
         % (pms, bms, gms))
+    halves = {}
+    if split:
+        for name, part in (
+                ("camera", lambda s: (0, s["camera_pixels"])),
+                ("random", lambda s: (s["camera_pixels"],
+                                      len(s["o"]) - s["camera_pixels"]))):
+            items = [(t, int(f) + part(s)[0], part(s)[1])
+                     for t, f, s in zip(trees, first, c4)]
+            Bh = batch.Batch(items)
+            hms = autotune.time_config(
+                lambda cfg, Bh=Bh: (lambda: Bh.ray(so, sd, vox, None, hit)),
+                bcfg, 3, "batch")
+            nr = sum(c for _, _, c in items)
+            halves[name] = {"rays": nr, "ms": round(hms, 3),
+                            "mrays_s": round(nr / hms / 1e3, 1)}
+        log("[bench] batched halves: %s" % json.dumps(halves))
     return {"scenes": len(c4), "rays": n,
             "voxels": int(sum(len(s["dots"]) for s in c4)),
             "plain_mrays_s": round(n / pms / 1e3, 1),
             "batched_mrays_s": round(n / bms / 1e3, 1),
             "graph_mrays_s": round(n / gms / 1e3, 1),
             "speedup_batched_vs_plain": round(pms / min(bms, gms), 3),
-            "plain_config": pcfg, "batched_config": bcfg}
+            "plain_config": pcfg, "batched_config": bcfg,
+            **({"halves": halves} if halves else {})}
 
 
 def aux_leg(G, dots, W, H, nf, ev0, ev1, stream, log):
@@ -550,6 +574,12 @@ def main():
                          "separated: modes (other render qualities), c3 "
                          "(16 M incoherent rays, 4 M balls), aux (calls "
                          "around the path); --quick runs none")
+    ap.add_argument("--batched-fixed", action="store_true",
+                    help="batched leg at the default launch configurations "
+                         "instead of autotuned ones (for profiling)")
+    ap.add_argument("--batched-split", action="store_true",
+                    help="batched leg: also time the camera and the random "
+                         "halves of the 32 scenes alone")
     ap.add_argument("--copy-calib-mib", type=int, default=0,
                     help="run a plain device copy of this size first (the "
                          "reference kernel for HBM counter calibration)")
@@ -680,7 +710,9 @@ def main():
     if acc:
         extra["fp64_accuracy"] = acc
     if not args.no_batched:
-        extra["batched"] = batched_leg(ev0, ev1, stream, log)
+        extra["batched"] = batched_leg(ev0, ev1, stream, log,
+                                       fixed=args.batched_fixed,
+                                       split=args.batched_split)
     if not args.no_serving:
         VT.set_launch_configs(default_cfgs)
         extra["serving"] = serving_leg(log, args.serve_seconds,

@@ -814,6 +814,19 @@ struct SpecOut {
             hz[r] = p.z;
         }
     }
+    // direct FAST/ADAPTIVE renders: the leaf node of every pixel (kEnd on
+    // a miss), and the hit points of corners 0 and 15 for ADAPTIVE's
+    // decision; the results themselves go to the outputs
+    __device__ __forceinline__ void put_node(uint32_t blk, int px, bool hit,
+                                             uint32_t nd, f3 p) const {
+        const uint32_t r = (uint32_t)px * stride + blk;
+        node[r] = hit ? nd : kEnd;
+        if (hit && (px == 0 || px == 15)) {
+            hx[r] = p.x;
+            hy[r] = p.y;
+            hz[r] = p.z;
+        }
+    }
 };
 
 // BEST-mode render of one job: every pixel is a root search
@@ -823,9 +836,11 @@ struct SpecOut {
 // pixels together: coherent rays, the same nodes at the same time.  Keeps
 // only the state BEST mode needs, which leaves registers for more waves
 // than BlockWork.  MODE 0: results to the job's outputs (BEST renders);
-// 1: root searches into SpecOut (FAST and ADAPTIVE renders); 2: the same
-// for one pixel per unit from a set of a block's pixels (pset below), the
-// ray-merging passes of ADAPTIVE renders.  Separate instantiations, so
+// 1: root searches into SpecOut (FAST and ADAPTIVE renders with accurate
+// ray merging); 2: the same for one pixel per unit from a set of a block's
+// pixels (pset below), the passes of ADAPTIVE renders with plain ray
+// merging; 3: results to the outputs and leaf nodes into SpecOut (FAST
+// and ADAPTIVE renders without merging).  Separate instantiations, so
 // that no unit pays for another mode's branches (a shared pixel-set branch
 // cost the BEST render 1%).
 template <int MODE>
@@ -913,8 +928,11 @@ struct BestWorkT {
                                            uint32_t node, f3 e, uint32_t &cur,
                                            f3 &org, f3 &d) const {
         const int p = render_order((int)(st.pos & 0xffu));
-        if constexpr (MODE != 0) spec.put(st.base >> 4, p, hit, vid, node, e);
-        else one.sink.put(st.base + (uint32_t)p, hit, vid, 0, e);
+        if constexpr (MODE == 1 || MODE == 2)
+            spec.put(st.base >> 4, p, hit, vid, node, e);
+        else
+            one.sink.put(st.base + (uint32_t)p, hit, vid, 0, e);
+        if constexpr (MODE == 3) spec.put_node(st.base >> 4, p, hit, node, e);
         st.pos++;
         if ((st.pos & 0xffu) < ((st.pos >> 8) & 0xffu)) {
             trace(st, cur, org, d);
@@ -934,6 +952,7 @@ struct BestWorkT {
 using BestWork = BestWorkT<0>;
 using BestSpecWork = BestWorkT<1>;
 using BestSetWork = BestWorkT<2>;
+using BestDirectWork = BestWorkT<3>;
 
 // ---------------------------------------------------------------------------
 // persistent traversal kernel
@@ -1424,10 +1443,18 @@ block_cull_kernel(RenderItem it, uint32_t npix, Ctrl *ctrl,
 // 150 units of the camera, renderer.c:274-278) is listed in `redo` instead
 // of replayed (redo_pass 0); once its odd pixels are traced too, a second
 // launch (redo_pass 1) replays the listed blocks.
+//
+// DIRECT (FAST and ADAPTIVE without merging): the first pass wrote its root
+// searches to the outputs and only the leaf nodes (and the corners' hit
+// points) to SpecOut.  Without copies no pixel can turn from a hit into a
+// miss: the replay stores the leaf tries that hit, the only results that
+// differ from the root searches, in SpecOut, and the mask of each block's
+// stored pixels in `changed`, by queue position, for emit_changed_kernel.
+template <bool DIRECT>
 __global__ void __launch_bounds__(256)
 resolve_kernel(RenderItem it, SpecOut spec, const uint32_t *__restrict__ queue,
                uint32_t qlen, Ctrl *ctrl, uint32_t *__restrict__ redo,
-               int redo_pass) {
+               int redo_pass, uint32_t *__restrict__ changed) {
     uint32_t qf, qt;
     if (redo_pass) {
         qf = qt = min((uint32_t)ctrl->rpair, qlen);
@@ -1448,20 +1475,30 @@ resolve_kernel(RenderItem it, SpecOut spec, const uint32_t *__restrict__ queue,
         const CamDev &c = it.cams[f];
         const f3 o = f3{c.px, c.py, c.pz};
         // the first pass's root search of pixel p (all five words loaded
-        // together: the node and point are only meaningful on a hit)
-        auto root = [&](int p, uint32_t &vid, uint32_t &nd, f3 &pt) {
+        // together: the node and point are only meaningful on a hit); in a
+        // direct render the leaf node alone, and the point for the corners
+        auto root = [&](int p, uint32_t &vid, uint32_t &nd, f3 &pt,
+                        bool corner) {
             const uint32_t r = (uint32_t)p * spec.stride + id;
+            if constexpr (DIRECT) {
+                nd = spec.node[r];
+                vid = kEnd;  // used by merge copies only
+                if (corner) pt = f3{spec.hx[r], spec.hy[r], spec.hz[r]};
+                return nd != kEnd;
+            }
             vid = spec.vid[r];
             nd = spec.node[r];
             pt = f3{spec.hx[r], spec.hy[r], spec.hz[r]};
             return vid != kEnd;
         };
+        uint32_t mask = 0;  // DIRECT: the pixels stored
         auto store = [&](int p, uint32_t vid, f3 pt) {
             const uint32_t r = (uint32_t)p * spec.stride + id;
             spec.vid[r] = vid;
             spec.hx[r] = pt.x;
             spec.hy[r] = pt.y;
             spec.hz[r] = pt.z;
+            mask |= 1u << p;
         };
         int mode = it.rnd_mode, merge = 0;
         uint32_t leaf = kEnd, vid_last = kEnd;
@@ -1474,14 +1511,14 @@ resolve_kernel(RenderItem it, SpecOut spec, const uint32_t *__restrict__ queue,
             merge = it.merge_mode == VOXG_QUALITY_RAY_MERGE;
             uint32_t v0, n0, v15, n15;
             f3 p0, p15;
-            const bool h0 = root(0, v0, n0, p0);
+            const bool h0 = root(0, v0, n0, p0, true);
             leaf = h0 ? n0 : kEnd;
             i = 1;
             if (h0) {
                 vid_last = v0;
                 inter_last = p0;
                 iend = 15;
-                if (root(15, v15, n15, p15)) {
+                if (root(15, v15, n15, p15, true)) {
                     const f3 dir1 = screen2world(c, xs, ys);
                     const f3 dir15 = screen2world(c, xs + 3, ys + 3);
                     const float d1 = sqr_metric(p0, o);
@@ -1522,10 +1559,11 @@ resolve_kernel(RenderItem it, SpecOut spec, const uint32_t *__restrict__ queue,
             if (wait) continue;
         }
         // BEST without merging: the root searches are the results
-        if (mode != VOXG_QUALITY_FAST && !merge) continue;
+        if (!DIRECT && mode != VOXG_QUALITY_FAST && !merge) continue;
         // the record of the leaf last tried (a FAST chain often keeps it)
         uint32_t loaded = kEnd;
         Node ln;
+        if (DIRECT && mode != VOXG_QUALITY_FAST) iend = i;
         for (; i < iend; ++i) {
             const int p = render_order(i);
             if (merge && (i & 1)) {
@@ -1537,7 +1575,7 @@ resolve_kernel(RenderItem it, SpecOut spec, const uint32_t *__restrict__ queue,
             // two memory round trips overlap
             uint32_t rvid, rnd;
             f3 rpt;
-            const bool rhit = root(p, rvid, rnd, rpt);
+            const bool rhit = root(p, rvid, rnd, rpt, false);
             if (mode == VOXG_QUALITY_FAST && leaf != kEnd) {
                 // renderer.c:294-300: the previous pixel's leaf first
                 const f3 d = screen2world(c, xs + (p & 3), ys + (p >> 2));
@@ -1563,6 +1601,7 @@ resolve_kernel(RenderItem it, SpecOut spec, const uint32_t *__restrict__ queue,
                 leaf = kEnd;
             }
         }
+        if constexpr (DIRECT) changed[b] = mask;
     }
 }
 
@@ -1617,6 +1656,61 @@ emit_kernel(RenderItem it, SpecOut spec, const uint32_t *__restrict__ queue,
             f3 pt = f3{0, 0, 0};
             if (hit) pt = f3{spx[j], spy[j], spz[j]};
             it.sink.put(sid[j >> 4] * 16u + (j & 15u), hit, v, 0, pt);
+        }
+        __syncthreads();
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        __threadfence();
+        if (atomicAdd(&ctrl->rdone, 1u) == gridDim.x - 1u) {
+            atomicExch(&ctrl->qpair, 0ull);
+            atomicExch(&ctrl->rpair, 0ull);
+            atomicExch(&ctrl->rdone, 0u);
+        }
+    }
+}
+
+// Third pass of a direct render: the pixels that the replay changed, from
+// SpecOut over the first pass's results in the outputs, transposed through
+// LDS as in emit_kernel (queue order: neighbouring lanes read neighbouring
+// blocks).  Every changed pixel is a hit.  The last workgroup clears the
+// queue counts for the next render.
+__global__ void __launch_bounds__(256)
+emit_changed_kernel(RenderItem it, SpecOut spec,
+                    const uint32_t *__restrict__ queue,
+                    const uint32_t *__restrict__ changed, uint32_t qlen,
+                    Ctrl *ctrl) {
+    __shared__ uint32_t sid[kEmitBlocks], smask[kEmitBlocks];
+    __shared__ uint32_t svid[kEmitBlocks * 16];
+    __shared__ float spx[kEmitBlocks * 16], spy[kEmitBlocks * 16],
+        spz[kEmitBlocks * 16];
+    uint32_t qf, qt;
+    queue_counts(ctrl->qpair, qlen, qf, qt);
+    const uint32_t tid = threadIdx.x;
+    for (uint32_t b0 = blockIdx.x * kEmitBlocks; b0 < qt;
+         b0 += gridDim.x * kEmitBlocks) {
+        const uint32_t nb = min(kEmitBlocks, qt - b0);
+        if (tid < nb) {
+            const uint32_t b = b0 + tid;
+            sid[tid] = queue[b < qf ? b : qlen - 1u - (b - qf)];
+            smask[tid] = changed[b];
+        }
+        __syncthreads();
+        const uint32_t k = tid & (kEmitBlocks - 1u);
+        for (uint32_t p = tid / kEmitBlocks; p < 16u; p += 256u / kEmitBlocks) {
+            if (k < nb && ((smask[k] >> p) & 1u)) {
+                const uint32_t r = p * spec.stride + sid[k];
+                svid[k * 16u + p] = spec.vid[r];
+                spx[k * 16u + p] = spec.hx[r];
+                spy[k * 16u + p] = spec.hy[r];
+                spz[k * 16u + p] = spec.hz[r];
+            }
+        }
+        __syncthreads();
+        for (uint32_t j = tid; j < nb * 16u; j += 256u) {
+            if ((smask[j >> 4] >> (j & 15u)) & 1u)
+                it.sink.put(sid[j >> 4] * 16u + (j & 15u), true, svid[j], 0,
+                            f3{spx[j], spy[j], spz[j]});
         }
         __syncthreads();
     }
@@ -2785,15 +2879,19 @@ static int render_run(const voxg_tree *t, Work &work,
 // (skip_odd) the first pass traces 9 of a block's 16 pixels, the ones a
 // block that keeps merging needs; the blocks whose merging stops get their
 // other 7 traced and are replayed by two more launches before the output
-// pass.  emit_kernel clears the queue counts; until it is enqueued the
-// stream's workspace stays dirty.
+// pass.  Without merging (direct) the first pass writes the outputs and
+// only leaf nodes to the scratch, and the output pass writes only the
+// pixels whose leaf try hit.  The output pass clears the queue counts;
+// until it is enqueued the stream's workspace stays dirty.
 static int render_speculative(const voxg_tree *t, const dev::RenderItem &one,
-                              size_t npix, bool skip_odd, hipStream_t stream) {
+                              size_t npix, bool skip_odd, bool direct,
+                              hipStream_t stream) {
     voxg_launch_config cfg = current_config(VOXG_LAUNCH_RENDER_BEST);
     cfg.render_cull = 1;  // the resolve pass reads the pre-pass's queue
     const size_t nblk = npix / 16;
     void *sp = nullptr;
     // root-search results, then the list of blocks whose merging stopped
+    // or (direct) the changed pixels' mask of each queued block
     int rc = voxg_internal::scratch(stream, npix * 20 + nblk * 4, &sp);
     if (rc) return rc;
     dev::SpecOut spec;
@@ -2804,6 +2902,7 @@ static int render_speculative(const voxg_tree *t, const dev::RenderItem &one,
     spec.hz = spec.hy + npix;
     spec.stride = (uint32_t)nblk;
     uint32_t *redo = skip_odd ? (uint32_t *)(spec.hz + npix) : nullptr;
+    uint32_t *changed = direct ? (uint32_t *)(spec.hz + npix) : nullptr;
     // one pixel per unit, into the scratch; the queue counts stay for the
     // passes below
     auto first_pass = [&](auto &work) {
@@ -2813,7 +2912,13 @@ static int render_speculative(const voxg_tree *t, const dev::RenderItem &one,
         work.keep_queue = 1;
         work.split = 4;
     };
-    if (skip_odd) {
+    if (direct) {
+        dev::BestDirectWork work;
+        first_pass(work);
+        work.group = 256u << work.split;
+        rc = render_run<dev::BestDirectWork, false>(t, work, cfg, npix,
+                                                    nullptr, stream);
+    } else if (skip_odd) {
         dev::BestSetWork work;
         first_pass(work);
         work.pset = 1;
@@ -2837,9 +2942,22 @@ static int render_speculative(const voxg_tree *t, const dev::RenderItem &one,
     const size_t grid = std::max<size_t>(
         1, std::min<size_t>((nblk + 255) / 256, (size_t)cu_count(devid) * 8));
     if (injected_failure()) return fail(VOXG_EIO, "injected failure");
-    hipLaunchKernelGGL(dev::resolve_kernel, dim3((unsigned)grid), dim3(256),
-                       0, stream, one, spec, ws->queue, (uint32_t)nblk,
-                       ws->ctrl, redo, 0);
+    if (direct) {
+        hipLaunchKernelGGL(dev::resolve_kernel<true>, dim3((unsigned)grid),
+                           dim3(256), 0, stream, one, spec, ws->queue,
+                           (uint32_t)nblk, ws->ctrl, nullptr, 0, changed);
+        HIPCHK(hipGetLastError());
+        if (injected_failure()) return fail(VOXG_EIO, "injected failure");
+        hipLaunchKernelGGL(dev::emit_changed_kernel, dim3((unsigned)grid),
+                           dim3(256), 0, stream, one, spec, ws->queue,
+                           changed, (uint32_t)nblk, ws->ctrl);
+        HIPCHK(hipGetLastError());
+        ws->dirty = false;
+        return 0;
+    }
+    hipLaunchKernelGGL(dev::resolve_kernel<false>, dim3((unsigned)grid),
+                       dim3(256), 0, stream, one, spec, ws->queue,
+                       (uint32_t)nblk, ws->ctrl, redo, 0, nullptr);
     HIPCHK(hipGetLastError());
     if (skip_odd) {
         // the odd pixels of the listed blocks, then their replay (the
@@ -2856,9 +2974,9 @@ static int render_speculative(const voxg_tree *t, const dev::RenderItem &one,
             cfg, dev_tree(t), t->stack_bound, odd, nblk * 7, stream, nullptr,
             false);
         if (rc) return rc;
-        hipLaunchKernelGGL(dev::resolve_kernel, dim3((unsigned)grid),
+        hipLaunchKernelGGL(dev::resolve_kernel<false>, dim3((unsigned)grid),
                            dim3(256), 0, stream, one, spec, ws->queue,
-                           (uint32_t)nblk, ws->ctrl, redo, 1);
+                           (uint32_t)nblk, ws->ctrl, redo, 1, nullptr);
         HIPCHK(hipGetLastError());
     }
     if (injected_failure()) return fail(VOXG_EIO, "injected failure");
@@ -2911,7 +3029,8 @@ static int render_launch(const voxg_tree *t, size_t n_frames,
     // which most blocks copy
     if (!COUNT && cfg.render_cull && !t->host.nodes.empty())
         return render_speculative(t, one, npix,
-                                  (q & VOXG_QUALITY_RAY_MERGE) != 0, stream);
+                                  (q & VOXG_QUALITY_RAY_MERGE) != 0,
+                                  one.merge_mode == 0, stream);
     // one 4x4 block per unit (renderer.c:214): the leaf of one pixel is
     // tried first for the next (the counting launch and renders without
     // the pre-pass take this path)
