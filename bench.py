@@ -197,8 +197,11 @@ def aux_leg(G, dots, W, H, nf, ev0, ev1, stream, log):
     copy of the headline frames (with the clear), vox_cd_collide for 4 M
     bodies, and vox_read_raw_data's selection over a 512^3 one-byte
     volume (8% selected).  HBM bytes: redraw 12 B per pixel (read 4, write
-    4 + clear 4); selection 2 x 1 B per sample read + 12 B per selected
-    voxel written.  Tree edits: 20,000 alternating inserts and deletes on
+    4 + clear 4); selection 1 B per sample read + 12 B per selected voxel
+    written (the compulsory bytes; the count pass also writes and the
+    write pass reads one selection bit per sample).  The selection is
+    timed as the Python call makes it: count, the count to the host, exact
+    allocation, corners.  Tree edits: 20,000 alternating inserts and deletes on
     the host tree of the headline scene (vox_insert_voxel /
     vox_delete_voxel), then one commit of the result as a new device
     version (flatten + upload)."""
@@ -263,8 +266,27 @@ def aux_leg(G, dots, W, H, nf, ev0, ev1, stream, log):
     res["raw_select_512cube_ms"] = round(ms, 4)
     res["raw_select_gsamples_s"] = round(flat.numel() / ms / 1e6, 1)
     res["raw_select_selected"] = int(m)
-    res["raw_select_gb_s"] = round((2.0 * flat.numel() + 12.0 * m) / ms / 1e6,
+    res["raw_select_gb_s"] = round((1.0 * flat.numel() + 12.0 * m) / ms / 1e6,
                                    1)
+    # the same selection in one voxg_raw_select call into an output of known
+    # capacity: the device passes alone, no host round trip
+    import ctypes as C
+    from voxvision_amd import _lib
+    L = _lib.load()
+    d3 = (C.c_uint * 3)(*dim)
+    one = (C.c_float * 3)(1.0, 1.0, 1.0)
+    cnt = torch.empty(1, dtype=torch.int64, device="cuda")
+    out = torch.empty((m, 3), dtype=torch.float32, device="cuda")
+    sp = C.c_void_p(stream.cuda_stream)
+
+    def select_one():
+        _lib.check(L.voxg_raw_select(flat.data_ptr(), d3, 1, 235, 256, one,
+                                     out.data_ptr(), m, cnt.data_ptr(), sp))
+    ms1 = timed(select_one, reps=5)
+    assert int(cnt.item()) == m and torch.equal(out, xyz[0])
+    res["raw_select_one_call_ms"] = round(ms1, 4)
+    res["raw_select_one_call_gb_s"] = round(
+        (1.0 * flat.numel() + 12.0 * m) / ms1 / 1e6, 1)
     # tree edits on the headline scene: host-side vox_insert_voxel /
     # vox_delete_voxel, then one commit (flatten + upload of a version)
     from voxvision_amd.edit import EditTree

@@ -25,6 +25,7 @@
  *   voxg_cd_collide_batch   vox_cd_collide            src/voxrnd/cd.h:47
  *   voxg_read_raw_data      vox_read_raw_data         src/voxtrees/datareader.h:35
  *   voxg_raw_select         its sample loop           src/voxtrees/datareader.c:49-75
+ *   voxg_raw_count / _write the same loop in two phases src/voxtrees/datareader.c:49-75
  *   voxg_edit_insert        vox_insert_voxel          src/voxtrees/tree.h:96
  *   voxg_edit_delete        vox_delete_voxel          src/voxtrees/tree.h:114
  *   voxg_edit_rebuild       vox_rebuild_tree          src/voxtrees/tree.h:86
@@ -445,6 +446,25 @@ int voxg_raw_select(const void *samples, const unsigned dim[3],
                     unsigned samplesize, uint64_t lo, uint64_t hi,
                     const float voxel[3], float *xyz_out, size_t capacity,
                     unsigned long long *count_out, hipStream_t stream);
+
+/* voxg_raw_select in two phases around the caller's allocation of the
+ * exact output (the same sample loop, datareader.c:49-75, split where the
+ * reference grows its dot array).  voxg_raw_count reads the samples once
+ * and leaves the selection in `work` (device, 16-byte aligned,
+ * voxg_raw_workspace_bytes(dim) bytes: a bit per sample and an offset per
+ * 4096 samples) and the number selected in *count_out (device, or NULL).
+ * voxg_raw_write then writes the selected voxels' corners from `work`
+ * alone, as voxg_raw_select does (at most `capacity` of them); it does not
+ * read the samples, so `work` must come from voxg_raw_count on a volume of
+ * the same dim, and stays valid until it is reused.
+ * voxg_raw_workspace_bytes returns 0 for a volume of 2^32 samples or more. */
+size_t voxg_raw_workspace_bytes(const unsigned dim[3]);
+int voxg_raw_count(const void *samples, const unsigned dim[3],
+                   unsigned samplesize, uint64_t lo, uint64_t hi, void *work,
+                   unsigned long long *count_out, hipStream_t stream);
+int voxg_raw_write(const unsigned dim[3], const float voxel[3],
+                   const void *work, float *xyz_out, size_t capacity,
+                   hipStream_t stream);
 
 /* vox_read_raw_data (datareader.c:11-85) with the test lo <= value < hi:
  * reads the file (its size must be dim[0]*dim[1]*dim[2]*samplesize,

@@ -107,6 +107,78 @@ def test_raw_select_matches_oracle(gpu, oracle, ss):
                                   want.view(np.uint32))
 
 
+@pytest.mark.parametrize("ss", [1, 3])
+def test_raw_select_one_call_capacity_and_empty(gpu, oracle, ss):
+    """voxg_raw_select in one call (the selection on the stream's scratch):
+    all corners; the first `capacity` of them when the output is smaller
+    than the count (which still reports every selected voxel); count only;
+    and a volume with no sample.  voxg_raw_write from a workspace left by
+    voxg_raw_count gives the same corners again after another selection
+    has used the stream's scratch."""
+    import ctypes as C
+    from voxvision_amd import _lib
+    L = gpu
+    st = C.c_void_p(torch.cuda.current_stream().cuda_stream)
+    rng = np.random.default_rng(10 + ss)
+    dim = (41, 33, 19)
+    n = dim[0] * dim[1] * dim[2]
+    raw = rng.integers(0, 256, n * ss).astype(np.uint8)
+    top = 1 << (8 * ss)
+    lo, hi = top // 3, top
+    V = (0.5, 1.0, 3.0)
+    want = oracle.read_raw(raw.tobytes(), dim, ss, lo, hi, voxel=V)
+    m = len(want)
+    assert m > 1000
+    buf = torch.from_numpy(raw).cuda()
+    d = (C.c_uint * 3)(*dim)
+    v = (C.c_float * 3)(*V)
+    cnt = torch.zeros(1, dtype=torch.int64, device="cuda")
+    out = torch.full((m + 5, 3), float("nan"), device="cuda")
+
+    def one(cap, ptr):
+        cnt.fill_(-1)
+        _lib.check(L.voxg_raw_select(buf.data_ptr(), d, ss, lo, hi, v, ptr,
+                                     cap, cnt.data_ptr(), st))
+        torch.cuda.synchronize()
+        return int(cnt.item())
+
+    assert one(m + 5, out.data_ptr()) == m
+    got = out.cpu().numpy()
+    assert np.array_equal(got[:m].view(np.uint32), want.view(np.uint32))
+    assert np.isnan(got[m:]).all()
+    out.fill_(float("nan"))
+    cap = m // 3
+    assert one(cap, out.data_ptr()) == m
+    got = out.cpu().numpy()
+    assert np.array_equal(got[:cap].view(np.uint32),
+                          want[:cap].view(np.uint32))
+    assert np.isnan(got[cap:]).all()
+    assert one(0, None) == m
+    # two-phase: count into a workspace, let another selection use the
+    # stream's scratch, then write from the workspace
+    nb = L.voxg_raw_workspace_bytes(d)
+    assert nb >= n // 8
+    work = torch.empty(nb, dtype=torch.uint8, device="cuda")
+    cnt2 = torch.zeros(1, dtype=torch.int64, device="cuda")
+    _lib.check(L.voxg_raw_count(buf.data_ptr(), d, ss, lo, hi,
+                                work.data_ptr(), cnt2.data_ptr(), st))
+    assert one(0, None) == m
+    out.fill_(float("nan"))
+    _lib.check(L.voxg_raw_write(d, v, work.data_ptr(), out.data_ptr(), m,
+                                st))
+    torch.cuda.synchronize()
+    assert int(cnt2.item()) == m
+    assert np.array_equal(out.cpu().numpy()[:m].view(np.uint32),
+                          want.view(np.uint32))
+    # no sample: count 0, nothing written
+    z = (C.c_uint * 3)(0, 7, 5)
+    cnt.fill_(-1)
+    _lib.check(L.voxg_raw_select(None, z, ss, lo, hi, v, out.data_ptr(), 4,
+                                 cnt.data_ptr(), st))
+    torch.cuda.synchronize()
+    assert int(cnt.item()) == 0
+
+
 def test_read_raw_data_builds_the_same_tree(gpu, oracle):
     """vox_read_raw_data on a file: the tree equals the oracle's
     vox_make_tree over the oracle's selection (structure hash), and it

@@ -30,7 +30,8 @@ EXPORTS = [
     "voxg_server_add_tree", "voxg_server_submit_render",
     "voxg_server_submit_balls", "voxg_server_wait", "voxg_server_get_stats",
     "voxg_server_now", "voxg_server_destroy", "voxg_cd_collide_batch",
-    "voxg_redraw", "voxg_raw_select", "voxg_read_raw_data",
+    "voxg_redraw", "voxg_raw_select", "voxg_raw_workspace_bytes",
+    "voxg_raw_count", "voxg_raw_write", "voxg_read_raw_data",
     "voxg_edit_create", "voxg_edit_insert", "voxg_edit_delete",
     "voxg_edit_rebuild", "voxg_edit_root", "voxg_edit_voxels",
     "voxg_edit_commit", "voxg_edit_release", "voxg_server_replace_tree",
@@ -146,6 +147,10 @@ def load():
     L.voxg_redraw.argtypes = [P, P, S, I, I, I, P]
     U64 = C.c_uint64
     L.voxg_raw_select.argtypes = [P, P, C.c_uint, U64, U64, P, P, S, P, P]
+    L.voxg_raw_workspace_bytes.argtypes = [P]
+    L.voxg_raw_workspace_bytes.restype = S
+    L.voxg_raw_count.argtypes = [P, P, C.c_uint, U64, U64, P, P, P]
+    L.voxg_raw_write.argtypes = [P, P, P, P, S, P]
     L.voxg_read_raw_data.argtypes = [C.c_char_p, P, C.c_uint, U64, U64, P,
                                      C.POINTER(P)]
     L.voxg_edit_create.argtypes = [P, S, S, P, C.POINTER(P)]

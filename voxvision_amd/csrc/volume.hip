@@ -46,23 +46,63 @@ redraw_kernel(uint4 *__restrict__ sq, uint4 *__restrict__ px, uint32_t ws,
 }
 
 // ---------------------------------------------------------------------------
-// vox_read_raw_data's sample loop.  A workgroup of 256 threads owns a tile
-// of kTile consecutive samples, 16 per thread.  Pass 1 counts the selected
-// samples of every tile, pass 2 turns the counts into tile offsets (one
-// workgroup) and the total, pass 3 recounts, ranks the threads of the tile
-// and writes the voxel corners in raster order.
+// vox_read_raw_data's sample loop as an ordered compaction.  A workgroup of
+// 256 threads owns a tile of kTile consecutive samples, kPer per thread.
+// Phase 1: raw_count_kernel reads the samples once (32 to 128 bytes per
+// thread, enough loads in flight to stream the volume), stores each
+// thread's selection as a 32-bit mask and counts the selected samples of
+// every tile; raw_scan_kernel turns the counts into offsets within groups
+// of kGroup tiles (many workgroups, coalesced), raw_total_kernel the group
+// sums into group offsets and the total.  Phase 2: raw_write_kernel reads
+// only the masks, ranks the threads of a tile, stages the tile's voxel
+// corners in LDS in raster order and stores them with consecutive threads
+// on consecutive words.  The masks and offsets live in a workspace (the
+// caller's, or the stream's scratch), so that the two phases can be split
+// around a host allocation of the exact output.
 // ---------------------------------------------------------------------------
-constexpr uint32_t kPer = 16;
+constexpr uint32_t kPer = 32;
 constexpr uint32_t kTile = 256 * kPer;
+constexpr uint32_t kGroup = 1024;   // tiles per scan group
+constexpr uint32_t kTotalThreads = 1024;
+constexpr uint32_t kStage = 1024;   // corners staged in LDS per round
+// fewer than 2^32 samples: at most 2^19 tiles, 512 groups
+static_assert((1ull << 32) / kTile / kGroup <= kTotalThreads,
+              "one workgroup scans the group sums");
 
 struct RawArgs {
     const uint8_t *data;
     uint64_t n;          // samples
-    uint32_t d1, d2;     // dim[1], dim[2]
     uint64_t lo, hi;     // the test lo <= sample < hi
-    float v0, v1, v2;    // vox_voxel
     int aligned;         // data is 16-byte aligned
 };
+
+// Workspace of a volume of n samples: the tile offsets (ntiles + 1 words,
+// the last one the total), the group offsets, then one 32-bit mask per
+// thread, each part padded to 256 bytes.
+struct RawWork {
+    uint32_t *tiles, *groups, *masks;
+};
+
+uint32_t raw_tiles(uint64_t n) { return (uint32_t)((n + kTile - 1) / kTile); }
+uint32_t raw_groups(uint64_t n) { return (raw_tiles(n) + kGroup - 1) / kGroup; }
+
+size_t pad256(size_t b) { return (b + 255) / 256 * 256; }
+
+size_t raw_work_bytes(uint64_t n) {
+    return pad256(((size_t)raw_tiles(n) + 1) * 4) +
+           pad256((size_t)raw_groups(n) * 4) + (size_t)raw_tiles(n) * 256 * 4;
+}
+
+RawWork raw_work(void *base, uint64_t n) {
+    uint8_t *p = static_cast<uint8_t *>(base);
+    RawWork w;
+    w.tiles = reinterpret_cast<uint32_t *>(p);
+    p += pad256(((size_t)raw_tiles(n) + 1) * 4);
+    w.groups = reinterpret_cast<uint32_t *>(p);
+    p += pad256((size_t)raw_groups(n) * 4);
+    w.masks = reinterpret_cast<uint32_t *>(p);
+    return w;
+}
 
 // The kPer samples from sample `first` on (little endian, SS bytes each)
 // and how many of them exist.
@@ -74,17 +114,18 @@ __device__ __forceinline__ uint32_t load_samples(const RawArgs &a,
     const uint32_t m = left < kPer ? (uint32_t)left : kPer;
     if (m == kPer && a.aligned) {
         // kPer * SS bytes starting at a multiple of 16
-        uint32_t w[4 * SS + 1];
+        constexpr int NQ = kPer * SS / 16;
+        uint32_t w[4 * NQ + 1];
         const uint4 *q = reinterpret_cast<const uint4 *>(a.data + first * SS);
 #pragma unroll
-        for (int k = 0; k < SS; k++) {
+        for (int k = 0; k < NQ; k++) {
             const uint4 x = q[k];
             w[4 * k + 0] = x.x;
             w[4 * k + 1] = x.y;
             w[4 * k + 2] = x.z;
             w[4 * k + 3] = x.w;
         }
-        w[4 * SS] = 0;
+        w[4 * NQ] = 0;
 #pragma unroll
         for (uint32_t k = 0; k < kPer; k++) {
             const uint32_t byte = k * SS, wi = byte >> 2, sh = 8u * (byte & 3u);
@@ -110,10 +151,11 @@ __device__ __forceinline__ bool selected(const RawArgs &a, uint32_t v) {
     return (uint64_t)v >= a.lo && (uint64_t)v < a.hi;
 }
 
-// Sum of x over the workgroup (256 threads); every thread gets the sum,
-// and `below` the sum over the threads before it.
+// Sum of x over the workgroup (NW waves); every thread gets the sum, and
+// `below` the sum over the threads before it.
+template <uint32_t NW>
 __device__ __forceinline__ uint32_t block_scan(uint32_t x, uint32_t &below) {
-    __shared__ uint32_t wsum[4];
+    __shared__ uint32_t wsum[NW];
     const uint32_t lane = threadIdx.x & 63u, w = threadIdx.x >> 6;
     uint32_t incl = x;
 #pragma unroll
@@ -124,7 +166,7 @@ __device__ __forceinline__ uint32_t block_scan(uint32_t x, uint32_t &below) {
     if (lane == 63) wsum[w] = incl;
     __syncthreads();
     uint32_t before = 0, total = 0;
-    for (uint32_t i = 0; i < 4; i++) {
+    for (uint32_t i = 0; i < NW; i++) {
         before += i < w ? wsum[i] : 0u;
         total += wsum[i];
     }
@@ -135,91 +177,112 @@ __device__ __forceinline__ uint32_t block_scan(uint32_t x, uint32_t &below) {
 
 template <int SS>
 __global__ void __launch_bounds__(256)
-raw_count_kernel(RawArgs a, uint32_t *__restrict__ tile_count) {
-    const uint64_t first = (uint64_t)blockIdx.x * kTile + threadIdx.x * kPer;
+raw_count_kernel(RawArgs a, RawWork w) {
+    const uint64_t g = (uint64_t)blockIdx.x * 256u + threadIdx.x;
     uint32_t v[kPer];
-    const uint32_t m = load_samples<SS>(a, first, v);
-    uint32_t c = 0;
+    const uint32_t m = load_samples<SS>(a, g * kPer, v);
+    uint32_t mask = 0;
 #pragma unroll
-    for (uint32_t k = 0; k < kPer; k++) c += (k < m && selected(a, v[k])) ? 1u : 0u;
+    for (uint32_t k = 0; k < kPer; k++)
+        mask |= (k < m && selected(a, v[k])) ? 1u << k : 0u;
+    w.masks[g] = mask;
     uint32_t below;
-    const uint32_t total = block_scan(c, below);
-    if (threadIdx.x == 0) tile_count[blockIdx.x] = total;
+    const uint32_t total = block_scan<4>((uint32_t)__popc(mask), below);
+    if (threadIdx.x == 0) w.tiles[blockIdx.x] = total;
 }
 
-// Exclusive prefix sum of the tile counts, in place, by one workgroup;
-// the total goes to *count.
-__global__ void __launch_bounds__(256)
-raw_scan_kernel(uint32_t *__restrict__ tile, uint32_t ntiles,
-                unsigned long long *__restrict__ count) {
-    uint32_t carry = 0;
-    for (uint32_t base = 0; base < ntiles; base += 256) {
-        const uint32_t i = base + threadIdx.x;
-        const uint32_t x = i < ntiles ? tile[i] : 0u;
-        uint32_t below;
-        const uint32_t total = block_scan(x, below);
-        if (i < ntiles) tile[i] = carry + below;
-        carry += total;
+// Offsets of the tiles within their group of kGroup, in place, and the
+// group's sum.
+__global__ void __launch_bounds__(kGroup)
+raw_scan_kernel(RawWork w, uint32_t ntiles) {
+    const uint32_t t = blockIdx.x * kGroup + threadIdx.x;
+    const uint32_t x = t < ntiles ? w.tiles[t] : 0u;
+    uint32_t below;
+    const uint32_t total = block_scan<kGroup / 64>(x, below);
+    if (t < ntiles) w.tiles[t] = below;
+    if (threadIdx.x == 0) w.groups[blockIdx.x] = total;
+}
+
+// Offsets of the groups, in place; the total after the last tile offset
+// and in *count.
+__global__ void __launch_bounds__(kTotalThreads)
+raw_total_kernel(RawWork w, uint32_t ntiles, uint32_t ngroups,
+                 unsigned long long *__restrict__ count) {
+    const uint32_t x = threadIdx.x < ngroups ? w.groups[threadIdx.x] : 0u;
+    uint32_t below;
+    const uint32_t total = block_scan<kTotalThreads / 64>(x, below);
+    if (threadIdx.x < ngroups) w.groups[threadIdx.x] = below;
+    if (threadIdx.x == 0) {
+        w.tiles[ntiles] = total;
+        if (count) *count = total;
     }
-    if (threadIdx.x == 0) *count = carry;
 }
 
-template <int SS>
+struct RawOut {
+    uint64_t n;
+    uint32_t d1, d2;
+    float v0, v1, v2;
+};
+
 __global__ void __launch_bounds__(256)
-raw_write_kernel(RawArgs a, const uint32_t *__restrict__ tile_base,
-                 float *__restrict__ xyz, uint64_t capacity) {
-    const uint64_t first = (uint64_t)blockIdx.x * kTile + threadIdx.x * kPer;
-    uint32_t v[kPer];
-    const uint32_t m = load_samples<SS>(a, first, v);
-    uint32_t c = 0;
-#pragma unroll
-    for (uint32_t k = 0; k < kPer; k++) c += (k < m && selected(a, v[k])) ? 1u : 0u;
+raw_write_kernel(RawOut a, RawWork w, float *__restrict__ xyz,
+                 uint64_t capacity) {
+    __shared__ float stage[3 * kStage];
+    const uint64_t g = (uint64_t)blockIdx.x * 256u + threadIdx.x;
+    const uint64_t first = g * kPer;
+    const uint32_t mask = first < a.n ? w.masks[g] : 0u;
     uint32_t below;
-    block_scan(c, below);
-    if (c == 0) return;
-    uint64_t out = (uint64_t)tile_base[blockIdx.x] + below;
+    const uint32_t total = block_scan<4>((uint32_t)__popc(mask), below);
+    // the tile's first corner (the same for every thread of the tile)
+    const uint64_t base = (uint64_t)w.tiles[blockIdx.x] +
+                          w.groups[blockIdx.x / kGroup];
+    if (base >= capacity) return;
     // raster position of the first sample: first = (i * d1 + j) * d2 + k
     const uint64_t ij = first / a.d2;
-    uint32_t k = (uint32_t)(first - ij * a.d2);
-    uint32_t j = (uint32_t)(ij % a.d1);
-    uint32_t i = (uint32_t)(ij / a.d1);
-#pragma unroll
-    for (uint32_t s = 0; s < kPer; s++) {
-        if (s < m && selected(a, v[s])) {
-            if (out < capacity) {
-                // datareader.c:66-68: i*vox_voxel[0] etc. in float
-                xyz[3 * out + 0] = (float)i * a.v0;
-                xyz[3 * out + 1] = (float)j * a.v1;
-                xyz[3 * out + 2] = (float)k * a.v2;
+    const uint32_t k0 = (uint32_t)(first - ij * a.d2);
+    const uint32_t j0 = (uint32_t)(ij % a.d1), i0 = (uint32_t)(ij / a.d1);
+    for (uint32_t r0 = 0; r0 < total; r0 += kStage) {
+        // the corners of ranks r0 .. r0 + kStage - 1 into LDS
+        if (mask && below < r0 + kStage && below + __popc(mask) > r0) {
+            // the set bits in order; the raster position advances by the
+            // gap to each (at most kPer steps in all)
+            uint32_t rank = below, k = k0, j = j0, i = i0, at = 0, mm = mask;
+            while (mm) {
+                const uint32_t s = (uint32_t)__ffs(mm) - 1u;
+                mm &= mm - 1u;
+                k += s - at;
+                at = s;
+                while (k >= a.d2) {
+                    k -= a.d2;
+                    if (++j == a.d1) {
+                        j = 0;
+                        i++;
+                    }
+                }
+                if (rank - r0 < kStage) {
+                    // datareader.c:66-68: i*vox_voxel[0] etc. in float
+                    float *c = stage + 3u * (rank - r0);
+                    c[0] = (float)i * a.v0;
+                    c[1] = (float)j * a.v1;
+                    c[2] = (float)k * a.v2;
+                }
+                rank++;
             }
-            out++;
         }
-        if (++k == a.d2) {
-            k = 0;
-            if (++j == a.d1) {
-                j = 0;
-                i++;
-            }
-        }
+        __syncthreads();
+        const uint32_t cnt = min(kStage, total - r0);
+        const uint64_t o0 = base + r0;
+        for (uint32_t t = threadIdx.x; t < 3u * cnt; t += 256u)
+            if (o0 + t / 3u < capacity) xyz[3u * o0 + t] = stage[t];
+        __syncthreads();
     }
 }
 
 template <int SS>
-int raw_select_ss(const RawArgs &a, uint32_t ntiles, uint32_t *tiles,
-                  unsigned long long *count, float *xyz, uint64_t capacity,
+void raw_count_ss(const RawArgs &a, uint32_t ntiles, const RawWork &w,
                   hipStream_t stream) {
     hipLaunchKernelGGL(raw_count_kernel<SS>, dim3(ntiles), dim3(256), 0,
-                       stream, a, tiles);
-    VOXG_HIPCHK(hipGetLastError());
-    hipLaunchKernelGGL(raw_scan_kernel, dim3(1), dim3(256), 0, stream, tiles,
-                       ntiles, count);
-    VOXG_HIPCHK(hipGetLastError());
-    if (xyz) {
-        hipLaunchKernelGGL(raw_write_kernel<SS>, dim3(ntiles), dim3(256), 0,
-                           stream, a, tiles, xyz, capacity);
-        VOXG_HIPCHK(hipGetLastError());
-    }
-    return 0;
+                       stream, a, w);
 }
 
 }  // namespace
@@ -264,43 +327,101 @@ static bool volume_samples(const unsigned dim[3], uint64_t limit,
     return true;
 }
 
+// Argument checks shared by the raw entry points: the sample count, or an
+// error.
+static int raw_samples(const unsigned dim[3], uint64_t *n) {
+    if (!dim) return fail(VOXG_EINVAL, "dim is NULL");
+    if (!volume_samples(dim, (1ull << 32) - kTile, n))
+        return fail(VOXG_ERANGE, "volume must hold fewer than 2^32 samples");
+    return 0;
+}
+
+size_t voxg_raw_workspace_bytes(const unsigned dim[3]) {
+    uint64_t n = 0;
+    if (!dim || !volume_samples(dim, (1ull << 32) - kTile, &n)) return 0;
+    return raw_work_bytes(n);
+}
+
+int voxg_raw_count(const void *samples, const unsigned dim[3],
+                   unsigned samplesize, uint64_t lo, uint64_t hi, void *work,
+                   unsigned long long *count_out, hipStream_t stream) {
+    uint64_t n = 0;
+    int rc = raw_samples(dim, &n);
+    if (rc) return rc;
+    if (samplesize < 1 || samplesize > 4)
+        return fail(VOXG_EINVAL, "samplesize must be 1, 2, 3 or 4");
+    if (!work) return fail(VOXG_EINVAL, "work is NULL");
+    if (n && !samples) return fail(VOXG_EINVAL, "samples is NULL");
+    if ((uintptr_t)work & 15u)
+        return fail(VOXG_EINVAL, "work must be 16-byte aligned");
+    const RawWork w = raw_work(work, n);
+    const uint32_t ntiles = raw_tiles(n), ngroups = raw_groups(n);
+    if (ntiles) {
+        RawArgs a;
+        a.data = static_cast<const uint8_t *>(samples);
+        a.n = n;
+        a.lo = lo;
+        a.hi = hi;
+        a.aligned = ((uintptr_t)samples & 15u) == 0;
+        switch (samplesize) {
+        case 1: raw_count_ss<1>(a, ntiles, w, stream); break;
+        case 2: raw_count_ss<2>(a, ntiles, w, stream); break;
+        case 3: raw_count_ss<3>(a, ntiles, w, stream); break;
+        default: raw_count_ss<4>(a, ntiles, w, stream); break;
+        }
+        VOXG_HIPCHK(hipGetLastError());
+        hipLaunchKernelGGL(raw_scan_kernel, dim3(ngroups), dim3(kGroup), 0,
+                           stream, w, ntiles);
+        VOXG_HIPCHK(hipGetLastError());
+    }
+    // the group offsets and the total (zero with no tile)
+    hipLaunchKernelGGL(raw_total_kernel, dim3(1), dim3(kTotalThreads), 0,
+                       stream, w, ntiles, ngroups, count_out);
+    VOXG_HIPCHK(hipGetLastError());
+    return 0;
+}
+
+int voxg_raw_write(const unsigned dim[3], const float voxel[3],
+                   const void *work, float *xyz_out, size_t capacity,
+                   hipStream_t stream) {
+    uint64_t n = 0;
+    int rc = raw_samples(dim, &n);
+    if (rc) return rc;
+    if (!work) return fail(VOXG_EINVAL, "work is NULL");
+    if ((uintptr_t)work & 15u)
+        return fail(VOXG_EINVAL, "work must be 16-byte aligned");
+    const uint32_t ntiles = raw_tiles(n);
+    if (!ntiles || capacity == 0) return 0;
+    if (!xyz_out) return fail(VOXG_EINVAL, "xyz_out is NULL");
+    RawOut a;
+    a.n = n;
+    a.d1 = dim[1];
+    a.d2 = dim[2];
+    a.v0 = voxel ? voxel[0] : 1.0f;
+    a.v1 = voxel ? voxel[1] : 1.0f;
+    a.v2 = voxel ? voxel[2] : 1.0f;
+    hipLaunchKernelGGL(raw_write_kernel, dim3(ntiles), dim3(256), 0, stream,
+                       a, raw_work(const_cast<void *>(work), n), xyz_out,
+                       (uint64_t)capacity);
+    VOXG_HIPCHK(hipGetLastError());
+    return 0;
+}
+
 int voxg_raw_select(const void *samples, const unsigned dim[3],
                     unsigned samplesize, uint64_t lo, uint64_t hi,
                     const float voxel[3], float *xyz_out, size_t capacity,
                     unsigned long long *count_out, hipStream_t stream) {
-    if (!dim || !count_out) return fail(VOXG_EINVAL, "dim or count_out is NULL");
-    if (samplesize < 1 || samplesize > 4)
-        return fail(VOXG_EINVAL, "samplesize must be 1, 2, 3 or 4");
+    if (!count_out) return fail(VOXG_EINVAL, "count_out is NULL");
     uint64_t n = 0;
-    if (!volume_samples(dim, (1ull << 32) - kTile, &n))
-        return fail(VOXG_ERANGE, "volume must hold fewer than 2^32 samples");
-    if (n && !samples) return fail(VOXG_EINVAL, "samples is NULL");
-    const uint32_t ntiles = (uint32_t)((n + kTile - 1) / kTile);
-    void *scr = nullptr;
-    int rc = voxg_internal::scratch(stream, (size_t)(ntiles + 1) * 4, &scr);
+    int rc = raw_samples(dim, &n);
     if (rc) return rc;
-    RawArgs a;
-    a.data = static_cast<const uint8_t *>(samples);
-    a.n = n;
-    a.d1 = dim[1];
-    a.d2 = dim[2];
-    a.lo = lo;
-    a.hi = hi;
-    a.v0 = voxel ? voxel[0] : 1.0f;
-    a.v1 = voxel ? voxel[1] : 1.0f;
-    a.v2 = voxel ? voxel[2] : 1.0f;
-    a.aligned = ((uintptr_t)samples & 15u) == 0;
-    if (n == 0) {
-        VOXG_HIPCHK(hipMemsetAsync(count_out, 0, sizeof(*count_out), stream));
-        return 0;
-    }
-    uint32_t *tiles = static_cast<uint32_t *>(scr);
-    switch (samplesize) {
-    case 1: return raw_select_ss<1>(a, ntiles, tiles, count_out, xyz_out, capacity, stream);
-    case 2: return raw_select_ss<2>(a, ntiles, tiles, count_out, xyz_out, capacity, stream);
-    case 3: return raw_select_ss<3>(a, ntiles, tiles, count_out, xyz_out, capacity, stream);
-    default: return raw_select_ss<4>(a, ntiles, tiles, count_out, xyz_out, capacity, stream);
-    }
+    void *scr = nullptr;
+    rc = voxg_internal::scratch(stream, raw_work_bytes(n), &scr);
+    if (rc) return rc;
+    rc = voxg_raw_count(samples, dim, samplesize, lo, hi, scr, count_out,
+                        stream);
+    if (rc || !xyz_out) return rc;
+    return voxg_raw_write(dim, voxel, scr, xyz_out, capacity, stream);
 }
 
 int voxg_read_raw_data(const char *filename, const unsigned dim[3],
@@ -331,6 +452,7 @@ int voxg_read_raw_data(const char *filename, const unsigned dim[3],
     uint8_t *host = nullptr, *dev = nullptr;
     unsigned long long *dcount = nullptr;
     float *dxyz = nullptr, *hxyz = nullptr;
+    void *work = nullptr;
     hipStream_t s = nullptr;
     int rc = 0;
     auto cleanup = [&]() {  // best effort: the first error is reported
@@ -343,6 +465,7 @@ int voxg_read_raw_data(const char *filename, const unsigned dim[3],
         if (dev) (void)hipFree(dev);
         if (dxyz) (void)hipFree(dxyz);
         if (dcount) (void)hipFree(dcount);
+        if (work) (void)hipFree(work);
         close(fd);
     };
 #define VOXG_TRY(expr)                                                     \
@@ -359,6 +482,7 @@ int voxg_read_raw_data(const char *filename, const unsigned dim[3],
     VOXG_TRY(hipHostMalloc((void **)&host, bytes ? bytes : 1));
     VOXG_TRY(hipMalloc((void **)&dev, bytes ? bytes : 16));
     VOXG_TRY(hipMalloc((void **)&dcount, sizeof(unsigned long long)));
+    VOXG_TRY(hipMalloc(&work, raw_work_bytes(n)));
     size_t got = 0;
     while (got < bytes) {
         ssize_t r = read(fd, host + got, bytes - got);
@@ -372,8 +496,7 @@ int voxg_read_raw_data(const char *filename, const unsigned dim[3],
     }
     VOXG_TRY(hipMemcpyAsync(dev, host, bytes, hipMemcpyHostToDevice, s));
     unsigned long long total = 0;
-    rc = voxg_raw_select(dev, dim, samplesize, lo, hi, voxel, nullptr, 0,
-                         dcount, s);
+    rc = voxg_raw_count(dev, dim, samplesize, lo, hi, work, dcount, s);
     if (rc) {
         cleanup();
         return rc;
@@ -384,8 +507,7 @@ int voxg_read_raw_data(const char *filename, const unsigned dim[3],
     if (total) {
         VOXG_TRY(hipMalloc((void **)&dxyz, (size_t)total * 3 * sizeof(float)));
         VOXG_TRY(hipHostMalloc((void **)&hxyz, (size_t)total * 3 * sizeof(float)));
-        rc = voxg_raw_select(dev, dim, samplesize, lo, hi, voxel, dxyz, total,
-                             dcount, s);
+        rc = voxg_raw_write(dim, voxel, work, dxyz, total, s);
         if (rc) {
             cleanup();
             return rc;

@@ -226,21 +226,28 @@ def raw_select(samples, dim, samplesize, lo=None, hi=None, threshold=None,
     samples is a device uint8 tensor of dim[0]*dim[1]*dim[2]*samplesize
     bytes; the test is lo <= sample < hi (or sample > threshold).  Returns
     the selected voxel corners as a device (m, 3) float tensor, in the
-    reader's order."""
+    reader's order: one pass over the samples (voxg_raw_count), the count
+    to the host, then the corners from the selection it recorded
+    (voxg_raw_write)."""
     L = _lib.load()
     lo, hi = _raw_range(samplesize, lo, hi, threshold)
     d = (C.c_uint * 3)(*[int(x) for x in dim])
     v = (C.c_float * 3)(*voxel)
-    cnt = torch.zeros(1, dtype=torch.int64, device="cuda")
+    nb = L.voxg_raw_workspace_bytes(d)
+    if nb == 0:
+        raise _lib.VoxgError("voxg_raw_workspace_bytes: the volume must hold "
+                             "fewer than 2^32 samples")
+    work = torch.empty(nb, dtype=torch.uint8, device="cuda")
+    cnt = torch.empty(1, dtype=torch.int64, device="cuda")
     st = _stream(stream)
-    check(L.voxg_raw_select(samples.data_ptr(), d, samplesize, lo, hi, v,
-                            None, 0, cnt.data_ptr(), st), "voxg_raw_select")
+    check(L.voxg_raw_count(samples.data_ptr(), d, samplesize, lo, hi,
+                           work.data_ptr(), cnt.data_ptr(), st),
+          "voxg_raw_count")
     m = int(cnt.item())
     xyz = torch.empty((m, 3), dtype=torch.float32, device="cuda")
     if m:
-        check(L.voxg_raw_select(samples.data_ptr(), d, samplesize, lo, hi, v,
-                                xyz.data_ptr(), m, cnt.data_ptr(), st),
-              "voxg_raw_select")
+        check(L.voxg_raw_write(d, v, work.data_ptr(), xyz.data_ptr(), m, st),
+              "voxg_raw_write")
     return xyz
 
 
