@@ -208,6 +208,7 @@ struct BView {
     static const float *center(N n) { return n->c; }
     static N child(N n, int i) { return n->child[i]; }
     const float *dot(N n, uint32_t k) const { return set + 4 * (n->off + k); }
+    const void *dots_ptr(N n) const { return set + 4 * n->off; }
     static const void *src(N) { return nullptr; }
 };
 
@@ -226,6 +227,7 @@ struct RView {
     static const float *center(N n) { return n->data.inner.center; }
     static N child(N n, int i) { return n->data.inner.children[i]; }
     const float *dot(N n, uint32_t k) const { return n->data.dots[k]; }
+    const void *dots_ptr(N n) const { return n->data.dots; }
     static const void *src(N n) { return n; }
 };
 
@@ -238,6 +240,82 @@ inline void dense_dims(const float *mn, const float *mx, const float V[3],
     }
 }
 
+// The flattener.  Node structures are scattered over the heap, so a walk
+// over them is bound by cache misses, one or two per node.  The
+// breadth-first order at each level is the concatenation, in order, of
+// the nodes at that level of the subtrees below the first level that
+// holds kSubtrees nodes; so the top levels are walked on one thread, and
+// the subtrees below them on a thread pool, each into its own per-level
+// lists, which are then placed at their offsets.
+constexpr size_t kSubtrees = 256;
+constexpr unsigned kMaxThreads = 16;
+
+// One node as a subtree walk records it, before its place is known.
+template <class N>
+struct Loc {
+    NodeRec r;      // box, centre, info; ref and escape come later
+    uint32_t nk;    // children (inner nodes)
+    uint32_t count;
+    const void *src;
+    const float *dots;  // regular leaf: its dots, 4 floats apart
+};
+
+struct Tally {
+    uint64_t inner = 0, leaves = 0, dense = 0, overflow = 0;
+    uint64_t ndots = 0, dense_ids = 0;
+    bool too_big = false;
+    void add(const Tally &o) {
+        inner += o.inner;
+        leaves += o.leaves;
+        dense += o.dense;
+        overflow += o.overflow;
+        ndots += o.ndots;
+        dense_ids += o.dense_ids;
+        too_big |= o.too_big;
+    }
+};
+
+// The record of node n (ref and escape left for the caller); counts it.
+template <class View>
+Loc<typename View::N> record(const View &view, typename View::N n,
+                             const float V[3], Tally &t) {
+    Loc<typename View::N> l;
+    std::memset(&l.r, 0, sizeof(l.r));
+    for (int k = 0; k < 3; k++) {
+        l.r.bmin[k] = View::bmin(n)[k];
+        l.r.bmax[k] = View::bmax(n)[k];
+    }
+    const uint32_t f = View::flags(n);
+    l.r.info = f & (NT_TYPE_MASK | NT_OVERFLOW);
+    l.r.ref = kEnd;
+    l.nk = 0;
+    l.count = View::count(n);
+    l.src = View::src(n);
+    l.dots = nullptr;
+    if ((f & NT_TYPE_MASK) == NT_INNER) {
+        for (int k = 0; k < 3; k++) l.r.c[k] = View::center(n)[k];
+        uint32_t mask = 0;
+        for (int c = 0; c < 8; c++)
+            if (!View::null(View::child(n, c))) mask |= 1u << c;
+        l.nk = (uint32_t)__builtin_popcount(mask);
+        l.r.info |= mask << 8;
+        t.inner++;
+    } else if ((f & NT_TYPE_MASK) == NT_LEAF) {
+        if (l.count > kMaxLeafDots) t.too_big = true;
+        l.r.info |= (l.count & kMaxLeafDots) << 8;
+        if (l.count) l.dots = view.dot(n, 0);
+        t.ndots += l.count;
+        t.leaves++;
+        if (f & NT_OVERFLOW) t.overflow++;
+    } else {
+        uint32_t dim[3];
+        dense_dims(View::bmin(n), View::bmax(n), V, dim);
+        t.dense_ids += (uint64_t)dim[0] * dim[1] * dim[2];
+        t.dense++;
+    }
+    return l;
+}
+
 template <class View>
 int flatten(const View &view, typename View::N root, const float V[3],
             HostTree &out, std::string &err) {
@@ -245,128 +323,246 @@ int flatten(const View &view, typename View::N root, const float V[3],
     out = HostTree();
     for (int l = 0; l < 3; l++) out.voxel[l] = V[l];
     if (View::null(root)) return 0;
+    Tally tally;
 
-    // 1. depth-first preorder: voxel ids (dots first, then dense voxels)
-    struct Acc {
-        uint64_t dots = 0, dense_ids = 0;
-    } acc;
-    {
-        std::vector<std::pair<N, uint32_t>> st{{root, 1}};
-        while (!st.empty()) {
-            N n = st.back().first;
-            uint32_t d = st.back().second;
-            st.pop_back();
-            out.depth = std::max(out.depth, d);
-            uint32_t f = View::flags(n);
-            if ((f & NT_TYPE_MASK) == NT_LEAF) {
-                if (View::count(n) > kMaxLeafDots) {
-                    err = "leaf with more than 2^24-1 voxels";
-                    return -34;
-                }
-                acc.dots += View::count(n);
-            } else if ((f & NT_TYPE_MASK) == NT_DENSE) {
-                uint32_t dim[3];
-                dense_dims(View::bmin(n), View::bmax(n), V, dim);
-                acc.dense_ids += (uint64_t)dim[0] * dim[1] * dim[2];
-            } else {
-                for (int i = 7; i >= 0; i--)
-                    if (!View::null(View::child(n, i)))
-                        st.push_back({View::child(n, i), d + 1});
+    // 1. the top levels on this thread, breadth first, until the next
+    //    level holds kSubtrees nodes (or there is none): records and
+    //    escape links of levels 1..lt; the next level's nodes are the
+    //    subtree roots, with their escape links
+    std::vector<Loc<N>> top;
+    std::vector<uint32_t> top_esc{kEnd};
+    std::vector<N> front{root}, next;
+    std::vector<uint32_t> front_esc{kEnd}, next_esc;
+    uint32_t lt = 0;
+    for (;;) {
+        if (front.size() >= kSubtrees && lt > 0) break;
+        // move the front into the top and expand it
+        const uint32_t base = (uint32_t)(top.size() + front.size());
+        next.clear();
+        next_esc.clear();
+        for (size_t i = 0; i < front.size(); i++) {
+            N n = front[i];
+            top.push_back(record(view, n, V, tally));
+            if (i > 0 || lt > 0) top_esc.push_back(front_esc[i]);
+            const uint32_t nk = top.back().nk;
+            uint32_t k = 0;
+            for (int c = 0; c < 8 && nk; c++) {
+                N ch = View::child(n, c);
+                if (View::null(ch)) continue;
+                next.push_back(ch);
+                const uint32_t g = base + (uint32_t)next.size() - 1u;
+                next_esc.push_back(k + 1 < nk ? g + 1u : front_esc[i]);
+                k++;
             }
         }
+        lt++;
+        front.swap(next);
+        front_esc.swap(next_esc);
+        if (front.empty()) break;
     }
-    if (acc.dots + acc.dense_ids >= 0xffffffffull) {
+    const size_t T = top.size(), S = front.size();
+
+    // 2. the subtrees, on a pool: per subtree, per level, its nodes'
+    //    records in breadth-first order
+    std::vector<std::vector<std::vector<Loc<N>>>> sub(S);
+    std::vector<Tally> tallies;
+    {
+        const unsigned hw = std::max(1u, std::thread::hardware_concurrency());
+        const unsigned nt =
+            (unsigned)std::min<size_t>(std::min(hw, kMaxThreads), S);
+        tallies.resize(std::max(1u, nt));
+        std::atomic<size_t> next_sub{0};
+        auto work = [&](unsigned w) {
+            Tally &t = tallies[w];
+            std::vector<N> lv, nx;
+            for (;;) {
+                const size_t s = next_sub.fetch_add(1);
+                if (s >= S) break;
+                auto &levels = sub[s];
+                lv.assign(1, front[s]);
+                while (!lv.empty()) {
+                    levels.emplace_back();
+                    auto &rec = levels.back();
+                    rec.reserve(lv.size());
+                    nx.clear();
+                    for (N n : lv) {
+                        rec.push_back(record(view, n, V, t));
+                        if (rec.back().nk)
+                            for (int c = 0; c < 8; c++) {
+                                N ch = View::child(n, c);
+                                if (!View::null(ch)) nx.push_back(ch);
+                            }
+                    }
+                    lv.swap(nx);
+                }
+            }
+        };
+        if (nt <= 1) {
+            work(0);
+        } else {
+            std::vector<std::thread> pool;
+            for (unsigned w = 1; w < nt; w++) pool.emplace_back(work, w);
+            work(0);
+            for (auto &th : pool) th.join();
+        }
+    }
+    for (const Tally &t : tallies) tally.add(t);
+    if (tally.too_big) {
+        err = "leaf with more than 2^24-1 voxels";
+        return -34;
+    }
+    if (tally.ndots + tally.dense_ids >= 0xffffffffull) {
         err = "more than 2^32-1 voxels in one tree";
         return -34;
     }
 
-    // 2. breadth-first records; children of a node are contiguous
-    std::vector<N> q;
-    q.push_back(root);
-    out.nodes.reserve(1024);
-    std::vector<uint32_t> esc(1, kEnd);
-    for (size_t i = 0; i < q.size(); i++) {
-        N n = q[i];
-        NodeRec r;
-        std::memset(&r, 0, sizeof(r));
-        for (int l = 0; l < 3; l++) {
-            r.bmin[l] = View::bmin(n)[l];
-            r.bmax[l] = View::bmax(n)[l];
+    // 3. places: level d of the subtrees is level lt + 1 + d of the tree;
+    //    a subtree's nodes at a level follow those of the subtrees before
+    size_t depth_sub = 0;
+    for (const auto &levels : sub) depth_sub = std::max(depth_sub, levels.size());
+    std::vector<std::vector<uint64_t>> off(S, std::vector<uint64_t>(depth_sub));
+    uint64_t total = T;
+    for (size_t d = 0; d < depth_sub; d++)
+        for (size_t s = 0; s < S; s++) {
+            off[s][d] = total;
+            if (d < sub[s].size()) total += sub[s][d].size();
         }
-        uint32_t f = View::flags(n);
-        r.info = f & (NT_TYPE_MASK | NT_OVERFLOW);
-        r.escape = esc[i];
-        if ((f & NT_TYPE_MASK) == NT_INNER) {
-            for (int l = 0; l < 3; l++) r.c[l] = View::center(n)[l];
-            uint32_t mask = 0, first = (uint32_t)q.size(), k = 0;
-            for (int c = 0; c < 8; c++)
-                if (!View::null(View::child(n, c))) mask |= 1u << c;
-            uint32_t nk = (uint32_t)__builtin_popcount(mask);
-            for (int c = 0; c < 8; c++) {
-                N ch = View::child(n, c);
-                if (View::null(ch)) continue;
-                q.push_back(ch);
-                esc.push_back(k + 1 < nk ? first + k + 1 : esc[i]);
-                k++;
-            }
-            r.ref = nk ? first : kEnd;
-            r.info |= mask << 8;
-            out.inner++;
-        } else if ((f & NT_TYPE_MASK) == NT_LEAF) {
-            r.info |= View::count(n) << 8;
-            out.leaves++;
-            if (f & NT_OVERFLOW) out.overflow++;
-        } else {
-            out.dense++;
-        }
-        out.nodes.push_back(r);
-        out.counts.push_back(View::count(n));
-        out.src.push_back(View::src(n));
-        if (q.size() >= 0xfffffff0ull) {
-            err = "too many nodes";
-            return -34;
-        }
+    if (total >= 0xfffffff0ull) {
+        err = "too many nodes";
+        return -34;
     }
-
-    // 3. leaf refs in canonical id order (second preorder walk)
-    std::vector<uint32_t> index_of;  // preorder position -> record index
+    out.depth = lt + (uint32_t)depth_sub;
+    out.nodes.resize(total);
+    out.counts.resize(total);
+    out.src.resize(total);
+    std::vector<const float *> dots(total, nullptr);
+    auto place = [&](const Loc<N> &l, uint64_t g, uint32_t ref, uint32_t esc) {
+        NodeRec r = l.r;
+        r.ref = ref;
+        r.escape = esc;
+        out.nodes[g] = r;
+        out.counts[g] = l.count;
+        out.src[g] = l.src;
+        dots[g] = l.dots;
+    };
+    // the top: children are numbered in order from 1 (breadth first)
     {
-        // map node -> record index through the BFS order
-        out.dots.resize(3 * acc.dots);
-        uint64_t dot_next = 0, dense_next = acc.dots;
-        // preorder over records: children are [ref, ref+popcount)
-        std::vector<uint32_t> st{0};
-        while (!st.empty()) {
-            uint32_t i = st.back();
-            st.pop_back();
-            NodeRec &r = out.nodes[i];
-            uint32_t t = r.info & NT_TYPE_MASK;
-            if (t == NT_LEAF) {
-                N n = q[i];
-                r.ref = (uint32_t)dot_next;
-                uint32_t cnt = r.info >> 8;
-                for (uint32_t k = 0; k < cnt; k++) {
-                    const float *p = view.dot(n, k);
-                    out.dots[3 * (dot_next + k) + 0] = p[0];
-                    out.dots[3 * (dot_next + k) + 1] = p[1];
-                    out.dots[3 * (dot_next + k) + 2] = p[2];
-                }
-                dot_next += cnt;
-            } else if (t == NT_DENSE) {
-                uint32_t dim[3];
-                dense_dims(r.bmin, r.bmax, V, dim);
-                r.ref = (uint32_t)dense_next;
-                dense_next += (uint64_t)dim[0] * dim[1] * dim[2];
-                // the device reads the dimensions from the centre slot
-                std::memcpy(r.c, dim, sizeof(dim));
-            } else {
-                uint32_t nk = (uint32_t)__builtin_popcount((r.info >> 8) & 0xff);
-                for (int k = (int)nk - 1; k >= 0; k--) st.push_back(r.ref + k);
-            }
+        uint32_t next_child = 1;
+        for (size_t i = 0; i < T; i++) {
+            const uint32_t ref = top[i].nk ? next_child : kEnd;
+            next_child += top[i].nk;
+            place(top[i], i, ref, top_esc[i]);
         }
     }
-    out.ndots = acc.dots;
-    out.voxel_ids = acc.dots + acc.dense_ids;
+    // the subtrees, in parallel again
+    {
+        const unsigned hw = std::max(1u, std::thread::hardware_concurrency());
+        const unsigned nt =
+            (unsigned)std::min<size_t>(std::min(hw, kMaxThreads), S);
+        std::atomic<size_t> next_sub{0};
+        auto work = [&]() {
+            std::vector<uint32_t> esc, nesc;
+            for (;;) {
+                const size_t s = next_sub.fetch_add(1);
+                if (s >= S) break;
+                const auto &levels = sub[s];
+                esc.assign(1, front_esc[s]);
+                for (size_t d = 0; d < levels.size(); d++) {
+                    nesc.clear();
+                    uint64_t child = d + 1 < levels.size() ? off[s][d + 1] : 0;
+                    for (size_t p = 0; p < levels[d].size(); p++) {
+                        const Loc<N> &l = levels[d][p];
+                        const uint32_t ref = l.nk ? (uint32_t)child : kEnd;
+                        for (uint32_t k = 0; k < l.nk; k++)
+                            nesc.push_back(k + 1 < l.nk
+                                               ? (uint32_t)(child + k + 1)
+                                               : esc[p]);
+                        child += l.nk;
+                        place(l, off[s][d] + p, ref, esc[p]);
+                    }
+                    esc.swap(nesc);
+                }
+            }
+        };
+        if (nt <= 1) {
+            work();
+        } else {
+            std::vector<std::thread> pool;
+            for (unsigned w = 1; w < nt; w++) pool.emplace_back(work);
+            work();
+            for (auto &th : pool) th.join();
+        }
+    }
+    out.inner = tally.inner;
+    out.leaves = tally.leaves;
+    out.dense = tally.dense;
+    out.overflow = tally.overflow;
+
+    // 4. leaf refs and voxel ids in canonical order: a depth-first preorder
+    //    over the records (children are [ref, ref + popcount)); dots first,
+    //    then dense voxels.  The leaves' dots are copied afterwards, on the
+    //    pool (each leaf's source is a cache miss of its own)
+    const uint64_t ndots = tally.ndots;
+    out.dots.resize(3 * ndots);
+    uint64_t dot_next = 0, dense_next = ndots;
+    std::vector<uint32_t> st{0}, leaves;
+    leaves.reserve(tally.leaves);
+    while (!st.empty()) {
+        uint32_t i = st.back();
+        st.pop_back();
+        NodeRec &r = out.nodes[i];
+        uint32_t t = r.info & NT_TYPE_MASK;
+        if (t == NT_LEAF) {
+            r.ref = (uint32_t)dot_next;
+            dot_next += r.info >> 8;
+            leaves.push_back(i);
+        } else if (t == NT_DENSE) {
+            uint32_t dim[3];
+            dense_dims(r.bmin, r.bmax, V, dim);
+            r.ref = (uint32_t)dense_next;
+            dense_next += (uint64_t)dim[0] * dim[1] * dim[2];
+            // the device reads the dimensions from the centre slot
+            std::memcpy(r.c, dim, sizeof(dim));
+        } else {
+            uint32_t nk = (uint32_t)__builtin_popcount((r.info >> 8) & 0xff);
+            for (int k = (int)nk - 1; k >= 0; k--) st.push_back(r.ref + k);
+        }
+    }
+    {
+        const unsigned hw = std::max(1u, std::thread::hardware_concurrency());
+        const size_t chunk = 1024;
+        const unsigned nt = (unsigned)std::min<size_t>(
+            std::min(hw, kMaxThreads), (leaves.size() + chunk - 1) / chunk);
+        std::atomic<size_t> next_leaf{0};
+        auto work = [&]() {
+            for (;;) {
+                const size_t b = next_leaf.fetch_add(chunk);
+                if (b >= leaves.size()) break;
+                const size_t e = std::min(leaves.size(), b + chunk);
+                for (size_t j = b; j < e; j++) {
+                    const NodeRec &r = out.nodes[leaves[j]];
+                    const uint32_t cnt = r.info >> 8;
+                    const float *p = dots[leaves[j]];
+                    float *q = out.dots.data() + 3 * (size_t)r.ref;
+                    for (uint32_t k = 0; k < cnt; k++) {
+                        q[3 * k + 0] = p[4 * k + 0];
+                        q[3 * k + 1] = p[4 * k + 1];
+                        q[3 * k + 2] = p[4 * k + 2];
+                    }
+                }
+            }
+        };
+        if (nt <= 1) {
+            work();
+        } else {
+            std::vector<std::thread> pool;
+            for (unsigned w = 1; w < nt; w++) pool.emplace_back(work);
+            work();
+            for (auto &th : pool) th.join();
+        }
+    }
+    out.ndots = ndots;
+    out.voxel_ids = ndots + tally.dense_ids;
     return 0;
 }
 
