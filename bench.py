@@ -410,6 +410,17 @@ def serving_leg(log, seconds, rate, seed=31):
         px = sum(p for k, p, _ in done if k == "r")
         nb = sum(p for k, p, _ in done if k == "b")
         sizes_g = [i["group_size"] for _, _, i in done]
+        # where the time goes: waiting for admission (the scheduler thread
+        # forming and launching groups) and service (launch to results)
+        wait = np.array([i["t_admit"] - i["t_submit"] for _, _, i in done])
+        serve = np.array([i["t_done"] - i["t_admit"] for _, _, i in done])
+        l4k = np.array([i["t_done"] - i["t_submit"] for k, p, i in done
+                        if k == "r" and p >= 3840 * 2160]) * 1e3
+        # the same over the first and the last half second of submissions
+        t_sub = np.array([i["t_submit"] for _, _, i in done])
+        t_first = t_sub.min()
+        early = lat[t_sub - t_first < 0.5]
+        late = lat[t_sub - t_first >= t_sub.max() - t_first - 0.5]
         return {"requests": len(done), "span_s": round(span, 3),
                 "p50_ms": round(float(np.percentile(lat, 50)), 3),
                 "p99_ms": round(float(np.percentile(lat, 99)), 3),
@@ -420,17 +431,29 @@ def serving_leg(log, seconds, rate, seed=31):
                 "sustained_mrays_s": round(px / span / 1e6, 1),
                 "sustained_mballs_s": round(nb / span / 1e6, 1),
                 "requests_per_s": round(len(done) / span, 1),
-                "mean_group_size": round(float(np.mean(sizes_g)), 2)}
+                "mean_group_size": round(float(np.mean(sizes_g)), 2),
+                "admit_wait_p50_ms": round(float(np.percentile(wait, 50)) * 1e3, 3),
+                "admit_wait_p99_ms": round(float(np.percentile(wait, 99)) * 1e3, 3),
+                "service_p50_ms": round(float(np.percentile(serve, 50)) * 1e3, 3),
+                "service_p99_ms": round(float(np.percentile(serve, 99)) * 1e3, 3),
+                "render_3840x2160_p50_ms": round(float(np.percentile(l4k, 50)), 3)
+                if len(l4k) else None,
+                "first_half_second_p99_ms": round(float(np.percentile(early, 99)), 3),
+                "last_half_second_p99_ms": round(float(np.percentile(late, 99)), 3)}
 
+    # capacity: the requests submitted without pacing (in flight at most as
+    # many as there are result buffers, 200).  It runs first, so that every
+    # stream has grown its buffers for the largest frames before the paced
+    # run measures latency (growing one waits for the stream: a cold
+    # server's first half second has a p99 of about 22 ms)
+    sat = run(paced=False)
+    st = S.stats()
+    sat["peak_groups_inflight"] = st["peak_groups_inflight"]
+    sat["peak_inflight"] = st["peak_inflight"]
     res = run(paced=True)
     res["offered_rate_per_s"] = rate
-    st = S.stats()
-    res["groups"] = st["groups"]
-    res["peak_groups_inflight"] = st["peak_groups_inflight"]
-    res["peak_inflight"] = st["peak_inflight"]
-    # capacity: the same requests submitted without pacing (in flight at
-    # most as many as there are result buffers, 200)
-    res["saturation"] = run(paced=False)
+    res["groups"] = S.stats()["groups"] - st["groups"]
+    res["saturation"] = sat
     S.close()
     log("[bench] serving leg: %s" % json.dumps(res))
     return res

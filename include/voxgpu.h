@@ -361,7 +361,9 @@ int voxg_batch_release(voxg_batch *batch);
  * others run overlap on the device.  Results are copied into the caller's
  * host buffers (pass pinned memory, e.g. hipHostMalloc, for asynchronous
  * copies) and equal those of voxg_render / voxg_ball_batch on the same
- * inputs.  A request that can never fit the budget is rejected
+ * inputs.  Ball coordinates are copied at submission into 64 MiB of
+ * pinned host memory the server owns (into pageable memory while that is
+ * full).  A request that can never fit the budget is rejected
  * (VOXG_ENOMEM). */
 typedef struct voxg_server voxg_server;
 typedef struct {

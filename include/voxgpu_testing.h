@@ -25,6 +25,13 @@ int voxg_test_fail_at(int checkpoint);
  * report.  Every access stays inside the buffers sized for the cap. */
 int voxg_test_stack_bound(unsigned bound);
 
+/* While hold is nonzero the server admits no request: submissions queue
+ * up, and the first admission after the hold is lifted takes them all
+ * as one group (up to the slot and memory limits), so that a test can
+ * form a group of a chosen mix. */
+struct voxg_server;
+int voxg_test_server_hold(struct voxg_server *s, int hold);
+
 #ifdef __cplusplus
 }
 #endif

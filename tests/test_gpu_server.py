@@ -111,6 +111,61 @@ def test_alone_equals_in_mix(gpu, oracle):
     S.close()
 
 
+def test_shared_small_frame_launch_equals_alone(gpu):
+    """A group with at least 8 small frames renders them in one shared
+    multi-job launch, the others through the render path: formed on
+    purpose by holding admission while 12 small frames of every quality,
+    two large frames and two ball batches queue up.  Every result equals
+    the same request run alone (a group of one: the render path)."""
+    from voxvision_amd import _lib, scenes, server
+    L = gpu
+    c1, t1, t2, dots2 = _setup()
+    tex = np.random.default_rng(4).integers(0, 256, 32 ** 3).astype(np.uint8)
+    S = server.Server(streams=2, texture=tex)
+    ids = [S.add_tree(t1), S.add_tree(t2)]
+    dots = [c1["dots"], dots2]
+    rng = np.random.default_rng(11)
+    jobs = []
+    for k in range(12):
+        W, H = [(320, 240), (640, 480)][k % 2]
+        t = k % 2
+        jobs.append(("r", ids[t], _cam_for(dots[t], W, H, 100 + k), W, H,
+                     [0, 1, 2, 6, 10][k % 5]))
+    for k in range(2):
+        jobs.append(("r", ids[k], _cam_for(dots[k], 1280, 720, 200 + k),
+                     1280, 720, [2, 1][k]))
+    for k in range(2):
+        c, r = scenes.random_balls(int(rng.integers(1000, 50000)),
+                                   *scenes.bbox_of(dots[k]), seed=300 + k)
+        jobs.append(("b", ids[k], c, r))
+
+    def submit(j):
+        if j[0] == "r":
+            return S.submit_render(j[1], j[2], j[3], j[4], j[5], colors=True)
+        return S.submit_balls(j[1], j[2], j[3])
+
+    _lib.check(L.voxg_test_server_hold(S._h, 1))
+    held = [submit(j) for j in jobs]
+    _lib.check(L.voxg_test_server_hold(S._h, 0))
+    infos = [S.wait(tk, 60.0) for tk, _ in held]
+    assert all(i["status"] == 0 for i in infos)
+    assert all(i["group_size"] == len(jobs) for i in infos)
+    for j, (tk, o) in zip(jobs, held):
+        tk2, o2 = submit(j)
+        info = S.wait(tk2, 60.0)
+        assert info["status"] == 0 and info["group_size"] == 1
+        if j[0] == "b":
+            assert torch.equal(o, o2)
+            continue
+        assert torch.equal(o["voxel"], o2["voxel"]), j[3:]
+        assert torch.equal(o["colors"], o2["colors"]), j[3:]
+        m = o["voxel"] != -1
+        assert m.any()
+        assert torch.equal(o["hit"].view(3, -1)[:, m],
+                           o2["hit"].view(3, -1)[:, m]), j[3:]
+    S.close()
+
+
 def test_admission_by_memory_and_slots(gpu):
     from voxvision_amd import server
     c1, t1, _, _ = _setup()

@@ -36,7 +36,8 @@ EXPORTS = [
     "voxg_edit_rebuild", "voxg_edit_root", "voxg_edit_voxels",
     "voxg_edit_commit", "voxg_edit_release", "voxg_server_replace_tree",
     "voxg_synchronize", "voxg_release_stream", "voxg_test_fail_at",
-    "voxg_test_stack_bound", "voxg_ray_batch_noalloc",
+    "voxg_test_stack_bound", "voxg_test_server_hold",
+    "voxg_ray_batch_noalloc",
     "voxg_lights_create", "voxg_lights_release", "voxg_lights_set_ambient",
     "voxg_lights_insert", "voxg_lights_delete", "voxg_lights_clear",
     "voxg_lights_count", "voxg_lights_get",
@@ -176,6 +177,7 @@ def load():
     L.voxg_lights_get.argtypes = [P, S, P, P]
     L.voxg_test_fail_at.argtypes = [I]
     L.voxg_test_stack_bound.argtypes = [C.c_uint]
+    L.voxg_test_server_hold.argtypes = [P, C.c_int]
     _lib = L
     return L
 

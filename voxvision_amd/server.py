@@ -149,7 +149,9 @@ class Server:
         c = np.ascontiguousarray(np.asarray(c, np.float32).T)
         r = np.ascontiguousarray(r, np.float32)
         n = len(r)
-        out = torch.zeros(n, dtype=torch.uint8, pin_memory=True)
+        # every byte is written by the server (an empty tensor: a zero fill
+        # would run on torch's CPU thread pool on the submitting thread)
+        out = torch.empty(n, dtype=torch.uint8, pin_memory=True)
         q = BallRequest()
         q.tree = tree_id
         q.n = n
