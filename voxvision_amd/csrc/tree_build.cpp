@@ -20,6 +20,8 @@
 #include <cstring>
 #include <deque>
 #include <mutex>
+#include <new>
+#include <system_error>
 #include <thread>
 #include <vector>
 
@@ -80,6 +82,8 @@ struct Builder {
     float *set;  // n x 4 working array (x, y, z, pad)
     NodePool pool;
     std::atomic<int> spare_threads{0};
+    // an allocation failed on a helper thread (rethrown once all are done)
+    std::atomic<bool> oom{false};
     std::atomic<bool> too_big{false};
 
     // tree.c:57-72
@@ -180,11 +184,21 @@ struct Builder {
         bool spawned[8] = {false};
         for (int idx = 0; idx < 8; idx++) {
             if (len[idx] >= (1u << 15) && spare_threads.fetch_sub(1) > 0) {
-                spawned[idx] = true;
-                th[idx] = std::thread([this, node, idx, &start, &len, off] {
-                    node->child[idx] = build(off + start[idx], len[idx]);
+                try {
+                    th[idx] = std::thread([this, node, idx, &start, &len, off] {
+                        try {
+                            node->child[idx] = build(off + start[idx], len[idx]);
+                        } catch (const std::bad_alloc &) {
+                            oom = true;
+                        }
+                        spare_threads.fetch_add(1);
+                    });
+                    spawned[idx] = true;
+                } catch (const std::system_error &) {
+                    // no thread to be had: this one builds the child
                     spare_threads.fetch_add(1);
-                });
+                    node->child[idx] = build(off + start[idx], len[idx]);
+                }
             } else {
                 if (len[idx] >= (1u << 15)) spare_threads.fetch_add(1);
                 node->child[idx] = build(off + start[idx], len[idx]);
@@ -249,6 +263,33 @@ inline void dense_dims(const float *mn, const float *mx, const float V[3],
 // lists, which are then placed at their offsets.
 constexpr size_t kSubtrees = 256;
 constexpr unsigned kMaxThreads = 16;
+
+// Runs work(w) on the calling thread (w = 0) and on up to nt - 1 more
+// threads; the work draws its items from a shared counter, so fewer
+// threads (when one cannot be started) only take longer.  An allocation
+// failure in any of them is rethrown here, as the callers expect.
+template <class F>
+void run_pool(unsigned nt, F &&work) {
+    std::atomic<bool> oom{false};
+    auto guarded = [&](unsigned w) {
+        try {
+            work(w);
+        } catch (const std::bad_alloc &) {
+            oom = true;
+        }
+    };
+    std::vector<std::thread> pool;
+    for (unsigned w = 1; w < nt; w++) {
+        try {
+            pool.emplace_back(guarded, w);
+        } catch (const std::system_error &) {
+            break;
+        }
+    }
+    guarded(0);
+    for (auto &th : pool) th.join();
+    if (oom) throw std::bad_alloc();
+}
 
 // One node as a subtree walk records it, before its place is known.
 template <class N>
@@ -397,14 +438,7 @@ int flatten(const View &view, typename View::N root, const float V[3],
                 }
             }
         };
-        if (nt <= 1) {
-            work(0);
-        } else {
-            std::vector<std::thread> pool;
-            for (unsigned w = 1; w < nt; w++) pool.emplace_back(work, w);
-            work(0);
-            for (auto &th : pool) th.join();
-        }
+        run_pool(nt, work);
     }
     for (const Tally &t : tallies) tally.add(t);
     if (tally.too_big) {
@@ -460,7 +494,7 @@ int flatten(const View &view, typename View::N root, const float V[3],
         const unsigned nt =
             (unsigned)std::min<size_t>(std::min(hw, kMaxThreads), S);
         std::atomic<size_t> next_sub{0};
-        auto work = [&]() {
+        auto work = [&](unsigned) {
             std::vector<uint32_t> esc, nesc;
             for (;;) {
                 const size_t s = next_sub.fetch_add(1);
@@ -484,14 +518,7 @@ int flatten(const View &view, typename View::N root, const float V[3],
                 }
             }
         };
-        if (nt <= 1) {
-            work();
-        } else {
-            std::vector<std::thread> pool;
-            for (unsigned w = 1; w < nt; w++) pool.emplace_back(work);
-            work();
-            for (auto &th : pool) th.join();
-        }
+        run_pool(nt, work);
     }
     out.inner = tally.inner;
     out.leaves = tally.leaves;
@@ -534,7 +561,7 @@ int flatten(const View &view, typename View::N root, const float V[3],
         const unsigned nt = (unsigned)std::min<size_t>(
             std::min(hw, kMaxThreads), (leaves.size() + chunk - 1) / chunk);
         std::atomic<size_t> next_leaf{0};
-        auto work = [&]() {
+        auto work = [&](unsigned) {
             for (;;) {
                 const size_t b = next_leaf.fetch_add(chunk);
                 if (b >= leaves.size()) break;
@@ -552,14 +579,7 @@ int flatten(const View &view, typename View::N root, const float V[3],
                 }
             }
         };
-        if (nt <= 1) {
-            work();
-        } else {
-            std::vector<std::thread> pool;
-            for (unsigned w = 1; w < nt; w++) pool.emplace_back(work);
-            work();
-            for (auto &th : pool) th.join();
-        }
+        run_pool(nt, work);
     }
     out.ndots = ndots;
     out.voxel_ids = ndots + tally.dense_ids;
@@ -592,6 +612,7 @@ int build_tree(const float *xyz, size_t n, size_t stride, const float V[3],
     int hw = (int)std::thread::hardware_concurrency();
     b.spare_threads = std::max(0, std::min(hw, 64) - 1);
     BNode *root = b.build(0, n);
+    if (b.oom) throw std::bad_alloc();
     if (b.too_big) {
         err = "leaf with more than 2^24-1 voxels";
         return -34;
@@ -653,8 +674,25 @@ void destroy_reference(struct vox_node *t) {
     std::free(t);
 }
 
+static int build_reference_(const float *xyz, size_t n, size_t stride,
+                            const float V[3], struct vox_node **root,
+                            std::string &err);
+
+// The edit API calls this without a handler of its own: an allocation
+// failure becomes a status here.
 int build_reference(const float *xyz, size_t n, size_t stride, const float V[3],
                     struct vox_node **root, std::string &err) {
+    try {
+        return build_reference_(xyz, n, stride, V, root, err);
+    } catch (const std::bad_alloc &) {
+        err = "host allocation failed";
+        return -12;
+    }
+}
+
+static int build_reference_(const float *xyz, size_t n, size_t stride,
+                            const float V[3], struct vox_node **root,
+                            std::string &err) {
     *root = nullptr;
     if (n == 0) return 0;
     if (n >= 0xffffffffull) {
@@ -674,6 +712,7 @@ int build_reference(const float *xyz, size_t n, size_t stride, const float V[3],
     int hw = (int)std::thread::hardware_concurrency();
     b.spare_threads = std::max(0, std::min(hw, 64) - 1);
     BNode *broot = b.build(0, n);
+    if (b.oom) throw std::bad_alloc();
     *root = to_reference(broot, work.data());
     if (!*root) {
         err = "host allocation failed";
