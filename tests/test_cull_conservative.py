@@ -135,3 +135,148 @@ def test_may_enter_never_rejects_an_exact_hit(oracle):
         # tight (loose answers are rays within the margin, mostly grazing)
         assert exact > total // 10
         assert loose < total // 10
+
+
+BLOCK_CHECK = r"""
+#define __host__
+#define __device__
+#include "cull.h"
+#include <math.h>
+#include <stdio.h>
+#include <stdint.h>
+extern "C" {
+#include "vox_oracle.h"
+}
+
+static uint64_t rng = 0x2545f4914f6cdd1dull;
+static uint32_t u32(void) {
+    rng ^= rng << 13; rng ^= rng >> 7; rng ^= rng << 17;
+    return (uint32_t)(rng >> 16);
+}
+static float uni(float a, float b) { return a + (b - a) * (u32() / 4294967296.0f); }
+
+int main(void) {
+    orc_set_voxel(1.0f, 1.0f, 1.0f);
+    long blocks = 0, outside = 0, bad = 0, culled = 0, hit_blocks = 0;
+    const float scales[] = {1.0f, 64.0f, 4096.0f, 1e5f};
+    for (int it = 0; it < 4000; it++) {
+        const float sc = scales[it % 4];
+        orc_camera cam;
+        float q[4], n = 0.0f;
+        for (int k = 0; k < 4; k++) q[k] = uni(-1, 1);
+        if (it % 3 == 1) {
+            // close to the identity or to a quarter turn about an axis:
+            // some direction components then hardly vary across a block,
+            // and the rounding bound E is what keeps them in the box
+            const int a = (int)(u32() % 4);
+            for (int k = 0; k < 4; k++)
+                q[k] = ldexpf(uni(-1, 1), -(int)(6 + u32() % 18));
+            q[0] += 1.0f;
+            if (a) q[a] += 1.0f;
+        }
+        for (int k = 0; k < 4; k++) n += q[k] * q[k];
+        n = sqrtf(n);
+        // unit quaternions, and every fifth camera a scaled one
+        const float s = (it % 5 == 0) ? uni(0.5f, 2.0f) : 1.0f;
+        for (int k = 0; k < 4; k++) cam.rotation[k] = s * q[k] / n;
+        cam.fov = uni(0.1f, 2.0f);
+        const int W = (it % 7 == 0) ? 3840 : 16 * (1 + (int)(u32() % 8));
+        const int H = (it % 7 == 0) ? 2160 : 4 * (1 + (int)(u32() % 16));
+        orc_camera_set_window_size(&cam, W, H);
+        for (int k = 0; k < 3; k++) cam.position[k] = uni(-sc, sc);
+        // a box in front of the camera, across the edges of the view or
+        // around the camera
+        const float fwd_v[3] = {0.0f, 1.0f, 0.0f};
+        float fwd[3];
+        orc_rotate_vector(cam.rotation, fwd_v, fwd);
+        const float dist = (it % 9 == 0) ? 0.0f : uni(0.5f, 20.0f) * sc / 4;
+        float lo[3], hi[3];
+        for (int k = 0; k < 3; k++) {
+            const float c = cam.position[k] + dist * fwd[k] +
+                            uni(-1.0f, 1.0f) * dist * cam.fov;
+            const float h = uni(0.01f, 1.0f) * (dist + sc / 8);
+            lo[k] = c - h;
+            hi[k] = c + h;
+        }
+        const CullOrigin co = cull_origin(lo, hi, cam.position);
+        float Q = 0.0f;
+        for (int k = 0; k < 4; k++) Q = fmaxf(Q, fabsf(cam.rotation[k]));
+        struct orc_box box;
+        orc_dot o;
+        for (int k = 0; k < 3; k++) {
+            box.min[k] = lo[k]; box.max[k] = hi[k]; o[k] = cam.position[k];
+        }
+        box.min[3] = box.max[3] = o[3] = 0.0f;
+        const int nbx = W / 4, nby = H / 4;
+        const int nb = (nbx * nby > 600) ? 600 : nbx * nby;
+        for (int b = 0; b < nb; b++) {
+            const int bi = (nbx * nby > 600) ? (int)(u32() % (nbx * nby)) : b;
+            const int x0 = (bi % nbx) * 4, y0 = (bi / nbx) * 4;
+            orc_dot d[16];
+            for (int p = 0; p < 16; p++)
+                orc_screen2world(&cam, x0 + (p & 3), y0 + (p >> 2), d[p]);
+            float V = 1.0f;
+            V = fmaxf(V, fabsf(fmaf(cam.mul, (float)x0, -cam.xsub)));
+            V = fmaxf(V, fabsf(fmaf(cam.mul, (float)(x0 + 3), -cam.xsub)));
+            V = fmaxf(V, fabsf(fmaf(-cam.mul, (float)y0, cam.ysub)));
+            V = fmaxf(V, fabsf(fmaf(-cam.mul, (float)(y0 + 3), cam.ysub)));
+            float corner[4][3];
+            const int cp[4] = {0, 3, 12, 15};
+            for (int k = 0; k < 4; k++)
+                for (int i = 0; i < 3; i++) corner[k][i] = d[cp[k]][i];
+            const DirBox db = block_dirs(corner, Q, V);
+            for (int p = 0; p < 16; p++)
+                for (int i = 0; i < 3; i++)
+                    if (!db.unsure && !(db.lo[i] <= d[p][i] && d[p][i] <= db.hi[i]))
+                        outside++;
+            const bool may = block_may_enter(co, db);
+            bool any = false;
+            for (int p = 0; p < 16; p++) {
+                orc_dot res;
+                any = any || orc_hit_box(&box, o, d[p], res);
+            }
+            blocks++;
+            culled += !may;
+            hit_blocks += any;
+            if (any && !may) {
+                if (bad < 10) printf("CULLED A HIT it=%d block (%d,%d)\n", it, x0, y0);
+                bad++;
+            }
+        }
+    }
+    printf("blocks %ld culled %ld hit_blocks %ld outside %ld bad %ld\n", blocks, culled,
+           hit_blocks, outside, bad);
+    return bad != 0 || outside != 0;
+}
+"""
+
+
+def test_block_test_never_culls_an_exact_hit(oracle):
+    """The pre-pass tests a whole 4x4 block from its corner rays
+    (block_dirs, block_may_enter in cull.h): the box it derives must hold
+    every computed direction of the block (orc_screen2world, the same
+    arithmetic as the device), and a block must never be culled while one
+    of its 16 rays passes the exact box test (orc_hit_box).  Random
+    cameras (unit and scaled quaternions, fields of view from narrow to
+    wide, frames up to 3840x2160), boxes across the edges of the view and
+    around the camera, coordinates from 1 to 1e5."""
+    lib = oracle.LIB_PATH
+    with tempfile.TemporaryDirectory() as d:
+        src = os.path.join(d, "block.cpp")
+        exe = os.path.join(d, "block")
+        open(src, "w").write(BLOCK_CHECK)
+        r = subprocess.run(
+            ["g++", "-O2", "-ffp-contract=off", "-I",
+             os.path.join(ROOT, "voxvision_amd", "csrc"), "-I",
+             os.path.dirname(lib), src, lib, "-o", exe,
+             "-Wl,-rpath," + os.path.dirname(lib), "-lm"],
+            capture_output=True, text=True)
+        if r.returncode:
+            pytest.fail(r.stderr)
+        r = subprocess.run([exe], capture_output=True, text=True, timeout=300)
+        assert r.returncode == 0, r.stdout[-3000:]
+        v = [int(x) for x in r.stdout.split()[1::2][-5:]]
+        blocks, culled, hit_blocks, outside, bad = v
+        assert bad == 0 and outside == 0
+        # informative: both culled blocks and blocks with hits are common
+        assert culled > blocks // 10 and hit_blocks > blocks // 10

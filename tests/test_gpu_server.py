@@ -166,6 +166,33 @@ def test_shared_small_frame_launch_equals_alone(gpu):
     S.close()
 
 
+def test_ball_batches_beyond_the_pinned_staging(gpu):
+    """Five held 1 M-ball batches need 80 MB of coordinates, more than the
+    server's 64 MiB of pinned staging: the last ones are kept in pageable
+    memory instead.  Every batch equals voxg_ball_batch on the same
+    balls."""
+    from voxvision_amd import _lib, scenes, server, voxtrees as VT
+    L = gpu
+    c1, t1, t2, dots2 = _setup()
+    S = server.Server(streams=2)
+    i1 = S.add_tree(t1)
+    n = 1 << 20
+    reqs = []
+    _lib.check(L.voxg_test_server_hold(S._h, 1))
+    for k in range(5):
+        c, r = scenes.random_balls(n, *c1["bbox"], seed=400 + k)
+        tk, out = S.submit_balls(i1, c, r)
+        reqs.append((tk, out, c, r))
+    _lib.check(L.voxg_test_server_hold(S._h, 0))
+    for tk, out, c, r in reqs:
+        assert S.wait(tk, 60.0)["status"] == 0
+        want = VT.tree_ball_collidep(t1, c, r)
+        torch.cuda.synchronize()
+        assert torch.equal(out, want.cpu())
+        assert 0 < int(out.sum()) < n
+    S.close()
+
+
 def test_admission_by_memory_and_slots(gpu):
     from voxvision_amd import server
     c1, t1, _, _ = _setup()

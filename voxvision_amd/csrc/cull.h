@@ -77,3 +77,77 @@ __host__ __device__ inline bool may_enter(const float lo[3], const float hi[3],
                                           Rcp rcp) {
     return may_enter_dir(cull_origin(lo, hi, o), d, rcp);
 }
+
+// ---- whole 4x4 blocks of camera rays ----
+//
+// In exact arithmetic screen2world is affine in the pixel coordinates (it
+// rotates (mul x - xsub, 1, ysub - mul y) by a fixed quaternion), so the
+// exact directions of a 4x4 block lie, component by component, between
+// those of its four corners.  A computed direction differs from the exact
+// one by less than E = 2^-16 (Q^2 + 1) V, Q the largest quaternion
+// component and V the largest magnitude among the unrotated vectors'
+// components: counted term by term its roundings add up to at most
+// 2^-24 (68 Q^2 + 3) V.  So the box [min - 3E, max + 3E] of the corner
+// directions holds every computed direction of the block (the extra E
+// covers the rounding of the bounds).
+struct DirBox {
+    float lo[3], hi[3];
+    bool unsure;  // a non-finite or huge input: every ray may enter
+};
+
+__host__ __device__ inline DirBox block_dirs(const float d[4][3], float Q,
+                                             float V) {
+    DirBox b;
+    const float E = 0x1p-16f * (Q * Q + 1.0f) * V;
+    bool finite = E <= 0x1p100f;
+    for (int i = 0; i < 3; ++i) {
+        float mn = d[0][i], mx = d[0][i];
+        for (int k = 0; k < 4; ++k) {
+            finite = finite && fabsf(d[k][i]) <= 0x1p100f;
+            mn = fminf(mn, d[k][i]);
+            mx = fmaxf(mx, d[k][i]);
+        }
+        b.lo[i] = mn - 3.0f * E;
+        b.hi[i] = mx + 3.0f * E;
+    }
+    b.unsure = !finite;
+    return b;
+}
+
+// A quotient moved outward past its rounding (and past gradual
+// underflow); infinities stay as they are.
+__host__ __device__ inline float cull_up(float q) {
+    return (q > 0.0f ? q * (1.0f + 0x1p-20f) : q * (1.0f - 0x1p-20f)) +
+           0x1p-126f;
+}
+__host__ __device__ inline float cull_down(float q) {
+    return (q > 0.0f ? q * (1.0f - 0x1p-20f) : q * (1.0f + 0x1p-20f)) -
+           0x1p-126f;
+}
+
+// False only when no ray o + t d, t >= 0, with d anywhere in the box b
+// comes within the margin of may_enter of the scene box: then, by the
+// argument above, hit_box rejects every ray of the block.  For each axis
+// the set of t at which some d_i in [lo_i, hi_i] puts o_i + t d_i between
+// the enlarged slab's ends is cut out by t lo_i <= b_i and t hi_i >= a_i
+// (a, b as in CullOrigin); the block may enter when those half-lines and
+// t >= 0 meet.  The quotients are exact divisions moved outward.  A block
+// some of whose directions could have every component below 2^-30 (may
+// _enter gives true for those) also gives true.
+__host__ __device__ inline bool block_may_enter(const CullOrigin &c,
+                                                const DirBox &db) {
+    if (c.unsure || db.unsure) return true;
+    float t0 = 0.0f, t1 = INFINITY, big = 0.0f;
+    bool ok = true;
+    for (int i = 0; i < 3; ++i) {
+        const float l = db.lo[i], u = db.hi[i], a = c.a[i], b = c.b[i];
+        big = fmaxf(big, l > 0.0f ? l : (u < 0.0f ? -u : 0.0f));
+        if (l > 0.0f) t1 = fminf(t1, cull_up(b / l));
+        else if (l < 0.0f) t0 = fmaxf(t0, cull_down(b / l));
+        else ok = ok && b >= 0.0f;
+        if (u > 0.0f) t0 = fmaxf(t0, cull_down(a / u));
+        else if (u < 0.0f) t1 = fminf(t1, cull_up(a / u));
+        else ok = ok && a <= 0.0f;
+    }
+    return big < 0x1p-29f || (ok && t0 <= t1);
+}
