@@ -570,6 +570,30 @@ def test_incoherent_rays_and_balls_at_full_size(gpu, oracle):
         assert np.array_equal(part.cpu().numpy(), got[a:b])
 
 
+@pytest.mark.parametrize("quality", [0, 2])
+def test_render_into_unaligned_voxel_plane(gpu, quality):
+    """The pre-pass writes the miss ids of the blocks it finishes in 16-byte
+    stores from the whole workgroup when the voxel plane is 16-byte
+    aligned, and one by one from the testing thread when it is not: a
+    plane offset by 4 bytes gets the same ids, and the words around it
+    stay untouched."""
+    from voxvision_amd import voxrnd, voxtrees as VT
+    c1 = _c1()
+    G = VT.make_tree(c1["dots"])
+    W, H, cam = c1["W"], c1["H"], c1["camera"]
+    n = W * H
+    ref = voxrnd.render(G, [cam], W, H, quality=quality, want_colors=False)
+    buf = torch.full((n + 2,), 7, dtype=torch.int32, device="cuda")
+    hit = torch.empty((3, n), dtype=torch.float32, device="cuda")
+    out = voxrnd.render(G, [cam], W, H, quality=quality, want_colors=False,
+                        out={"voxel": buf[1:n + 1], "hit": hit})
+    torch.cuda.synchronize()
+    assert buf[1:n + 1].data_ptr() % 16 == 4
+    assert torch.equal(out["voxel"], ref["voxel"])
+    assert int(buf[0]) == 7 and int(buf[n + 1]) == 7
+    assert (ref["voxel"] == -1).float().mean() > 0.05
+
+
 @pytest.mark.parametrize("quality", [0, 1, 2, 2 | 4, 2 | 8])
 def test_render_three_pass_edge_cases(gpu, oracle, quality):
     """FAST, ADAPTIVE and ADAPTIVE with either kind of ray merging take

@@ -1331,7 +1331,9 @@ block_cull_kernel(RenderItem it, uint32_t npix, Ctrl *ctrl,
         const float o3[3] = {c.px, c.py, c.pz};
         co0 = cull_origin(lo3, hi3, o3);
     }
-    // the miss ids of a finished block: 64 contiguous bytes
+    // the miss ids of the finished blocks (64 contiguous bytes each) are
+    // written after the tests, by the whole workgroup; a voxel plane that
+    // is not 16-byte aligned takes them one by one from the testing thread
     uint32_t *const vox = it.sink.voxel;
     const bool vox16 = ((uintptr_t)vox & 15u) == 0;
     // one thread per block, 4 blocks per thread: the four corner rays bound
@@ -1379,22 +1381,23 @@ block_cull_kernel(RenderItem it, uint32_t npix, Ctrl *ctrl,
                 cls = all ? 2 : 1;
             }
         }
-        if (valid && cls == 0 && vox) {
-            // every pixel of the block is a miss: its 16 voxel ids
+        if (valid && cls == 0 && vox && !vox16) {
             uint32_t *v = vox + (size_t)blk * 16u;
-            if (vox16) {
-                const uint4 m = make_uint4(kEnd, kEnd, kEnd, kEnd);
-                uint4 *v4 = reinterpret_cast<uint4 *>(v);
-                v4[0] = m;
-                v4[1] = m;
-                v4[2] = m;
-                v4[3] = m;
-            } else {
-                for (int k = 0; k < 16; ++k) v[k] = kEnd;
-            }
+            for (int k = 0; k < 16; ++k) v[k] = kEnd;
         }
         enter[lb] = valid ? cls : 0;
         if (COUNT && valid && cls == 0) my_missed++;
+    }
+    __syncthreads();
+    if (vox && vox16) {
+        // the finished blocks' miss ids, consecutive threads on
+        // consecutive 16 bytes (a wave stores 1 KiB at a time)
+        const uint4 m = make_uint4(kEnd, kEnd, kEnd, kEnd);
+        uint4 *v4 = reinterpret_cast<uint4 *>(vox + (size_t)tile0);
+        for (uint32_t e = tid; e < kCullBlocks * 4; e += 256) {
+            const uint32_t lb = e >> 2;
+            if (((tile0 >> 4) + lb) * 16u < npix && enter[lb] == 0) v4[e] = m;
+        }
     }
     __syncthreads();
     // ordered compaction of the 1024 block flags (thread t owns blocks
