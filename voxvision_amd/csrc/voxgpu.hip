@@ -803,6 +803,14 @@ struct SpecOut {
     uint32_t *vid, *node;  // voxel id or kEnd; leaf node of a hit
     float *hx, *hy, *hz;   // hit point
     uint32_t stride;
+    // direct renders only: hit points of corners 0 and 15 (index
+    // corner * stride + block); the replay's changed pixels, appended per
+    // wave of the replay into the region of its 64 queue positions
+    // (1024 entries from 16 x the region's first position; vid and hx..hz
+    // hold the results, ekey the position within the region | pixel << 6)
+    // and the number of entries of each region
+    float *cx, *cy, *cz;
+    uint32_t *ekey, *ecount;
     __device__ __forceinline__ void put(uint32_t blk, int px, bool hit,
                                         uint32_t v, uint32_t nd, f3 p) const {
         const uint32_t r = (uint32_t)px * stride + blk;
@@ -822,9 +830,10 @@ struct SpecOut {
         const uint32_t r = (uint32_t)px * stride + blk;
         node[r] = hit ? nd : kEnd;
         if (hit && (px == 0 || px == 15)) {
-            hx[r] = p.x;
-            hy[r] = p.y;
-            hz[r] = p.z;
+            const uint32_t c = (px == 15 ? stride : 0u) + blk;
+            cx[c] = p.x;
+            cy[c] = p.y;
+            cz[c] = p.z;
         }
     }
 };
@@ -1481,14 +1490,15 @@ block_cull_kernel(RenderItem it, uint32_t npix, Ctrl *ctrl,
 // DIRECT (FAST and ADAPTIVE without merging): the first pass wrote its root
 // searches to the outputs and only the leaf nodes (and the corners' hit
 // points) to SpecOut.  Without copies no pixel can turn from a hit into a
-// miss: the replay stores the leaf tries that hit, the only results that
-// differ from the root searches, in SpecOut, and the mask of each block's
-// stored pixels in `changed`, by queue position, for emit_changed_kernel.
+// miss: the replay appends the leaf tries that hit, the only results that
+// differ from the root searches, to its wave's region of SpecOut (a
+// ballot and a prefix per step, so the stores of a step are contiguous)
+// for emit_changed_kernel, which then reads only those.
 template <bool DIRECT>
 __global__ void __launch_bounds__(256)
 resolve_kernel(RenderItem it, SpecOut spec, const uint32_t *__restrict__ queue,
                uint32_t qlen, Ctrl *ctrl, uint32_t *__restrict__ redo,
-               int redo_pass, uint32_t *__restrict__ changed) {
+               int redo_pass) {
     uint32_t qf, qt;
     if (redo_pass) {
         qf = qt = min((uint32_t)ctrl->rpair, qlen);
@@ -1517,7 +1527,10 @@ resolve_kernel(RenderItem it, SpecOut spec, const uint32_t *__restrict__ queue,
             if constexpr (DIRECT) {
                 nd = spec.node[r];
                 vid = kEnd;  // used by merge copies only
-                if (corner) pt = f3{spec.hx[r], spec.hy[r], spec.hz[r]};
+                if (corner) {
+                    const uint32_t k = (p == 15 ? spec.stride : 0u) + id;
+                    pt = f3{spec.cx[k], spec.cy[k], spec.cz[k]};
+                }
                 return nd != kEnd;
             }
             vid = spec.vid[r];
@@ -1525,14 +1538,12 @@ resolve_kernel(RenderItem it, SpecOut spec, const uint32_t *__restrict__ queue,
             pt = f3{spec.hx[r], spec.hy[r], spec.hz[r]};
             return vid != kEnd;
         };
-        uint32_t mask = 0;  // DIRECT: the pixels stored
         auto store = [&](int p, uint32_t vid, f3 pt) {
             const uint32_t r = (uint32_t)p * spec.stride + id;
             spec.vid[r] = vid;
             spec.hx[r] = pt.x;
             spec.hy[r] = pt.y;
             spec.hz[r] = pt.z;
-            mask |= 1u << p;
         };
         int mode = it.rnd_mode, merge = 0;
         uint32_t leaf = kEnd, vid_last = kEnd;
@@ -1597,7 +1608,58 @@ resolve_kernel(RenderItem it, SpecOut spec, const uint32_t *__restrict__ queue,
         // the record of the leaf last tried (a FAST chain often keeps it)
         uint32_t loaded = kEnd;
         Node ln;
-        if (DIRECT && mode != VOXG_QUALITY_FAST) iend = i;
+        if constexpr (DIRECT) {
+            // every lane of the wave takes 16 steps (its own pixels are
+            // steps i .. iend - 1; a BEST block of ADAPTIVE has none), so
+            // that each step's appends see one count on all lanes
+            const int i0 = i, i1 = mode == VOXG_QUALITY_FAST ? iend : i;
+            const uint32_t region = b & ~63u;
+            uint32_t run = 0;
+            for (int k = 0; k < 16; ++k) {
+                const int ii = i0 + k;
+                const int p = render_order(ii & 15);
+                bool st = false;
+                uint32_t svid = kEnd;
+                f3 spt = o;
+                if (ii < i1) {
+                    uint32_t rvid, rnd;
+                    f3 rpt;
+                    const bool rhit = root(p, rvid, rnd, rpt, false);
+                    bool kept = false;
+                    if (leaf != kEnd) {
+                        // renderer.c:294-300: the previous pixel's leaf first
+                        const f3 d = screen2world(c, xs + (p & 3), ys + (p >> 2));
+                        if (leaf != loaded) {
+                            ln = load_node(T.nodes, leaf);
+                            loaded = leaf;
+                        }
+                        f3 e;
+                        if (hit_box(ln.bmin, ln.bmax, o, d, e) &&
+                            leaf_search<false>(T, ln, e, d, svid, spt, nt)) {
+                            st = true;
+                            kept = true;  // the leaf stays the same
+                        }
+                    }
+                    if (!kept) leaf = rhit ? rnd : kEnd;
+                }
+                // the leaf tries that hit, appended to the wave's region
+                const unsigned long long m = __ballot(st);
+                if (st) {
+                    const uint32_t e = region * 16u + run + lanes_below(m);
+                    spec.ekey[e] = (b - region) | ((uint32_t)p << 6);
+                    spec.vid[e] = svid;
+                    spec.hx[e] = spt.x;
+                    spec.hy[e] = spt.y;
+                    spec.hz[e] = spt.z;
+                }
+                run += (uint32_t)__popcll(m);
+                if (__ballot(ii + 1 < i1) == 0) break;
+            }
+            const unsigned long long act = __ballot(true);
+            if (lane_id() == (uint32_t)(__ffsll((long long)act) - 1))
+                spec.ecount[region >> 6] = run;
+            continue;
+        }
         for (; i < iend; ++i) {
             const int p = render_order(i);
             if (merge && (i & 1)) {
@@ -1635,7 +1697,6 @@ resolve_kernel(RenderItem it, SpecOut spec, const uint32_t *__restrict__ queue,
                 leaf = kEnd;
             }
         }
-        if constexpr (DIRECT) changed[b] = mask;
     }
 }
 
@@ -1704,52 +1765,64 @@ emit_kernel(RenderItem it, SpecOut spec, const uint32_t *__restrict__ queue,
     }
 }
 
-// Third pass of a direct render: the pixels that the replay changed, from
-// SpecOut over the first pass's results in the outputs, transposed through
-// LDS as in emit_kernel (queue order: neighbouring lanes read neighbouring
-// blocks).  Every changed pixel is a hit.  The last workgroup clears the
-// queue counts for the next render.
+// Third pass of a direct render: the pixels whose leaf try hit, from the
+// regions the replay's waves appended them to, over the first pass's
+// results in the outputs.  A workgroup takes one region (64 queued
+// blocks) at a time: it reads the region's entries, contiguous, places
+// them in LDS by block and pixel, then writes each block's run of the
+// outputs with 16 neighbouring lanes, as emit_kernel does.  (Reading the
+// changed blocks' scratch rows instead read about as much as a full output
+// pass, 0.25 ms on the benchmark frames; storing from the entries directly,
+// one pixel of 64 blocks per instruction, took 0.5 ms.)  Every changed
+// pixel is a hit.  The last workgroup clears the queue counts for the next
+// render.
 __global__ void __launch_bounds__(256)
 emit_changed_kernel(RenderItem it, SpecOut spec,
-                    const uint32_t *__restrict__ queue,
-                    const uint32_t *__restrict__ changed, uint32_t qlen,
+                    const uint32_t *__restrict__ queue, uint32_t qlen,
                     Ctrl *ctrl) {
-    __shared__ uint32_t sid[kEmitBlocks], smask[kEmitBlocks];
-    __shared__ uint32_t svid[kEmitBlocks * 16];
-    __shared__ float spx[kEmitBlocks * 16], spy[kEmitBlocks * 16],
-        spz[kEmitBlocks * 16];
+    __shared__ uint32_t sid[64], smask[64];
+    __shared__ uint32_t svid[1024];
+    __shared__ float spx[1024], spy[1024], spz[1024];
     uint32_t qf, qt;
     queue_counts(ctrl->qpair, qlen, qf, qt);
+    const uint32_t nreg = (qt + 63u) / 64u;
     const uint32_t tid = threadIdx.x;
-    for (uint32_t b0 = blockIdx.x * kEmitBlocks; b0 < qt;
-         b0 += gridDim.x * kEmitBlocks) {
-        const uint32_t nb = min(kEmitBlocks, qt - b0);
-        if (tid < nb) {
-            const uint32_t b = b0 + tid;
-            sid[tid] = queue[b < qf ? b : qlen - 1u - (b - qf)];
-            smask[tid] = changed[b];
+    for (uint32_t r = blockIdx.x; r < nreg; r += gridDim.x) {
+        const uint32_t nb = min(64u, qt - 64u * r);
+        if (tid < 64u) {
+            smask[tid] = 0u;
+            if (tid < nb) {
+                const uint32_t bq = 64u * r + tid;
+                sid[tid] = queue[bq < qf ? bq : qlen - 1u - (bq - qf)];
+            }
         }
+        // at most 15 entries per block of the region
+        const uint32_t cnt = min(spec.ecount[r], 15u * nb);
+        const uint32_t base = r * 1024u;
         __syncthreads();
-        const uint32_t k = tid & (kEmitBlocks - 1u);
-        for (uint32_t p = tid / kEmitBlocks; p < 16u; p += 256u / kEmitBlocks) {
-            if (k < nb && ((smask[k] >> p) & 1u)) {
-                const uint32_t r = p * spec.stride + sid[k];
-                svid[k * 16u + p] = spec.vid[r];
-                spx[k * 16u + p] = spec.hx[r];
-                spy[k * 16u + p] = spec.hy[r];
-                spz[k * 16u + p] = spec.hz[r];
+        for (uint32_t e = tid; e < cnt; e += 256u) {
+            const uint32_t key = spec.ekey[base + e];
+            const uint32_t lb = key & 63u, p = (key >> 6) & 15u;
+            if (lb < nb) {
+                const uint32_t j = lb * 16u + p;
+                svid[j] = spec.vid[base + e];
+                spx[j] = spec.hx[base + e];
+                spy[j] = spec.hy[base + e];
+                spz[j] = spec.hz[base + e];
+                atomicOr(&smask[lb], 1u << p);
             }
         }
         __syncthreads();
         for (uint32_t j = tid; j < nb * 16u; j += 256u) {
-            if ((smask[j >> 4] >> (j & 15u)) & 1u)
-                it.sink.put(sid[j >> 4] * 16u + (j & 15u), true, svid[j], 0,
+            const uint32_t lb = j >> 4;
+            if ((smask[lb] >> (j & 15u)) & 1u)
+                it.sink.put(sid[lb] * 16u + (j & 15u), true, svid[j], 0,
                             f3{spx[j], spy[j], spz[j]});
         }
         __syncthreads();
     }
     __syncthreads();
-    if (threadIdx.x == 0) {
+    if (tid == 0) {
         __threadfence();
         if (atomicAdd(&ctrl->rdone, 1u) == gridDim.x - 1u) {
             atomicExch(&ctrl->qpair, 0ull);
@@ -2924,11 +2997,16 @@ static int render_speculative(const voxg_tree *t, const dev::RenderItem &one,
     cfg.render_cull = 1;  // the resolve pass reads the pre-pass's queue
     const size_t nblk = npix / 16;
     void *sp = nullptr;
-    // root-search results, then the list of blocks whose merging stopped
-    // or (direct) the changed pixels' mask of each queued block
-    int rc = voxg_internal::scratch(stream, npix * 20 + nblk * 4, &sp);
+    // root-search results (voxel id, leaf node, hit point: 20 bytes per
+    // pixel), then the list of blocks whose merging stopped or (direct) the
+    // appended entries' keys, the regions' counts and the corners' points
+    const size_t nreg = (nblk + 63) / 64;
+    const size_t bytes = npix * 20 + (direct ? npix * 4 + nreg * 4 + nblk * 24
+                                             : nblk * 4);
+    int rc = voxg_internal::scratch(stream, bytes, &sp);
     if (rc) return rc;
     dev::SpecOut spec;
+    std::memset(&spec, 0, sizeof(spec));
     spec.vid = (uint32_t *)sp;
     spec.node = spec.vid + npix;
     spec.hx = (float *)(spec.node + npix);
@@ -2936,7 +3014,13 @@ static int render_speculative(const voxg_tree *t, const dev::RenderItem &one,
     spec.hz = spec.hy + npix;
     spec.stride = (uint32_t)nblk;
     uint32_t *redo = skip_odd ? (uint32_t *)(spec.hz + npix) : nullptr;
-    uint32_t *changed = direct ? (uint32_t *)(spec.hz + npix) : nullptr;
+    if (direct) {
+        spec.ekey = (uint32_t *)(spec.hz + npix);
+        spec.ecount = spec.ekey + npix;
+        spec.cx = (float *)(spec.ecount + nreg);
+        spec.cy = spec.cx + 2 * nblk;
+        spec.cz = spec.cy + 2 * nblk;
+    }
     // one pixel per unit, into the scratch; the queue counts stay for the
     // passes below
     auto first_pass = [&](auto &work) {
@@ -2979,19 +3063,19 @@ static int render_speculative(const voxg_tree *t, const dev::RenderItem &one,
     if (direct) {
         hipLaunchKernelGGL(dev::resolve_kernel<true>, dim3((unsigned)grid),
                            dim3(256), 0, stream, one, spec, ws->queue,
-                           (uint32_t)nblk, ws->ctrl, nullptr, 0, changed);
+                           (uint32_t)nblk, ws->ctrl, nullptr, 0);
         HIPCHK(hipGetLastError());
         if (injected_failure()) return fail(VOXG_EIO, "injected failure");
         hipLaunchKernelGGL(dev::emit_changed_kernel, dim3((unsigned)grid),
                            dim3(256), 0, stream, one, spec, ws->queue,
-                           changed, (uint32_t)nblk, ws->ctrl);
+                           (uint32_t)nblk, ws->ctrl);
         HIPCHK(hipGetLastError());
         ws->dirty = false;
         return 0;
     }
     hipLaunchKernelGGL(dev::resolve_kernel<false>, dim3((unsigned)grid),
                        dim3(256), 0, stream, one, spec, ws->queue,
-                       (uint32_t)nblk, ws->ctrl, redo, 0, nullptr);
+                       (uint32_t)nblk, ws->ctrl, redo, 0);
     HIPCHK(hipGetLastError());
     if (skip_odd) {
         // the odd pixels of the listed blocks, then their replay (the
@@ -3010,7 +3094,7 @@ static int render_speculative(const voxg_tree *t, const dev::RenderItem &one,
         if (rc) return rc;
         hipLaunchKernelGGL(dev::resolve_kernel<false>, dim3((unsigned)grid),
                            dim3(256), 0, stream, one, spec, ws->queue,
-                           (uint32_t)nblk, ws->ctrl, redo, 1, nullptr);
+                           (uint32_t)nblk, ws->ctrl, redo, 1);
         HIPCHK(hipGetLastError());
     }
     if (injected_failure()) return fail(VOXG_EIO, "injected failure");
