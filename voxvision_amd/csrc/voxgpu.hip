@@ -1612,19 +1612,30 @@ resolve_kernel(RenderItem it, SpecOut spec, const uint32_t *__restrict__ queue,
             // every lane of the wave takes 16 steps (its own pixels are
             // steps i .. iend - 1; a BEST block of ADAPTIVE has none), so
             // that each step's appends see one count on all lanes
-            const int i0 = i, i1 = mode == VOXG_QUALITY_FAST ? iend : i;
+            const int i0 = i;
+            int i1 = mode == VOXG_QUALITY_FAST ? iend : i;
             const uint32_t region = b & ~63u;
             uint32_t run = 0;
+            // which pixels' root searches hit: all 16 loads in flight at
+            // once, rather than one per step; a block with none has nothing
+            // to replay (its leaf stays empty), and neither has a wave
+            uint32_t hits = 0;
+            if (i1 > i0)
+                for (int q = 0; q < 16; ++q)
+                    hits |= (spec.node[(uint32_t)q * spec.stride + id] != kEnd)
+                                ? 1u << q : 0u;
+            if (hits == 0) i1 = i0;
             for (int k = 0; k < 16; ++k) {
+                if (__ballot(i0 + k < i1) == 0) break;
                 const int ii = i0 + k;
                 const int p = render_order(ii & 15);
                 bool st = false;
                 uint32_t svid = kEnd;
                 f3 spt = o;
                 if (ii < i1) {
-                    uint32_t rvid, rnd;
-                    f3 rpt;
-                    const bool rhit = root(p, rvid, rnd, rpt, false);
+                    const bool rhit = (hits >> p) & 1u;
+                    const uint32_t rnd =
+                        rhit ? spec.node[(uint32_t)p * spec.stride + id] : kEnd;
                     bool kept = false;
                     if (leaf != kEnd) {
                         // renderer.c:294-300: the previous pixel's leaf first
@@ -1653,7 +1664,6 @@ resolve_kernel(RenderItem it, SpecOut spec, const uint32_t *__restrict__ queue,
                     spec.hz[e] = spt.z;
                 }
                 run += (uint32_t)__popcll(m);
-                if (__ballot(ii + 1 < i1) == 0) break;
             }
             const unsigned long long act = __ballot(true);
             if (lane_id() == (uint32_t)(__ffsll((long long)act) - 1))
