@@ -1778,55 +1778,90 @@ emit_kernel(RenderItem it, SpecOut spec, const uint32_t *__restrict__ queue,
 // Third pass of a direct render: the pixels whose leaf try hit, from the
 // regions the replay's waves appended them to, over the first pass's
 // results in the outputs.  A workgroup takes one region (64 queued
-// blocks) at a time: it reads the region's entries, contiguous, places
-// them in LDS by block and pixel, then writes each block's run of the
-// outputs with 16 neighbouring lanes, as emit_kernel does.  (Reading the
-// changed blocks' scratch rows instead read about as much as a full output
-// pass, 0.25 ms on the benchmark frames; storing from the entries directly,
-// one pixel of 64 blocks per instruction, took 0.5 ms.)  Every changed
-// pixel is a hit.  The last workgroup clears the queue counts for the next
+// blocks) at a time: it places the region's entries in LDS by block and
+// pixel, then writes each block's run of the outputs with 16 neighbouring
+// lanes, as emit_kernel does.  The loads of a region (its block ids and up
+// to 4 entries per thread, all independent) go out while the previous
+// region is written, and its entry count one region earlier still, so a
+// region costs no chain of memory round trips.  (Reading the changed
+// blocks' scratch rows instead read about as much as a full output pass,
+// 0.25 ms on the benchmark frames; storing from the entries directly, one
+// pixel of 64 blocks per instruction, took 0.5 ms.)  Every changed pixel
+// is a hit.  The last workgroup clears the queue counts for the next
 // render.
 __global__ void __launch_bounds__(256)
 emit_changed_kernel(RenderItem it, SpecOut spec,
                     const uint32_t *__restrict__ queue, uint32_t qlen,
                     Ctrl *ctrl) {
-    __shared__ uint32_t sid[64], smask[64];
+    __shared__ uint32_t sid[64];
     __shared__ uint32_t svid[1024];
     __shared__ float spx[1024], spy[1024], spz[1024];
     uint32_t qf, qt;
     queue_counts(ctrl->qpair, qlen, qf, qt);
     const uint32_t nreg = (qt + 63u) / 64u;
-    const uint32_t tid = threadIdx.x;
-    for (uint32_t r = blockIdx.x; r < nreg; r += gridDim.x) {
-        const uint32_t nb = min(64u, qt - 64u * r);
-        if (tid < 64u) {
-            smask[tid] = 0u;
-            if (tid < nb) {
-                const uint32_t bq = 64u * r + tid;
-                sid[tid] = queue[bq < qf ? bq : qlen - 1u - (bq - qf)];
-            }
+    const uint32_t tid = threadIdx.x, G = gridDim.x;
+    auto nblocks = [&](uint32_t r) { return min(64u, qt - 64u * r); };
+    // a region's entry count as written (loaded unconditionally, from a
+    // clamped index, so that no copy of it waits for the load), then
+    // bounded: at most 15 entries per block of the region
+    auto raw_count = [&](uint32_t r) { return spec.ecount[min(r, nreg - 1u)]; };
+    auto count = [&](uint32_t r, uint32_t raw) -> uint32_t {
+        return r < nreg ? min(raw, 15u * nblocks(r)) : 0u;
+    };
+    uint32_t bid = 0, key[4], vid[4];
+    float x[4], y[4], z[4];
+    auto fetch = [&](uint32_t r, uint32_t cnt) {
+        if (r >= nreg) return;
+        if (tid < nblocks(r)) {
+            const uint32_t bq = 64u * r + tid;
+            bid = queue[bq < qf ? bq : qlen - 1u - (bq - qf)];
         }
-        // at most 15 entries per block of the region
-        const uint32_t cnt = min(spec.ecount[r], 15u * nb);
         const uint32_t base = r * 1024u;
+#pragma unroll
+        for (int k = 0; k < 4; ++k) {
+            const uint32_t e = base + tid + 256u * k;
+            if (tid + 256u * k < cnt) {
+                key[k] = spec.ekey[e];
+                vid[k] = spec.vid[e];
+                x[k] = spec.hx[e];
+                y[k] = spec.hy[e];
+                z[k] = spec.hz[e];
+            }
+        }
+    };
+    uint32_t r = blockIdx.x;
+    if (r >= nreg) r = nreg;  // (nreg may be 0: no loads at all)
+    uint32_t cnt = 0, raw_next = 0;
+    if (r < nreg) {
+        cnt = count(r, raw_count(r));
+        raw_next = raw_count(r + G);
+        fetch(r, cnt);
+    }
+    for (; r < nreg; r += G) {
+        const uint32_t nb = nblocks(r);
+        for (uint32_t j = tid; j < 1024u; j += 256u) svid[j] = kEnd;
+        if (tid < 64u) sid[tid] = bid;
         __syncthreads();
-        for (uint32_t e = tid; e < cnt; e += 256u) {
-            const uint32_t key = spec.ekey[base + e];
-            const uint32_t lb = key & 63u, p = (key >> 6) & 15u;
-            if (lb < nb) {
+#pragma unroll
+        for (int k = 0; k < 4; ++k) {
+            const uint32_t lb = key[k] & 63u, p = (key[k] >> 6) & 15u;
+            if (tid + 256u * k < cnt && lb < nb) {
                 const uint32_t j = lb * 16u + p;
-                svid[j] = spec.vid[base + e];
-                spx[j] = spec.hx[base + e];
-                spy[j] = spec.hy[base + e];
-                spz[j] = spec.hz[base + e];
-                atomicOr(&smask[lb], 1u << p);
+                svid[j] = vid[k];
+                spx[j] = x[k];
+                spy[j] = y[k];
+                spz[j] = z[k];
             }
         }
         __syncthreads();
+        // the next region's loads, in flight during this one's stores
+        cnt = count(r + G, raw_next);
+        raw_next = raw_count(r + 2u * G);
+        fetch(r + G, cnt);
         for (uint32_t j = tid; j < nb * 16u; j += 256u) {
-            const uint32_t lb = j >> 4;
-            if ((smask[lb] >> (j & 15u)) & 1u)
-                it.sink.put(sid[lb] * 16u + (j & 15u), true, svid[j], 0,
+            const uint32_t v = svid[j];
+            if (v != kEnd)
+                it.sink.put(sid[j >> 4] * 16u + (j & 15u), true, v, 0,
                             f3{spx[j], spy[j], spz[j]});
         }
         __syncthreads();
