@@ -280,3 +280,277 @@ def test_block_test_never_culls_an_exact_hit(oracle):
         assert bad == 0 and outside == 0
         # informative: both culled blocks and blocks with hits are common
         assert culled > blocks // 10 and hit_blocks > blocks // 10
+
+
+DEEP_CHECK = r"""
+#define __host__
+#define __device__
+#include "cull.h"
+#include <math.h>
+#include <stdio.h>
+#include <stdlib.h>
+#include <stdint.h>
+#include <vector>
+#include <algorithm>
+extern "C" {
+#include "vox_oracle.h"
+}
+
+static uint64_t rng = 0x1f83d9abfb41bd6bull;
+static uint32_t u32(void) {
+    rng ^= rng << 13; rng ^= rng >> 7; rng ^= rng << 17;
+    return (uint32_t)(rng >> 16);
+}
+static float uni(float a, float b) { return a + (b - a) * (u32() / 4294967296.0f); }
+
+static int depth_of(const struct orc_node *n) {
+    if (!n || (n->flags & ORC_LEAF_MASK)) return 1;
+    int d = 0;
+    for (int k = 0; k < 8; k++)
+        if (n->data.inner.children[k]) d = std::max(d, depth_of(n->data.inner.children[k]));
+    return d + 1;
+}
+
+// the deepest level with at most 64 nodes, leaves above it included
+static std::vector<const struct orc_node *> cut_of(const struct orc_node *root) {
+    std::vector<const struct orc_node *> cur{root}, next;
+    for (;;) {
+        next.clear();
+        bool inner = false;
+        for (auto *n : cur) {
+            if (n->flags & ORC_LEAF_MASK) { next.push_back(n); continue; }
+            inner = true;
+            for (int k = 0; k < 8; k++)
+                if (n->data.inner.children[k]) next.push_back(n->data.inner.children[k]);
+        }
+        if (!inner || next.size() > 64) break;
+        cur.swap(next);
+    }
+    return cur;
+}
+
+// a unit quaternion turning (0, 1, 0), the camera's forward axis, to t
+static void aim(const float t[3], float q[4]) {
+    float n = sqrtf(t[0] * t[0] + t[1] * t[1] + t[2] * t[2]);
+    float f[3] = {t[0] / n, t[1] / n, t[2] / n};
+    // cross((0,1,0), f) = (f.z, 0, -f.x), dot = f.y
+    q[0] = 1.0f + f[1]; q[1] = f[2]; q[2] = 0.0f; q[3] = -f[0];
+    float m = sqrtf(q[0] * q[0] + q[1] * q[1] + q[2] * q[2] + q[3] * q[3]);
+    for (int k = 0; k < 4; k++) q[k] /= m;
+}
+
+static float nudge(float x, int k) {
+    for (; k > 0; --k) x = nextafterf(x, INFINITY);
+    for (; k < 0; ++k) x = nextafterf(x, -INFINITY);
+    return x;
+}
+
+struct R0 { float operator()(float x) const { return 1.0f / x; } };
+struct Rup { float operator()(float x) const { return nextafterf(1.0f / x, INFINITY); } };
+struct Rdn { float operator()(float x) const { return nextafterf(1.0f / x, -INFINITY); } };
+
+// one box and one direction (a direction box of a single point): an exact
+// hit of orc_hit_box must pass block_may_enter_rcp
+static long single(void) {
+    long bad = 0;
+    orc_set_voxel(1.0f, 1.0f, 1.0f);
+    const float scales[] = {1.0f, 64.0f, 4096.0f, 1e6f};
+    for (int it = 0; it < 300000; it++) {
+        const float sc = scales[it % 4];
+        float lo[3], hi[3], o[3], d[3], p[3];
+        for (int i = 0; i < 3; i++) {
+            lo[i] = (it & 8) ? floorf(uni(-sc, sc)) : uni(-sc, sc);
+            hi[i] = lo[i] + ((it & 8) ? floorf(uni(1, sc)) + 1 : uni(1e-3f, sc));
+        }
+        const int kind = it % 6;
+        for (int i = 0; i < 3; i++) {
+            float c = 0.5f * (lo[i] + hi[i]), h = hi[i] - lo[i];
+            o[i] = (kind == 5) ? uni(lo[i], hi[i]) : c + uni(-4, 4) * h;
+        }
+        for (int i = 0; i < 3; i++) {  // a corner, an edge, a face or near
+            int w = u32() % 3;
+            p[i] = (kind <= 2 && w == 0) ? lo[i] : (kind <= 2 && w == 1) ? hi[i]
+                                                 : uni(lo[i] - 0.1f * (hi[i] - lo[i]),
+                                                       hi[i] + 0.1f * (hi[i] - lo[i]));
+            p[i] = nudge(p[i], (int)(u32() % 9) - 4);
+        }
+        for (int i = 0; i < 3; i++) d[i] = p[i] - o[i];
+        if (kind == 3) d[u32() % 3] = 0.0f;
+        if (kind == 4) d[u32() % 3] = ((u32() & 1) ? 1 : -1) *
+                                      ldexpf(1.0f, -(int)(u32() % 140));
+        for (int s = 0; s < 2; s++) {
+            if (s) for (int i = 0; i < 3; i++) d[i] = -d[i];
+            struct orc_box box;
+            orc_dot oo, dd, res;
+            for (int i = 0; i < 3; i++) {
+                box.min[i] = lo[i]; box.max[i] = hi[i]; oo[i] = o[i]; dd[i] = d[i];
+            }
+            box.min[3] = box.max[3] = oo[3] = dd[3] = 0.0f;
+            DirBox db;
+            for (int i = 0; i < 3; i++) db.lo[i] = db.hi[i] = d[i];
+            db.unsure = false;
+            const CullOrigin c = deep_origin(lo, hi, o, cull_scale(lo, hi, o));
+            const bool may = block_may_enter_rcp(c, dir_rcp(db, R0())) &&
+                             block_may_enter_rcp(c, dir_rcp(db, Rup())) &&
+                             block_may_enter_rcp(c, dir_rcp(db, Rdn()));
+            if (orc_hit_box(&box, oo, dd, res) && !may) {
+                if (bad < 10)
+                    printf("MISSED lo=(%a %a %a) hi=(%a %a %a) o=(%a %a %a) d=(%a %a %a)\n",
+                           lo[0], lo[1], lo[2], hi[0], hi[1], hi[2], o[0], o[1], o[2],
+                           d[0], d[1], d[2]);
+                bad++;
+            }
+        }
+    }
+    return bad;
+}
+
+int main(void) {
+    long blocks = 0, root_kept = 0, deep_culled = 0, hit_blocks = 0, bad = 0;
+    const long single_bad = single();
+    const float offsets[] = {0.0f, 1000.0f, -60000.0f};
+    for (int sc = 0; sc < 18; sc++) {
+        const float V = (sc % 3 == 2) ? 0.5f : 1.0f;
+        orc_set_voxel(V, V, V);
+        const float off = offsets[sc % 3];
+        // blobs of voxels in a 64^3 grid (shells when sc is odd)
+        std::vector<float> pts;
+        const int nb = 2 + (int)(u32() % 5);
+        float bc[6][4];
+        for (int b = 0; b < nb; b++) {
+            for (int i = 0; i < 3; i++) bc[b][i] = uni(8, 56);
+            bc[b][3] = uni(3, 14);
+        }
+        for (int x = 0; x < 64; x++)
+            for (int y = 0; y < 64; y++)
+                for (int z = 0; z < 64; z++)
+                    for (int b = 0; b < nb; b++) {
+                        float dx = x - bc[b][0], dy = y - bc[b][1], dz = z - bc[b][2];
+                        float r = sqrtf(dx * dx + dy * dy + dz * dz);
+                        bool in = (sc & 1) ? fabsf(r - bc[b][3]) < 1.0f : r < bc[b][3];
+                        if (in) {
+                            pts.push_back(off + x * V); pts.push_back(off + y * V);
+                            pts.push_back(off + z * V); pts.push_back(0.0f);
+                            break;
+                        }
+                    }
+        const size_t n = pts.size() / 4;
+        orc_dot *set = (orc_dot *)aligned_alloc(16, n * sizeof(orc_dot));
+        for (size_t i = 0; i < n; i++)
+            for (int k = 0; k < 4; k++) set[i][k] = pts[4 * i + k];
+        struct orc_node *tree = orc_make_tree(set, n);
+        free(set);
+        if (!tree) continue;
+        const int D = depth_of(tree);
+        if (D > (int)kDeepCullMaxDepth) { printf("DEEP TREE %d\n", D); return 2; }
+        std::vector<const struct orc_node *> cut = cut_of(tree);
+        const float *lo = tree->bounding_box.min, *hi = tree->bounding_box.max;
+        const float ctr[3] = {off + 32 * V, off + 32 * V, off + 32 * V};
+        for (int ci = 0; ci < 24; ci++) {
+            orc_camera cam;
+            float dir[3], nn = 0.0f;
+            for (int k = 0; k < 3; k++) { dir[k] = uni(-1, 1); nn += dir[k] * dir[k]; }
+            nn = sqrtf(nn);
+            const float dist = uni(40, 160) * V;
+            float t[3];
+            for (int k = 0; k < 3; k++) {
+                cam.position[k] = ctr[k] + dist * dir[k] / nn;
+                t[k] = ctr[k] + uni(-20, 20) * V - cam.position[k];
+            }
+            if (ci % 6 == 5)  // inside the scene box
+                for (int k = 0; k < 3; k++) cam.position[k] = ctr[k] + uni(-30, 30) * V;
+            aim(t, cam.rotation);
+            cam.fov = uni(0.3f, 1.6f);
+            const int W = 16 * (4 + (int)(u32() % 12)), H = 4 * (8 + (int)(u32() % 24));
+            orc_camera_set_window_size(&cam, W, H);
+            float Q = 0.0f;
+            for (int k = 0; k < 4; k++) Q = fmaxf(Q, fabsf(cam.rotation[k]));
+            const CullOrigin co = cull_origin(lo, hi, cam.position);
+            const float S = cull_scale(lo, hi, cam.position);
+            orc_dot o;
+            for (int k = 0; k < 3; k++) o[k] = cam.position[k];
+            o[3] = 0.0f;
+            for (int by = 0; by < H / 4; by++)
+                for (int bx = 0; bx < W / 4; bx++) {
+                    const int x0 = bx * 4, y0 = by * 4;
+                    orc_dot d[16];
+                    for (int p = 0; p < 16; p++)
+                        orc_screen2world(&cam, x0 + (p & 3), y0 + (p >> 2), d[p]);
+                    float Vb = 1.0f;
+                    Vb = fmaxf(Vb, fabsf(fmaf(cam.mul, (float)x0, -cam.xsub)));
+                    Vb = fmaxf(Vb, fabsf(fmaf(cam.mul, (float)(x0 + 3), -cam.xsub)));
+                    Vb = fmaxf(Vb, fabsf(fmaf(-cam.mul, (float)y0, cam.ysub)));
+                    Vb = fmaxf(Vb, fabsf(fmaf(-cam.mul, (float)(y0 + 3), cam.ysub)));
+                    float corner[4][3];
+                    const int cp[4] = {0, 3, 12, 15};
+                    for (int k = 0; k < 4; k++)
+                        for (int i = 0; i < 3; i++) corner[k][i] = d[cp[k]][i];
+                    const DirBox db = block_dirs(corner, Q, Vb);
+                    blocks++;
+                    bool any = false;
+                    for (int p = 0; p < 16; p++) {
+                        orc_dot res;
+                        any = any || orc_ray_tree_intersection(tree, o, d[p], res) != NULL;
+                    }
+                    hit_blocks += any;
+                    if (!block_may_enter(co, db)) continue;
+                    root_kept++;
+                    if (co.unsure || db.unsure) continue;
+                    // reciprocals one ulp off, either way by block
+                    const DirRcp r = (x0 & 4) ? dir_rcp(db, Rup()) : dir_rcp(db, Rdn());
+                    bool keep = r.any;
+                    for (size_t b = 0; b < cut.size() && !keep; b++)
+                        keep = block_may_enter_rcp(
+                            deep_origin(cut[b]->bounding_box.min,
+                                        cut[b]->bounding_box.max, cam.position, S), r);
+                    if (keep) continue;
+                    deep_culled++;
+                    if (any) {
+                        if (bad < 10) printf("CULLED A HIT scene %d cam %d block (%d,%d)\n", sc, ci, x0, y0);
+                        bad++;
+                    }
+                }
+        }
+        orc_destroy_tree(tree);
+    }
+    printf("single_bad %ld blocks %ld root_kept %ld deep_culled %ld hit_blocks %ld bad %ld\n",
+           single_bad, blocks, root_kept, deep_culled, hit_blocks, bad);
+    return bad != 0 || single_bad != 0;
+}
+"""
+
+
+def test_cut_test_never_culls_a_hit(oracle):
+    """Beyond the root box, the pre-pass finishes a block when none of its
+    rays comes near any box of a cut of the tree (deep_origin, dir_rcp,
+    block_may_enter_rcp in cull.h).  Checked against the oracle's own
+    traversal (orc_ray_tree_intersection, search.c:8-153): no block culled
+    that way holds a ray that hits.  And box by box, with single
+    directions aimed at corners, edges and faces and nudged by a few ulps
+    (as for may_enter): no ray that the exact box test (orc_hit_box) lets in
+    is rejected, with reciprocals exact and one ulp off either way (the
+    device reciprocal is within one ulp).  Scenes of voxel blobs and shells,
+    voxel sizes 1 and 0.5, coordinates offset to 1e3 and -6e4, cameras
+    around the scene and inside its box."""
+    lib = oracle.LIB_PATH
+    with tempfile.TemporaryDirectory() as d:
+        src = os.path.join(d, "deep.cpp")
+        exe = os.path.join(d, "deep")
+        open(src, "w").write(DEEP_CHECK)
+        r = subprocess.run(
+            ["g++", "-O2", "-std=c++17", "-ffp-contract=off", "-I",
+             os.path.join(ROOT, "voxvision_amd", "csrc"), "-I",
+             os.path.dirname(lib), src, lib, "-o", exe,
+             "-Wl,-rpath," + os.path.dirname(lib), "-lm"],
+            capture_output=True, text=True)
+        if r.returncode:
+            pytest.fail(r.stderr)
+        r = subprocess.run([exe], capture_output=True, text=True, timeout=600)
+        assert r.returncode == 0, r.stdout[-3000:]
+        v = [int(x) for x in r.stdout.split()[1::2][-6:]]
+        single_bad, blocks, root_kept, deep_culled, hit_blocks, bad = v
+        assert single_bad == 0 and bad == 0
+        # informative: the cut finishes a good share of the blocks that
+        # the root box keeps, and many blocks hold hits
+        assert deep_culled > root_kept // 10
+        assert hit_blocks > blocks // 20

@@ -18,6 +18,7 @@
 // rcp(x) must return 1/x within 1 ulp for 2^-60 <= |x| <= 2^128.
 #pragma once
 #include <math.h>
+#include <stdint.h>
 
 // What may_enter needs of the box and the origin, computed once per origin
 // (a camera position is the origin of all of its pixels' rays).
@@ -26,15 +27,22 @@ struct CullOrigin {
     bool unsure;       // o not finite or too large: every ray may enter
 };
 
+// S: the largest magnitude among the coordinates of o, lo and hi
+__host__ __device__ inline float cull_scale(const float lo[3],
+                                            const float hi[3],
+                                            const float o[3]) {
+    float S = 0.0f;
+    for (int i = 0; i < 3; ++i)
+        S = fmaxf(S, fmaxf(fabsf(o[i]), fmaxf(fabsf(lo[i]), fabsf(hi[i]))));
+    return S;
+}
+
 __host__ __device__ inline CullOrigin cull_origin(const float lo[3],
                                                   const float hi[3],
                                                   const float o[3]) {
-    float S = 0.0f;
     bool finite = true;
-    for (int i = 0; i < 3; ++i) {
-        finite = finite && fabsf(o[i]) <= 0x1p127f;
-        S = fmaxf(S, fmaxf(fabsf(o[i]), fmaxf(fabsf(lo[i]), fabsf(hi[i]))));
-    }
+    for (int i = 0; i < 3; ++i) finite = finite && fabsf(o[i]) <= 0x1p127f;
+    const float S = cull_scale(lo, hi, o);
     const float M = fmaxf(S * 0x1p-17f, 0x1p-100f);
     CullOrigin c;
     for (int i = 0; i < 3; ++i) {
@@ -150,4 +158,115 @@ __host__ __device__ inline bool block_may_enter(const CullOrigin &c,
         else ok = ok && a <= 0.0f;
     }
     return big < 0x1p-29f || (ok && t0 <= t1);
+}
+
+// ---- boxes below the root ----
+//
+// The traversal reports a hit at a point it computed inside a voxel's box
+// (a dense leaf: inside the leaf's box), so inside the box of every node
+// above that leaf: a node's box is the tight box of its voxels.  Each point
+// the traversal computes on the way -- a box test's entry point, RN(m d +
+// e) with m > 0 and one coordinate set to the box face, or a plane point
+// fma(k, d, e) with k > 0 (geom-sse.c:84-142) -- lies, coordinate by
+// coordinate, within 4 u S (u = 2^-24) of the point e + m d of the line
+// through the point e before it, S the largest coordinate magnitude of the
+// camera origin o and the root box: every such point lies in a node's box
+// or is o.  The first e is o, and a path from the root to a voxel computes
+// at most 2 D + 1 points in a tree of D levels (an entry point and a plane
+// point per level, then the voxel's entry point; a FAST leaf try computes
+// two), so a reported hit lies within (8 D + 4) u S of a point o + T d,
+// T >= 0, coordinate by coordinate.
+//
+// Hence, with the box of a node enlarged by M = 2^-14 S (1024 u S; the
+// rounding of a = (lo - M) - o and b = (hi + M) - o takes at most 4 u S of
+// it), a block of rays none of which comes within that margin of any box
+// of a cut of the tree (nodes below which every voxel lies) gets no hit in
+// any quality mode, provided 8 D + 12 < 1024: trees of at most
+// kDeepCullMaxDepth levels.
+constexpr unsigned kDeepCullMaxDepth = 120;
+
+// The CullOrigin of a box below the root, S = cull_scale() of the root box
+// and o (the caller has checked the root's CullOrigin is not unsure).
+__host__ __device__ inline CullOrigin deep_origin(const float lo[3],
+                                                  const float hi[3],
+                                                  const float o[3], float S) {
+    const float M = fmaxf(S * 0x1p-14f, 0x1p-100f);
+    CullOrigin c;
+    for (int i = 0; i < 3; ++i) {
+        c.a[i] = (lo[i] - M) - o[i];
+        c.b[i] = (hi[i] + M) - o[i];
+    }
+    c.unsure = false;
+    return c;
+}
+
+// A block tested against many boxes: the direction box's bounds are
+// inverted once (rcp, within 1 ulp as for may_enter_dir), and each slab
+// bound b / l becomes a product b * (1 / l) * (1 +- 2^-20) that lies
+// outside the exact quotient: the factor of larger magnitude where that
+// moves the bound outward, the smaller one elsewhere (the reciprocal, the
+// factor and the product are off by less than 4.01 2^-24 relative), and
+// 2^-126 more for underflow, as cull_up and cull_down do for an exact
+// quotient.  A bound below 2^-60 in magnitude counts as zero: when the
+// test goes on at all, some component of every direction in the box is at
+// least 2^-29 in magnitude, so a ray can stay within M of a box only for
+// t <= 2.01 S / 2^-29, over which that component moves the ray less than
+// 2^-30 S -- the margin above covers it.
+struct DirRcp {
+    // 1 / lo and 1 / hi times 1 + 2^-20 (big) and 1 - 2^-20 (small); all
+    // 0 for a bound counted as zero
+    float lbig[3], lsmall[3], ubig[3], usmall[3];
+    int8_t ls[3], us[3];  // signs of the bounds (0: counted as zero)
+    bool any;             // unsure, or tiny directions: every box may be entered
+};
+
+// the `any` of dir_rcp alone
+__host__ __device__ inline bool dir_any(const DirBox &db) {
+    float big = 0.0f;
+    for (int i = 0; i < 3; ++i) {
+        const float l = db.lo[i], u = db.hi[i];
+        big = fmaxf(big, l > 0.0f ? l : (u < 0.0f ? -u : 0.0f));
+    }
+    return db.unsure || !(big >= 0x1p-29f);
+}
+
+template <class Rcp>
+__host__ __device__ inline DirRcp dir_rcp(const DirBox &db, Rcp rcp) {
+    DirRcp r;
+    for (int i = 0; i < 3; ++i) {
+        const float l = db.lo[i], u = db.hi[i];
+        const bool lz = !(fabsf(l) >= 0x1p-60f), uz = !(fabsf(u) >= 0x1p-60f);
+        const float rl = rcp(lz ? 1.0f : l), ru = rcp(uz ? 1.0f : u);
+        r.lbig[i] = lz ? 0.0f : rl * (1.0f + 0x1p-20f);
+        r.lsmall[i] = lz ? 0.0f : rl * (1.0f - 0x1p-20f);
+        r.ubig[i] = uz ? 0.0f : ru * (1.0f + 0x1p-20f);
+        r.usmall[i] = uz ? 0.0f : ru * (1.0f - 0x1p-20f);
+        r.ls[i] = lz ? 0 : (l > 0.0f ? 1 : -1);
+        r.us[i] = uz ? 0 : (u > 0.0f ? 1 : -1);
+    }
+    r.any = dir_any(db);
+    return r;
+}
+
+// block_may_enter with the bounds of dir_rcp (false only when no direction
+// of the box comes within the box's margin, as above).  Per axis: t lo <= b
+// (lo > 0: t <= b / lo; lo < 0: t >= b / lo) and t hi >= a (hi > 0:
+// t >= a / hi; hi < 0: t <= a / hi).
+__host__ __device__ inline bool block_may_enter_rcp(const CullOrigin &c,
+                                                    const DirRcp &r) {
+    if (r.any || c.unsure) return true;
+    float t0 = 0.0f, t1 = INFINITY;
+    bool ok = true;
+    for (int i = 0; i < 3; ++i) {
+        const float a = c.a[i], b = c.b[i];
+        const float ql = b * (b > 0.0f ? r.lbig[i] : r.lsmall[i]);
+        const float qu = a * (a > 0.0f ? r.usmall[i] : r.ubig[i]);
+        t1 = r.ls[i] > 0 ? fminf(t1, ql + 0x1p-126f) : t1;
+        t0 = r.ls[i] < 0 ? fmaxf(t0, ql - 0x1p-126f) : t0;
+        ok = ok && (r.ls[i] != 0 || b >= 0.0f);
+        t0 = r.us[i] > 0 ? fmaxf(t0, qu - 0x1p-126f) : t0;
+        t1 = r.us[i] < 0 ? fminf(t1, qu + 0x1p-126f) : t1;
+        ok = ok && (r.us[i] != 0 || a <= 0.0f);
+    }
+    return ok && t0 <= t1;
 }

@@ -59,9 +59,65 @@ static uint32_t stack_bound_of(const HostTree &h) {
 // point (16 B), then the node's pending list (4 B) in a second array.
 constexpr size_t kEntryBytes = sizeof(uint4) + sizeof(uint32_t);
 
+// The render pre-pass's cut of the tree: the nodes of the deepest level
+// that has at most kCutMax of them, counting the leaves of the levels above
+// (every voxel lies below exactly one), as boxes of 6 floats (min, max),
+// most voxels first so that a block that may enter finds its box early.
+// Empty (no test beyond the root box) when the cut would be the root alone
+// or the tree is deeper than the bound of the test (cull.h).
+constexpr size_t kCutMax = 64;  // the pre-pass tables at most 64 boxes
+
+static std::vector<float> cut_of(const HostTree &h) {
+    std::vector<float> out;
+    if (h.nodes.empty() || h.depth > kDeepCullMaxDepth) return out;
+    std::vector<uint32_t> cur{0u}, next;
+    for (;;) {
+        next.clear();
+        bool inner = false;
+        for (uint32_t i : cur) {
+            const NodeRec &n = h.nodes[i];
+            if (node_type(n.info) == NT_INNER) {
+                inner = true;
+                const uint32_t k = (uint32_t)__builtin_popcount(child_mask(n.info));
+                for (uint32_t c = 0; c < k; ++c) next.push_back(n.ref + c);
+            } else {
+                next.push_back(i);
+            }
+        }
+        if (!inner || next.size() > kCutMax) break;
+        cur.swap(next);
+    }
+    if (cur.size() < 2) return out;
+    if (h.counts.size() == h.nodes.size())
+        std::stable_sort(cur.begin(), cur.end(), [&](uint32_t x, uint32_t y) {
+            return h.counts[x] > h.counts[y];
+        });
+    for (uint32_t i : cur) {
+        const NodeRec &n = h.nodes[i];
+        out.insert(out.end(), {n.bmin[0], n.bmin[1], n.bmin[2], n.bmax[0],
+                               n.bmax[1], n.bmax[2]});
+    }
+    return out;
+}
+
+// HBM of an uploaded tree: node records, dots, then the cut's boxes
+struct TreeBytes {
+    size_t dots_off, cut_off, total;
+};
+
+static TreeBytes tree_bytes(const HostTree &h, size_t cut_floats) {
+    const size_t nb = h.nodes.size() * sizeof(NodeRec);
+    const size_t db = h.dots.size() * sizeof(float);
+    TreeBytes b;
+    b.dots_off = (nb + 255) & ~(size_t)255;
+    b.cut_off = (b.dots_off + db + 255) & ~(size_t)255;
+    b.total = cut_floats ? b.cut_off + cut_floats * sizeof(float)
+                         : b.dots_off + db;
+    return b;
+}
+
 static size_t device_bytes_of(const HostTree &h) {
-    size_t nb = h.nodes.size() * sizeof(NodeRec);
-    return ((nb + 255) & ~(size_t)255) + h.dots.size() * sizeof(float);
+    return tree_bytes(h, cut_of(h).size()).total;
 }
 
 // ---------------------------------------------------------------------------
@@ -177,7 +233,8 @@ struct Tree {
     float vx, vy, vz;
     uint32_t empty;
     float ivx, ivy, ivz;  // 1/V when V is a power of two, else 0
-    uint32_t pad;
+    uint32_t ncut;        // boxes of the render pre-pass's cut (0: none)
+    const float *__restrict__ cut;  // 6 floats per box: min, max
 };
 
 // x / v, as a multiplication when v is a power of two (iv = 1/v exactly):
@@ -1320,11 +1377,19 @@ __global__ void __launch_bounds__(256)
 block_cull_kernel(RenderItem it, uint32_t npix, Ctrl *ctrl,
                   uint32_t *__restrict__ queue, uint32_t qlen) {
     __shared__ uint8_t enter[kCullBlocks];
+    __shared__ uint16_t dlist[kCullBlocks];
+    __shared__ float ddir[6][kCullBlocks];
+    __shared__ float cutab[2][kCutMax][6];
     __shared__ uint32_t wave_sum[4];
     __shared__ uint32_t gbase[2];
+    __shared__ uint32_t ndeep;
     __shared__ unsigned long long missed;
     const uint32_t tid = threadIdx.x, lane = lane_id(), w = tid >> 6;
-    if (tid == 0) missed = 0;
+    if (tid == 0) {
+        missed = 0;
+        ndeep = 0;
+    }
+    __syncthreads();
     const uint32_t tile0 = blockIdx.x * kCullTile;
     uint32_t my_missed = 0;
     // the host launches this kernel only for a tree with a root record
@@ -1345,41 +1410,55 @@ block_cull_kernel(RenderItem it, uint32_t npix, Ctrl *ctrl,
     // is not 16-byte aligned takes them one by one from the testing thread
     uint32_t *const vox = it.sink.voxel;
     const bool vox16 = ((uintptr_t)vox & 15u) == 0;
-    // one thread per block, 4 blocks per thread: the four corner rays bound
-    // the directions of the block's 16 rays (block_dirs, cull.h)
+    // a block's camera and the box of its 16 directions (block_dirs,
+    // cull.h: the four corner rays bound the others)
+    auto block_geom = [&](uint32_t blk, const CamDev *&cam, float (&dc)[4][3]) {
+        const uint32_t f = it.bpf.div(blk);
+        const uint32_t cs = blk - f * it.bpf.d;
+        const uint32_t row = it.ws.div(cs);
+        const CamDev &c = it.cams[f];
+        cam = &c;
+        const int x0 = (int)(cs - row * it.ws.d) * 4, y0 = (int)row * 4;
+        for (int k = 0; k < 4; ++k) {
+            const f3 d = screen2world(c, x0 + 3 * (k & 1), y0 + 3 * (k >> 1));
+            dc[k][0] = d.x;
+            dc[k][1] = d.y;
+            dc[k][2] = d.z;
+        }
+        // the bound's scale: the largest quaternion component and the
+        // largest unrotated component (screen2world's vector)
+        const float Q = fmaxf(fmaxf(fabsf(c.qw), fabsf(c.qi)),
+                              fmaxf(fabsf(c.qj), fabsf(c.qk)));
+        float V = 1.0f;
+        V = fmaxf(V, fabsf(__builtin_fmaf(c.mul, (float)x0, -c.xsub)));
+        V = fmaxf(V, fabsf(__builtin_fmaf(c.mul, (float)(x0 + 3), -c.xsub)));
+        V = fmaxf(V, fabsf(__builtin_fmaf(-c.mul, (float)y0, c.ysub)));
+        V = fmaxf(V, fabsf(__builtin_fmaf(-c.mul, (float)(y0 + 3), c.ysub)));
+        return block_dirs(dc, Q, V);
+    };
+    // (not when counting: the counts add one root box test per finished
+    // ray, which a deeper test would not match)
+    const bool deep = !COUNT && it.t.ncut != 0;
+    // one thread per block, 4 blocks per thread
     for (uint32_t j = 0; j < kCullBlocks / 256; ++j) {
         const uint32_t lb = j * 256 + tid;
         const uint32_t blk = (tile0 >> 4) + lb;
         const bool valid = blk * 16u < npix;
         uint8_t cls = 0;
+        bool listed = false;
+        uint32_t fs = 0;
+        DirBox db;
         if (valid && !it.t.empty) {
-            const uint32_t f = it.bpf.div(blk);
-            const uint32_t cs = blk - f * it.bpf.d;
-            const uint32_t row = it.ws.div(cs);
-            const CamDev &c = it.cams[f];
-            const int x0 = (int)(cs - row * it.ws.d) * 4, y0 = (int)row * 4;
+            const CamDev *c;
+            float dc[4][3];
+            db = block_geom(blk, c, dc);
             CullOrigin co = co0;
-            if (f != f0) {
-                const float o3[3] = {c.px, c.py, c.pz};
+            fs = it.bpf.div(blk) - f0;
+            if (fs != 0) {
+                const float o3[3] = {c->px, c->py, c->pz};
                 co = cull_origin(lo3, hi3, o3);
             }
-            float dc[4][3];
-            for (int k = 0; k < 4; ++k) {
-                const f3 d = screen2world(c, x0 + 3 * (k & 1), y0 + 3 * (k >> 1));
-                dc[k][0] = d.x;
-                dc[k][1] = d.y;
-                dc[k][2] = d.z;
-            }
-            // the bound's scale: the largest quaternion component and the
-            // largest unrotated component (screen2world's vector)
-            const float Q = fmaxf(fmaxf(fabsf(c.qw), fabsf(c.qi)),
-                                  fmaxf(fabsf(c.qj), fabsf(c.qk)));
-            float V = 1.0f;
-            V = fmaxf(V, fabsf(__builtin_fmaf(c.mul, (float)x0, -c.xsub)));
-            V = fmaxf(V, fabsf(__builtin_fmaf(c.mul, (float)(x0 + 3), -c.xsub)));
-            V = fmaxf(V, fabsf(__builtin_fmaf(-c.mul, (float)y0, c.ysub)));
-            V = fmaxf(V, fabsf(__builtin_fmaf(-c.mul, (float)(y0 + 3), c.ysub)));
-            if (block_may_enter(co, block_dirs(dc, Q, V))) {
+            if (block_may_enter(co, db)) {
                 // 2: the four corner rays enter the box, 1: some ray may
                 // (the queue order only)
                 bool all = true;
@@ -1388,6 +1467,30 @@ block_cull_kernel(RenderItem it, uint32_t npix, Ctrl *ctrl,
                               return __builtin_amdgcn_rcpf(v);
                           });
                 cls = all ? 2 : 1;
+                // listed for the test against the cut's boxes when its
+                // frame is the tile's first or second (the boxes are
+                // tabled for those two; a tile of small frames holds
+                // more, whose blocks stay queued)
+                listed = deep && fs < 2u && !co.unsure && !db.unsure;
+            }
+        }
+        if (deep) {
+            // in lane order, one LDS atomic per wave: a wave's 64 list
+            // entries then come from neighbouring blocks
+            const unsigned long long m = __ballot(listed);
+            if (m) {
+                const uint32_t first = (uint32_t)(__ffsll((long long)m) - 1);
+                uint32_t base = 0;
+                if (lane == first) base = atomicAdd(&ndeep, (uint32_t)__popcll(m));
+                base = __shfl(base, (int)first);
+                if (listed) {
+                    const uint32_t k = base + lanes_below(m);
+                    dlist[k] = (uint16_t)(lb | (fs << 15));
+                    for (int i = 0; i < 3; ++i) {
+                        ddir[i][k] = db.lo[i];
+                        ddir[3 + i][k] = db.hi[i];
+                    }
+                }
             }
         }
         if (valid && cls == 0 && vox && !vox16) {
@@ -1396,6 +1499,101 @@ block_cull_kernel(RenderItem it, uint32_t npix, Ctrl *ctrl,
         }
         enter[lb] = valid ? cls : 0;
         if (COUNT && valid && cls == 0) my_missed++;
+    }
+    if (deep) {
+        // The listed blocks, one per lane: finished as well when none of
+        // their rays comes near any box of the cut (deep_origin,
+        // block_may_enter_rcp: cull.h).  The boxes' slab bounds relative to
+        // the camera (a, b of deep_origin) are tabled for the tile's first
+        // two frames.  A wave first tests every box once against the union
+        // of its 64 blocks' direction boxes (lane j, box j): a box that no
+        // direction of the union comes near is none of its blocks' concern.
+        const uint32_t ncut = min(it.t.ncut, (uint32_t)kCutMax);
+        const uint32_t nfr = npix / 16u / it.bpf.d;
+        if (tid < 2u * ncut) {
+            const uint32_t fs = tid / ncut, j = tid - fs * ncut;
+            if (f0 + fs < nfr) {
+                const CamDev &c = it.cams[f0 + fs];
+                const float o3[3] = {c.px, c.py, c.pz};
+                const float *bx = it.t.cut + 6u * j;
+                const CullOrigin cj =
+                    deep_origin(bx, bx + 3, o3, cull_scale(lo3, hi3, o3));
+                for (int i = 0; i < 3; ++i) {
+                    cutab[fs][j][i] = cj.a[i];
+                    cutab[fs][j][3 + i] = cj.b[i];
+                }
+            }
+        }
+        __syncthreads();
+        auto cut_box = [&](uint32_t fs, uint32_t j) {
+            CullOrigin cj;
+            for (int i = 0; i < 3; ++i) {
+                cj.a[i] = cutab[fs][j][i];
+                cj.b[i] = cutab[fs][j][3 + i];
+            }
+            cj.unsure = false;
+            return cj;
+        };
+        const uint32_t nd = ndeep;
+        for (uint32_t k0 = tid & ~63u; k0 < nd; k0 += 256) {
+            const uint32_t k = k0 + lane;
+            const bool have = k < nd;
+            uint32_t e = 0;
+            DirBox db;
+            for (int i = 0; i < 3; ++i) {
+                db.lo[i] = INFINITY;
+                db.hi[i] = -INFINITY;
+            }
+            db.unsure = false;
+            if (have) {
+                e = dlist[k];
+                for (int i = 0; i < 3; ++i) {
+                    db.lo[i] = ddir[i][k];
+                    db.hi[i] = ddir[3 + i][k];
+                }
+            }
+            const uint32_t fs = e >> 15;
+            DirBox ub;
+            for (int i = 0; i < 3; ++i) {
+                float lo = db.lo[i], hi = db.hi[i];
+                for (int o = 32; o > 0; o >>= 1) {
+                    lo = fminf(lo, __shfl_xor(lo, o));
+                    hi = fmaxf(hi, __shfl_xor(hi, o));
+                }
+                ub.lo[i] = lo;
+                ub.hi[i] = hi;
+            }
+            ub.unsure = false;
+            auto rcp = [](float v) { return __builtin_amdgcn_rcpf(v); };
+            const DirRcp ur = dir_rcp(ub, rcp);
+            const bool two = __ballot(have && fs == 1u) != 0;
+            const unsigned long long m0 =
+                __ballot(lane < ncut && block_may_enter_rcp(cut_box(0, lane), ur));
+            const unsigned long long m1 =
+                two ? __ballot(lane < ncut &&
+                               block_may_enter_rcp(cut_box(1, lane), ur))
+                    : 0ull;
+            if (have) {
+                // (a block whose union leaves no box finishes without its
+                // own reciprocals)
+                unsigned long long m = fs ? m1 : m0;
+                bool keep = dir_any(db);
+                if (m != 0 && !keep) {
+                    const DirRcp r = dir_rcp(db, rcp);
+                    for (; m != 0 && !keep; m &= m - 1ull)
+                        keep = block_may_enter_rcp(
+                            cut_box(fs, (uint32_t)(__ffsll((long long)m) - 1)), r);
+                }
+                if (!keep) {
+                    const uint32_t lb = e & 0x3ffu;
+                    enter[lb] = 0;
+                    if (vox && !vox16) {
+                        uint32_t *v = vox + (size_t)((tile0 >> 4) + lb) * 16u;
+                        for (int q = 0; q < 16; ++q) v[q] = kEnd;
+                    }
+                }
+            }
+        }
     }
     __syncthreads();
     if (vox && vox16) {
@@ -2018,6 +2216,8 @@ struct voxg_tree {
     HostTree host;
     NodeRec *d_nodes = nullptr;
     float *d_dots = nullptr;
+    float *d_cut = nullptr;
+    uint32_t ncut = 0;
     void *d_mem = nullptr;
     size_t bytes = 0;
     int device = 0;
@@ -2294,6 +2494,8 @@ dev::Tree dev_tree(const voxg_tree *t) {
     d.ivy = pow2_inverse(d.vy);
     d.ivz = pow2_inverse(d.vz);
     d.empty = t->host.nodes.empty() ? 1u : 0u;
+    d.ncut = t->ncut;
+    d.cut = t->d_cut;
     return d;
 }
 
@@ -2431,15 +2633,27 @@ int upload(voxg_tree *t) {
     const HostTree &h = t->host;
     size_t nb = h.nodes.size() * sizeof(NodeRec);
     size_t db = h.dots.size() * sizeof(float);
-    size_t nb_al = (nb + 255) & ~(size_t)255;
-    t->bytes = nb_al + db;
+    std::vector<float> cut;
+    try {
+        cut = cut_of(h);
+    } catch (const std::bad_alloc &) {
+        return fail(VOXG_ENOMEM, "host allocation failed");
+    }
+    const TreeBytes tb = tree_bytes(h, cut.size());
+    t->bytes = tb.total;
     HIPCHK(hipGetDevice(&t->device));
     if (t->bytes == 0) return 0;
     HIPCHK(hipMalloc(&t->d_mem, t->bytes));
     t->d_nodes = (NodeRec *)t->d_mem;
-    t->d_dots = (float *)((char *)t->d_mem + nb_al);
+    t->d_dots = (float *)((char *)t->d_mem + tb.dots_off);
     if (nb) HIPCHK(hipMemcpy(t->d_nodes, h.nodes.data(), nb, hipMemcpyHostToDevice));
     if (db) HIPCHK(hipMemcpy(t->d_dots, h.dots.data(), db, hipMemcpyHostToDevice));
+    if (!cut.empty()) {
+        t->d_cut = (float *)((char *)t->d_mem + tb.cut_off);
+        t->ncut = (uint32_t)(cut.size() / 6);
+        HIPCHK(hipMemcpy(t->d_cut, cut.data(), cut.size() * sizeof(float),
+                         hipMemcpyHostToDevice));
+    }
     return 0;
 }
 
