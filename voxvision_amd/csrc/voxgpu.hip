@@ -1357,10 +1357,14 @@ rti_kernel(Tree t, Work work_arg, uint32_t n, Ctrl *ctrl, uint32_t *ovf,
 // with the conservative block_may_enter (cull.h: false only when the
 // traversal's exact hit_box rejects every ray of the block; testing each
 // of the 16 rays instead made the pass 0.31 ms on the benchmark frames,
-// bound by that per-pixel arithmetic).  A block that no ray can enter is
-// finished here -- every pixel is a miss in every quality mode -- and the
-// others go, in block order, to the queue the traversal kernel takes its
-// blocks from.  A workgroup covers kCullTile pixels (1024 blocks): it marks
+// bound by that per-pixel arithmetic).  A block that may enter the root box
+// is then tested against the boxes of the tree's cut (cut_of) with the
+// wider margin of deep_origin (cull.h), except in counting launches.  A
+// block that no ray can enter, or that no ray can take near a box of the
+// cut, is finished here -- every pixel is a miss in every quality mode --
+// and the others go, in block order, to the queue the traversal kernel
+// takes its blocks from.  A workgroup covers kCullTile pixels (1024
+// blocks): it marks
 // the entering blocks in LDS, compacts them with a workgroup scan and
 // reserves its queue ranges with a single 64-bit atomic (the queue
 // counters share one address, so their atomics are serialised: one per
