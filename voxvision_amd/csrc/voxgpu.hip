@@ -880,12 +880,13 @@ struct SpecOut {
         }
     }
     // direct FAST/ADAPTIVE renders: the leaf node of every pixel (kEnd on
-    // a miss), and the hit points of corners 0 and 15 for ADAPTIVE's
-    // decision; the results themselves go to the outputs
+    // a miss), a block's 16 together (index block * 16 + pixel: the
+    // neighbouring blocks that a wave finishes together write whole lines),
+    // and the hit points of corners 0 and 15 for ADAPTIVE's decision; the
+    // results themselves go to the outputs
     __device__ __forceinline__ void put_node(uint32_t blk, int px, bool hit,
                                              uint32_t nd, f3 p) const {
-        const uint32_t r = (uint32_t)px * stride + blk;
-        node[r] = hit ? nd : kEnd;
+        node[blk * 16u + (uint32_t)px] = hit ? nd : kEnd;
         if (hit && (px == 0 || px == 15)) {
             const uint32_t c = (px == 15 ? stride : 0u) + blk;
             cx[c] = p.x;
@@ -1727,7 +1728,7 @@ resolve_kernel(RenderItem it, SpecOut spec, const uint32_t *__restrict__ queue,
                         bool corner) {
             const uint32_t r = (uint32_t)p * spec.stride + id;
             if constexpr (DIRECT) {
-                nd = spec.node[r];
+                nd = spec.node[id * 16u + (uint32_t)p];
                 vid = kEnd;  // used by merge copies only
                 if (corner) {
                     const uint32_t k = (p == 15 ? spec.stride : 0u) + id;
@@ -1822,10 +1823,15 @@ resolve_kernel(RenderItem it, SpecOut spec, const uint32_t *__restrict__ queue,
             // once, rather than one per step; a block with none has nothing
             // to replay (its leaf stays empty), and neither has a wave
             uint32_t hits = 0;
-            if (i1 > i0)
-                for (int q = 0; q < 16; ++q)
-                    hits |= (spec.node[(uint32_t)q * spec.stride + id] != kEnd)
-                                ? 1u << q : 0u;
+            if (i1 > i0) {
+                const uint4 *n4 = reinterpret_cast<const uint4 *>(spec.node) + id * 4u;
+                for (int q = 0; q < 4; ++q) {
+                    const uint4 v = n4[q];
+                    hits |= ((v.x != kEnd ? 1u : 0u) | (v.y != kEnd ? 2u : 0u) |
+                             (v.z != kEnd ? 4u : 0u) | (v.w != kEnd ? 8u : 0u))
+                            << (4 * q);
+                }
+            }
             if (hits == 0) i1 = i0;
             for (int k = 0; k < 16; ++k) {
                 if (__ballot(i0 + k < i1) == 0) break;
@@ -1837,7 +1843,7 @@ resolve_kernel(RenderItem it, SpecOut spec, const uint32_t *__restrict__ queue,
                 if (ii < i1) {
                     const bool rhit = (hits >> p) & 1u;
                     const uint32_t rnd =
-                        rhit ? spec.node[(uint32_t)p * spec.stride + id] : kEnd;
+                        rhit ? spec.node[id * 16u + (uint32_t)p] : kEnd;
                     bool kept = false;
                     if (leaf != kEnd) {
                         // renderer.c:294-300: the previous pixel's leaf first
