@@ -6,6 +6,8 @@
 #ifndef VOXGPU_TESTING_H
 #define VOXGPU_TESTING_H
 
+#include <stddef.h>
+
 #ifdef __cplusplus
 extern "C" {
 #endif
@@ -31,6 +33,14 @@ int voxg_test_stack_bound(unsigned bound);
  * form a group of a chosen mix. */
 struct voxg_server;
 int voxg_test_server_hold(struct voxg_server *s, int hold);
+
+/* The render pre-pass's cut of the tree that voxg_tree_build would make
+ * from these voxels (host only, no device needed): up to cap boxes as 6
+ * floats each (min xyz, max xyz) into out, their number into *count (0:
+ * no cut, the tree is too deep or the cut would be the root alone). */
+int voxg_test_plan_cut(const float *xyz, size_t n, size_t stride,
+                       const float voxel[3], float *out, size_t cap,
+                       size_t *count);
 
 #ifdef __cplusplus
 }

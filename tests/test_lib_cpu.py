@@ -419,3 +419,51 @@ def test_lights_match_the_oracle(voxlib, oracle, seed):
     O.clear()
     assert len(G) == 0 and len(O) == 0
     assert np.array_equal(G.light(pts), O.light(pts))
+
+
+def _cut_scenes():
+    yield from _scenes()
+    from voxvision_amd import scenes
+    # a sparse plate of voxels at z = 0 and three above it: the root's
+    # children above the plate hold one voxel each and are leaves, which
+    # the cut must keep from the level above its own
+    rng = np.random.default_rng(5)
+    ii = np.arange(40, dtype=np.float32)
+    X, Y = np.meshgrid(ii, ii, indexing="ij")
+    keep = rng.random(X.size) < 0.6
+    plate = np.stack([X.ravel(), Y.ravel(), np.zeros(X.size, np.float32)],
+                     1)[keep]
+    above = np.array([[3, 4, 50], [30, 5, 60], [20, 33, 40]], np.float32)
+    yield "plate_above", np.concatenate([plate, above]), (1.0, 1.0, 1.0)
+    yield "ct64", scenes.volume_to_dots(scenes.ct_volume(64, seed=3)), \
+        (1.0, 1.0, 1.0)
+
+
+@pytest.mark.parametrize("case", list(range(8)))
+def test_tree_cut_covers_every_voxel(voxlib, case):
+    """The render pre-pass finishes a block when its rays miss every box
+    of the tree's cut, so every voxel's box must lie in one of those boxes
+    (a cut that left out the leaves of the levels above it would drop the
+    voxels below them).  At most 64 boxes, and more than one, or none."""
+    import ctypes as C
+    name, dots, V = list(_cut_scenes())[case]
+    d = np.ascontiguousarray(dots, np.float32)
+    v = (C.c_float * 3)(*V)
+    out = np.zeros((64, 6), np.float32)
+    cnt = C.c_size_t()
+    assert voxlib.voxg_test_plan_cut(d.ctypes.data, len(d), 3, v,
+                                     out.ctypes.data, 64, C.byref(cnt)) == 0
+    n = cnt.value
+    assert n == 0 or 2 <= n <= 64, (name, n)
+    if n == 0:
+        return
+    box = out[:n]
+    lo = d[:, None, :]
+    hi = (d + np.asarray(V, np.float32))[:, None, :]
+    inside = ((box[None, :, 0:3] <= lo) & (hi <= box[None, :, 3:6])).all(2)
+    assert inside.any(1).all(), name
+    # a cap below the cut's size is refused
+    if n > 1:
+        assert voxlib.voxg_test_plan_cut(d.ctypes.data, len(d), 3, v,
+                                         out.ctypes.data, n - 1,
+                                         C.byref(cnt)) == -22

@@ -36,7 +36,7 @@ EXPORTS = [
     "voxg_edit_rebuild", "voxg_edit_root", "voxg_edit_voxels",
     "voxg_edit_commit", "voxg_edit_release", "voxg_server_replace_tree",
     "voxg_synchronize", "voxg_release_stream", "voxg_test_fail_at",
-    "voxg_test_stack_bound", "voxg_test_server_hold",
+    "voxg_test_stack_bound", "voxg_test_server_hold", "voxg_test_plan_cut",
     "voxg_ray_batch_noalloc",
     "voxg_lights_create", "voxg_lights_release", "voxg_lights_set_ambient",
     "voxg_lights_insert", "voxg_lights_delete", "voxg_lights_clear",
@@ -178,6 +178,7 @@ def load():
     L.voxg_test_fail_at.argtypes = [I]
     L.voxg_test_stack_bound.argtypes = [C.c_uint]
     L.voxg_test_server_hold.argtypes = [P, C.c_int]
+    L.voxg_test_plan_cut.argtypes = [P, S, S, P, P, S, C.POINTER(S)]
     _lib = L
     return L
 

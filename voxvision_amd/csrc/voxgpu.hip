@@ -44,6 +44,7 @@
 #include "lights.h"
 #include "tree_build.h"
 #include "voxg_internal.h"
+#include "../../include/voxgpu_testing.h"
 #include "voxg_layout.h"
 
 using namespace voxg;
@@ -2842,6 +2843,32 @@ int voxg_tree_plan(const float *xyz, size_t n, size_t stride,
     if (rc) return fail(rc, err);
     if (info) fill_info(h, info);
     if (hash) *hash = tree_hash(h);
+    return 0;
+}
+
+int voxg_test_plan_cut(const float *xyz, size_t n, size_t stride,
+                       const float voxel[3], float *out, size_t cap,
+                       size_t *count) {
+    if ((n && !xyz) || !count || (cap && !out))
+        return fail(VOXG_EINVAL, "NULL argument");
+    if (stride < 3) return fail(VOXG_EINVAL, "stride must be >= 3");
+    float V[3] = {1.0f, 1.0f, 1.0f};
+    if (voxel) std::memcpy(V, voxel, sizeof(V));
+    if (!(V[0] > 0.0f && V[1] > 0.0f && V[2] > 0.0f))
+        return fail(VOXG_EINVAL, "voxel size must be positive");
+    HostTree h;
+    std::string err;
+    std::vector<float> cut;
+    try {
+        const int rc = build_tree(xyz, n, stride, V, h, err);
+        if (rc) return fail(rc, err);
+        cut = cut_of(h);
+    } catch (const std::bad_alloc &) {
+        return fail(VOXG_ENOMEM, "host allocation failed");
+    }
+    *count = cut.size() / 6;
+    if (*count > cap) return fail(VOXG_EINVAL, "cap is too small");
+    if (!cut.empty()) std::memcpy(out, cut.data(), cut.size() * sizeof(float));
     return 0;
 }
 
