@@ -718,7 +718,10 @@ def main():
                                          2),
                 "timed": "render call: block_cull_kernel + rti_kernel",
                 "node_visits_per_ray": round(cnt["node_visits"] / n, 4),
-                "voxel_tests_per_ray": round(cnt["voxel_tests"] / n, 4)}
+                "voxel_tests_per_ray": round(cnt["voxel_tests"] / n, 4),
+                # stack entries that left the LDS ring for the spill
+                # buffer (20 B out and back each), counting launch
+                "stack_spills": int(cnt["stack_spills"])}
 
     extra = {"hit_rate": round(hits / n, 4)}
     legs = set() if args.quick else set(x for x in args.legs.split(",") if x)
