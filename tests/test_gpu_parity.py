@@ -646,3 +646,47 @@ def test_render_three_pass_edge_cases(gpu, oracle, quality):
     if quality == 2 | 4:
         # plain merging starts in every block and stops in some
         assert 0 < st_in["merge_blocks"] < st_in["blocks"]
+
+
+@pytest.mark.parametrize("quality", [0, 2])
+def test_cut_keeps_blocks_near_lone_voxels(gpu, oracle, quality):
+    """The pre-pass finishes blocks whose rays miss every box of the tree's
+    cut.  Here three lone voxels above a sparse plate sit in leaves one
+    level above the cut's own: cameras aimed at them (and grazing past
+    them) must still see them, as the oracle does."""
+    from voxvision_amd import camera, voxrnd, voxtrees as VT
+    rng = np.random.default_rng(5)
+    ii = np.arange(40, dtype=np.float32)
+    X, Y = np.meshgrid(ii, ii, indexing="ij")
+    keep = rng.random(X.size) < 0.6
+    plate = np.stack([X.ravel(), Y.ravel(), np.zeros(X.size, np.float32)],
+                     1)[keep]
+    lone = np.array([[3, 4, 50], [30, 5, 60], [20, 33, 40]], np.float32)
+    dots = np.concatenate([plate, lone])
+    T = oracle.Tree(dots)
+    G = VT.make_tree(dots)
+    W, H = 320, 240
+    fov = np.tan(np.radians(20))
+    cams = []
+    for p in lone:
+        c = p + 0.5
+        for off in ([60, 5, 3], [-40, 30, 10], [5, -70, -2]):
+            cams.append(camera.make_camera(c + np.asarray(off, np.float32),
+                                           c + rng.normal(0, 0.3, 3), fov,
+                                           W, H))
+    out = voxrnd.render(G, cams, W, H, quality=quality, want_colors=False)
+    torch.cuda.synchronize()
+    vg = _np_u32(out["voxel"]).reshape(len(cams), -1)
+    hg = out["hit"].cpu().numpy().reshape(3, len(cams), -1)
+    lone_ids = set()
+    for i, cam in enumerate(cams):
+        _, vox_o, hit_o, _ = T.render(cam, W, H, quality)
+        assert np.array_equal(vg[i], vox_o), i
+        m = vox_o != MISS
+        assert np.array_equal(hg[:, i, m].view(np.uint32),
+                              hit_o[:, m].view(np.uint32)), i
+        lone_ids |= set(int(v) for v in np.unique(vox_o[m]))
+    # the lone voxels were seen (their ids are among the hits)
+    coords = G.voxel_coords(np.array(sorted(lone_ids), np.uint32))
+    seen = {tuple(map(float, c)) for c in coords}
+    assert all(tuple(map(float, p)) in seen for p in lone)
