@@ -578,7 +578,7 @@ __device__ __forceinline__ float sqr_metric(f3 a, f3 b) {
 // box at e, hits first: a dense leaf's voxel at e, or the voxel whose entry
 // point is L1-closest to e (strictly, in stored order; an entry at e ends
 // the search).  Writes the voxel id and the hit point; false: no voxel.
-template <bool COUNT>
+template <bool COUNT, bool PREFETCH = false>
 __device__ __forceinline__ bool leaf_search(const Tree &T, const Node &nd,
                                             f3 e, f3 d, uint32_t &vid,
                                             f3 &bres,
@@ -592,7 +592,47 @@ __device__ __forceinline__ bool leaf_search(const Tree &T, const Node &nd,
     const uint32_t cnt = nd.info >> 8;
     float best = INFINITY;
     bool hit = false;
-    for (uint32_t k = 0; k < cnt; k++) {
+    uint32_t k0 = 0;
+    if constexpr (PREFETCH) {
+        // the first 7 voxels' corners loaded together, not one round trip
+        // per voxel
+        float qx[7], qy[7], qz[7];
+        const float *base = T.dots + 3u * nd.ref;
+#pragma unroll
+        for (int k = 0; k < 7; ++k) {
+            qx[k] = qy[k] = qz[k] = 0.0f;
+            if ((uint32_t)k < cnt) {
+                qx[k] = base[3 * k];
+                qy[k] = base[3 * k + 1];
+                qz[k] = base[3 * k + 2];
+            }
+        }
+#pragma unroll
+        for (int k = 0; k < 7; ++k) {
+            if ((uint32_t)k < cnt) {
+                f3 vmin = f3{qx[k], qy[k], qz[k]};
+                f3 vmax = f3{vmin.x + V.x, vmin.y + V.y, vmin.z + V.z};
+                f3 far;
+                if (COUNT) nt++;
+                if (hit_box(vmin, vmax, e, d, far)) {
+                    float m = abs_metric(e, far);
+                    if (m == 0.0f) {
+                        bres = far;
+                        vid = nd.ref + (uint32_t)k;
+                        return true;
+                    }
+                    if (m < best) {
+                        best = m;
+                        bres = far;
+                        vid = nd.ref + (uint32_t)k;
+                        hit = true;
+                    }
+                }
+            }
+        }
+        k0 = 7;
+    }
+    for (uint32_t k = k0; k < cnt; k++) {
         const float *p = T.dots + 3u * (nd.ref + k);
         f3 vmin = f3{p[0], p[1], p[2]};
         f3 vmax = f3{vmin.x + V.x, vmin.y + V.y, vmin.z + V.z};
@@ -1855,7 +1895,7 @@ resolve_kernel(RenderItem it, SpecOut spec, const uint32_t *__restrict__ queue,
                         }
                         f3 e;
                         if (hit_box(ln.bmin, ln.bmax, o, d, e) &&
-                            leaf_search<false>(T, ln, e, d, svid, spt, nt)) {
+                            leaf_search<false, true>(T, ln, e, d, svid, spt, nt)) {
                             st = true;
                             kept = true;  // the leaf stays the same
                         }
@@ -1901,7 +1941,7 @@ resolve_kernel(RenderItem it, SpecOut spec, const uint32_t *__restrict__ queue,
                 f3 e, pt;
                 uint32_t vid = kEnd;
                 if (hit_box(ln.bmin, ln.bmax, o, d, e) &&
-                    leaf_search<false>(T, ln, e, d, vid, pt, nt)) {
+                    leaf_search<false, true>(T, ln, e, d, vid, pt, nt)) {
                     store(p, vid, pt);
                     vid_last = vid;
                     inter_last = pt;
