@@ -122,7 +122,9 @@ __device__ __forceinline__ uint32_t load_samples(const RawArgs &a,
         const uint4 *q = reinterpret_cast<const uint4 *>(a.data + first * SS);
 #pragma unroll
         for (int k = 0; k < NQ; k++) {
-            const uint4 x = q[k];
+            typedef unsigned int u4 __attribute__((ext_vector_type(4)));
+            const u4 xv = __builtin_nontemporal_load(reinterpret_cast<const u4 *>(q + k));
+            const uint4 x = make_uint4(xv.x, xv.y, xv.z, xv.w);
             w[4 * k + 0] = x.x;
             w[4 * k + 1] = x.y;
             w[4 * k + 2] = x.z;
@@ -277,7 +279,8 @@ raw_write_kernel(RawOut a, RawWork w, float *__restrict__ xyz,
         const uint32_t cnt = min(kStage, total - r0);
         const uint64_t o0 = base + r0;
         for (uint32_t t = threadIdx.x; t < 3u * cnt; t += 256u)
-            if (o0 + t / 3u < capacity) xyz[3u * o0 + t] = stage[t];
+            if (o0 + t / 3u < capacity)
+                __builtin_nontemporal_store(stage[t], xyz + 3u * o0 + t);
         __syncthreads();
     }
 }
