@@ -1,0 +1,131 @@
+"""GPU parity on scenes unlike the benchmark's: voxel sizes other than 1
+(half, anisotropic, an inexact 0.001), coordinates far from the origin,
+and degenerate dot sets (a line, a plane, one dot, two far-apart dots).
+Rays include axis-parallel ones through voxel corners and face centres and
+ones with a zero direction component; renders (BEST and ADAPTIVE) take the
+pre-pass with the tree's cut, whose margin scales with the scene.  Same bar
+as the other parity tests: bit-exact voxel ids and hit points, exact ball
+booleans, against the oracle on identical inputs.
+"""
+import numpy as np
+import pytest
+import torch
+
+pytestmark = pytest.mark.gpu
+
+MISS = 0xFFFFFFFF
+
+
+def _u32(t):
+    return (t.cpu().numpy().astype(np.int64) & 0xFFFFFFFF).astype(np.uint32)
+
+
+def _scenes():
+    from voxvision_amd import scenes
+    rng = np.random.default_rng(41)
+    out = []
+    occ = rng.random((20, 18, 22)) < 0.15
+    out.append(("sparse, voxel 0.5, offset 1000.25",
+                scenes.volume_to_dots(occ, (0.5,) * 3) + np.float32(1000.25),
+                (0.5, 0.5, 0.5)))
+    v = scenes.value_noise(24, 3)
+    occ = v > np.quantile(v, 0.7)
+    aniso = (1.0, 0.5, 3.0)
+    out.append(("blobs, voxel (1, 0.5, 3), offset (-6e4, 17, 2.5e3)",
+                scenes.volume_to_dots(occ, aniso) +
+                np.array([-6e4, 17, 2.5e3], np.float32), aniso))
+    out.append(("blobs, voxel 0.001", scenes.volume_to_dots(occ, (1e-3,) * 3),
+                (1e-3,) * 3))
+    line = np.zeros((40, 3), np.float32)
+    line[:, 0] = np.sort(rng.choice(90, 40, replace=False))
+    out.append(("a line of dots", line, (1.0,) * 3))
+    ii = np.arange(30, dtype=np.float32)
+    X, Y = np.meshgrid(ii, ii, indexing="ij")
+    plane = np.stack([X.ravel(), Y.ravel(), np.full(X.size, 5, np.float32)], 1)
+    out.append(("a plane of dots",
+                plane[rng.random(len(plane)) < 0.5], (1.0,) * 3))
+    out.append(("one dot", np.array([[3, 4, 5]], np.float32), (1.0,) * 3))
+    out.append(("two far-apart dots",
+                np.array([[0, 0, 0], [100, -50, 20]], np.float32), (1.0,) * 3))
+    return out
+
+
+def _edge_rays(dots, voxel, rng, n=1200):
+    """Rays through voxel corners, edge and face centres and voxel centres:
+    axis-parallel ones, and ones with one zero direction component."""
+    v = np.asarray(voxel, np.float32)
+    ext = float(np.max(dots.max(0) - dots.min(0) + v))
+    tgt = dots[rng.integers(0, len(dots), n)] + \
+        rng.choice([0.0, 0.5, 1.0], (n, 3)).astype(np.float32) * v
+    d = np.zeros((n, 3), np.float32)
+    half = n // 2
+    ax = rng.integers(0, 3, half)
+    d[np.arange(half), ax] = rng.choice([-1.0, 1.0], half)
+    g = rng.standard_normal((n - half, 3)).astype(np.float32)
+    g[np.arange(n - half), rng.integers(0, 3, n - half)] = 0.0
+    d[half:] = g
+    o = (tgt - d * np.float32(2.0 * ext)).astype(np.float32)
+    return o, d
+
+
+def _cameras(dots, voxel, rng, W, H):
+    from voxvision_amd import camera
+    v = np.asarray(voxel, np.float32)
+    lo, hi = dots.min(0), dots.max(0) + v
+    ctr = (lo + hi) / 2
+    diag = float(np.linalg.norm(hi - lo))
+    fov = np.tan(np.radians(30))
+    u = rng.normal(size=3)
+    u /= np.linalg.norm(u)
+    dot = dots[rng.integers(0, len(dots))] + 0.5 * v
+    return [camera.make_camera(ctr + 1.5 * diag * u, ctr, fov, W, H),
+            camera.make_camera(ctr, ctr + diag * rng.normal(size=3), fov, W, H),
+            camera.make_camera(dot + 4.0 * v * np.array([1.0, 0.7, 0.3]),
+                               dot, fov, W, H)]
+
+
+@pytest.mark.parametrize("k", range(7))
+def test_scaled_offset_and_degenerate_scenes(gpu, oracle, k):
+    from voxvision_amd import scenes, voxrnd, voxtrees as VT
+    name, dots, voxel = _scenes()[k]
+    print("scene:", name, len(dots), "dots", flush=True)
+    rng = np.random.default_rng(100 + k)
+    T = oracle.Tree(dots, voxel)
+    G = VT.make_tree(dots, voxel)
+    assert G.hash() == T.hash(), name
+    bmin, bmax = scenes.bbox_of(dots, voxel)
+
+    o, d = scenes.random_rays(20000, bmin, bmax, seed=200 + k)
+    eo, ed = _edge_rays(dots, voxel, rng)
+    o = np.concatenate([o, eo])
+    d = np.concatenate([d, ed])
+    vox_o, hit_o = T.rays(o, d)
+    vox_g, hit_g = VT.ray_tree_intersection(G, o, d)
+    torch.cuda.synchronize()
+    assert np.array_equal(_u32(vox_g), vox_o), \
+        "%s: voxel ids differ on %d rays" % (name, int((_u32(vox_g) != vox_o).sum()))
+    m = vox_o != MISS
+    assert m.any(), name
+    assert np.array_equal(hit_g.cpu().numpy()[:, m].view(np.uint32),
+                          hit_o[:, m].view(np.uint32)), name
+
+    s = float(min(voxel))
+    c, r = scenes.random_balls(5000, bmin, bmax, seed=300 + k,
+                               rmin=0.25 * s, rmax=3.0 * float(max(voxel)))
+    want = T.balls(c, r)
+    got = VT.tree_ball_collidep(G, c, r).cpu().numpy()
+    assert np.array_equal(got, want), name
+
+    W, H = 96, 64
+    cams = _cameras(dots, voxel, rng, W, H)
+    for quality in (0, 2):
+        out = voxrnd.render(G, cams, W, H, quality=quality, want_colors=False)
+        torch.cuda.synchronize()
+        vg = _u32(out["voxel"]).reshape(len(cams), -1)
+        hg = out["hit"].cpu().numpy().reshape(3, len(cams), -1)
+        for i, cam in enumerate(cams):
+            _, vo, ho, _ = T.render(cam, W, H, quality)
+            assert np.array_equal(vg[i], vo), (name, quality, i)
+            mm = vo != MISS
+            assert np.array_equal(hg[:, i, mm].view(np.uint32),
+                                  ho[:, mm].view(np.uint32)), (name, quality, i)
