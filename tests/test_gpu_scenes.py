@@ -2,10 +2,11 @@
 (half, anisotropic, an inexact 0.001), coordinates far from the origin,
 and degenerate dot sets (a line, a plane, one dot, two far-apart dots).
 Rays include axis-parallel ones through voxel corners and face centres and
-ones with a zero direction component; renders (BEST and ADAPTIVE) take the
-pre-pass with the tree's cut, whose margin scales with the scene.  Same bar
-as the other parity tests: bit-exact voxel ids and hit points, exact ball
-booleans, against the oracle on identical inputs.
+ones with a zero direction component; renders (every quality) take the
+pre-pass with the tree's cut, whose margin scales with the scene; one
+batched launch holds all the scenes at once, each tree with its own voxel
+size.  Same bar as the other parity tests: bit-exact voxel ids and hit
+points, exact ball booleans, against the oracle on identical inputs.
 """
 import numpy as np
 import pytest
@@ -118,7 +119,7 @@ def test_scaled_offset_and_degenerate_scenes(gpu, oracle, k):
 
     W, H = 96, 64
     cams = _cameras(dots, voxel, rng, W, H)
-    for quality in (0, 2):
+    for quality in (0, 1, 2, 2 | 4, 2 | 8):
         out = voxrnd.render(G, cams, W, H, quality=quality, want_colors=False)
         torch.cuda.synchronize()
         vg = _u32(out["voxel"]).reshape(len(cams), -1)
@@ -129,3 +130,41 @@ def test_scaled_offset_and_degenerate_scenes(gpu, oracle, k):
             mm = vo != MISS
             assert np.array_equal(hg[:, i, mm].view(np.uint32),
                                   ho[:, mm].view(np.uint32)), (name, quality, i)
+
+
+def test_batch_of_scaled_scenes(gpu, oracle):
+    """The seven scenes, each with its own voxel size and offset, in one
+    batched launch: every ray's result is the oracle's for its own tree."""
+    from voxvision_amd import batch, scenes, voxtrees as VT
+    sc = _scenes()
+    trees, rays, items, first = [], [], [], 0
+    for k, (name, dots, voxel) in enumerate(sc):
+        rng = np.random.default_rng(500 + k)
+        bmin, bmax = scenes.bbox_of(dots, voxel)
+        o, d = scenes.random_rays(3000 + 1000 * k, bmin, bmax, seed=600 + k)
+        eo, ed = _edge_rays(dots, voxel, rng, n=400)
+        o, d = np.concatenate([o, eo]), np.concatenate([d, ed])
+        t = VT.make_tree(dots, voxel)
+        trees.append(t)
+        rays.append((o, d))
+        items.append((t, first, len(o)))
+        first += len(o)
+    o = np.concatenate([r[0] for r in rays])
+    d = np.concatenate([r[1] for r in rays])
+    B = batch.Batch(items)
+    n = len(o)
+    vox = torch.empty(n, dtype=torch.int32, device="cuda")
+    leaf = torch.empty(n, dtype=torch.int32, device="cuda")
+    hit = torch.full((3, n), float("nan"), device="cuda")
+    B.ray(batch.soa(o), batch.soa(d), vox, leaf, hit)
+    torch.cuda.synchronize()
+    vb = _u32(vox)
+    hb = hit.cpu().numpy()
+    for (name, dots, voxel), (t, f, cnt), (ro, rd) in zip(sc, items, rays):
+        vo, ho = oracle.Tree(dots, voxel).rays(ro, rd)
+        sl = slice(f, f + cnt)
+        assert np.array_equal(vb[sl], vo), name
+        m = vo != MISS
+        assert np.array_equal(hb[:, sl][:, m].view(np.uint32),
+                              ho[:, m].view(np.uint32)), name
+    B.release()
