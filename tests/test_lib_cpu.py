@@ -232,18 +232,27 @@ def test_edit_reference_scripts_match_oracle(oracle):
     _run_script(E, T, [("ins", (6, 6, 6)), ("ins", (0, 0, 0))])
 
 
-@pytest.mark.parametrize("scene", ["sparse", "dense_cube", "blobs"])
+@pytest.mark.parametrize("scene", ["sparse", "dense_cube", "blobs",
+                                   "anisotropic", "tiny_voxel"])
 def test_edit_random_sequences_match_oracle(oracle, scene):
     """Random inserts and deletes (misses, hits, aligned and unaligned
-    points, negative coordinates, a non-unit voxel) on sparse, dense and
-    blob scenes: product and oracle trees are identical after every edit,
-    and vox_rebuild_tree agrees too."""
+    points, negative coordinates, non-unit voxels: 0.5, (1, 0.5, 3) and
+    2^-10) on sparse, dense and blob scenes: product and oracle trees are
+    identical after every edit, and vox_rebuild_tree agrees too."""
     from voxvision_amd import scenes
     from voxvision_amd.edit import EditTree
     O = oracle
-    rng = np.random.default_rng({"sparse": 1, "dense_cube": 2, "blobs": 3}[scene])
+    rng = np.random.default_rng({"sparse": 1, "dense_cube": 2, "blobs": 3,
+                                 "anisotropic": 4, "tiny_voxel": 5}[scene])
     V = (1.0, 1.0, 1.0)
-    if scene == "sparse":
+    if scene in ("anisotropic", "tiny_voxel"):
+        V = (1.0, 0.5, 3.0) if scene == "anisotropic" else (2.0 ** -10,) * 3
+        idx = np.unique(rng.integers(-6, 10, (300, 3)), axis=0)
+        dots = (idx.astype(np.float32) * np.asarray(V, np.float32))
+        lo = -8 * np.asarray(V)
+        hi = 12 * np.asarray(V)
+        n_ops = 500
+    elif scene == "sparse":
         dots = np.unique(rng.integers(0, 16, (400, 3)), axis=0).astype(np.float32)
         lo, hi, n_ops = -2, 18, 800
     elif scene == "dense_cube":
@@ -261,7 +270,7 @@ def test_edit_random_sequences_match_oracle(oracle, scene):
     for _ in range(n_ops):
         p = rng.uniform(lo, hi, 3)
         if rng.random() < 0.3:
-            p = np.floor(p)
+            p = np.floor(p / np.asarray(V)) * np.asarray(V)
         op = "ins" if rng.random() < 0.5 else "del"
         a = E.insert(p) if op == "ins" else E.delete(p)
         b = T.insert(p) if op == "ins" else T.delete(p)
@@ -271,6 +280,45 @@ def test_edit_random_sequences_match_oracle(oracle, scene):
     R = T.rebuild()
     E.rebuild()
     assert E.hash() == R.hash()
+
+
+def test_edit_with_an_inexact_voxel_size_stops_at_a_bad_dense_leaf(oracle):
+    """With a voxel size that is not exact in binary (0.001), a dense leaf's
+    box divided by it can truncate one short of its voxel count
+    (get_dimensions, geom.c:184-196); listing that leaf's voxels is then
+    undefined and the reference stops on an assertion (tree.c:389).  The
+    edit that meets such a leaf fails with VOXG_ERANGE and the tree refuses
+    every later edit, rebuild and commit; until then every step equals the
+    oracle's."""
+    from voxvision_amd import _lib
+    from voxvision_amd.edit import EditTree
+    rng = np.random.default_rng(5)
+    V = (1e-3,) * 3
+    idx = np.unique(rng.integers(-6, 10, (300, 3)), axis=0)
+    dots = idx.astype(np.float32) * np.asarray(V, np.float32)
+    E = EditTree(dots, voxel=V)
+    T = oracle.Tree(dots, voxel=V)
+    lo, hi = -8 * np.asarray(V), 12 * np.asarray(V)
+    stopped = None
+    for step in range(500):
+        p = rng.uniform(lo, hi, 3)
+        if rng.random() < 0.3:
+            p = np.floor(p / np.asarray(V)) * np.asarray(V)
+        ins = rng.random() < 0.5
+        try:
+            a = E.insert(p) if ins else E.delete(p)
+        except _lib.VoxgError as e:
+            assert "dense leaf" in str(e)
+            stopped = step
+            break
+        b = T.insert(p) if ins else T.delete(p)
+        assert a == bool(b)
+        assert E.hash() == T.hash(), step
+    assert stopped is not None
+    for call in (lambda: E.insert((0.0, 0.0, 0.0)),
+                 lambda: E.delete((0.0, 0.0, 0.0)), E.rebuild):
+        with pytest.raises(_lib.VoxgError):
+            call()
 
 
 def test_launch_configs_per_kind():

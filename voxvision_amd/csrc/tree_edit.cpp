@@ -23,6 +23,12 @@ using namespace voxg;
 struct voxg_edit_tree {
     struct vox_node *root = nullptr;
     float V[3] = {1.0f, 1.0f, 1.0f};
+    // set when an edit met a dense leaf whose box, divided by the voxel
+    // size and truncated (geom.c:184-196), does not give its voxel count:
+    // the reference stops there on an assertion (tree.c:389, 530, 671) and
+    // its listing of that leaf's voxels is undefined; every later call on
+    // the tree fails with this message
+    std::string broken;
 };
 
 namespace {
@@ -43,6 +49,7 @@ struct Editor {
     float V[3];
     std::string err;
     bool oom = false;
+    mutable bool bad_dense = false;  // a dense leaf's dimensions disagree with its count
 
     vox_node *alloc_node() {
         auto *n = static_cast<vox_node *>(std::aligned_alloc(16, sizeof(vox_node)));
@@ -152,6 +159,7 @@ struct Editor {
                 }
                 cur[0] += V[0];
             }
+            if (count != t->dots_num) bad_dense = true;  // tree.c:389
             return count;
         }
         for (int i = 0; i < 8; i++) {
@@ -237,6 +245,8 @@ struct Editor {
                 } else {
                     node = dense_leaf(t->bounding_box);
                     if (!node) return;
+                    if (node->dots_num != t->dots_num + 1)  // tree.c:530
+                        bad_dense = true;
                     destroy_reference(t);
                 }
                 *slot = node;
@@ -308,6 +318,9 @@ struct Editor {
         vox_node *c = dense_leaf(bb);
         n->data.inner.children[0] = c;
         if (!c || !n->data.inner.children[1 << idx]) return nullptr;
+        if (c->dots_num + n->data.inner.children[1 << idx]->dots_num !=
+            t->dots_num)  // tree.c:671
+            bad_dense = true;
         c->dots_num--;
         c->bounding_box.min[idx] += V[idx];
         destroy_reference(t);
@@ -372,7 +385,7 @@ struct Editor {
                         return;
                     }
                     if (oom) return;
-                    vox_dot set[kMaxDots];
+                    vox_dot set[kMaxDots] = {};
                     flatten(t, set);
                     unsigned i = 0;
                     while (i < t->dots_num && !equal3(set[i], v)) i++;
@@ -439,9 +452,22 @@ int voxg_edit_create(const float *xyz, size_t n, size_t stride,
     return 0;
 }
 
+// A dense leaf whose dimensions disagree with its voxel count (possible
+// when the voxel size is not exact in binary, so that box extents divided
+// by it truncate one short): the reference asserts; here the edit that met
+// it fails, and so does every later call on the tree.
+static int broken_dense(voxg_edit_tree *t) {
+    t->broken =
+        "an edit met a dense leaf whose box does not give its voxel count "
+        "(voxel size not exact in binary floating point); the tree is no "
+        "longer usable: rebuild it from its dots";
+    return fail_edit(t->broken, VOXG_ERANGE);
+}
+
 static int edit_op(voxg_edit_tree *t, const float xyz[3], int insert,
                    int *changed) {
     if (!t || !xyz) return fail_edit("NULL argument", VOXG_EINVAL);
+    if (!t->broken.empty()) return fail_edit(t->broken, VOXG_ERANGE);
     Editor e;
     for (int l = 0; l < 3; l++) e.V[l] = t->V[l];
     float v[3] = {xyz[0], xyz[1], xyz[2]};
@@ -454,6 +480,7 @@ static int edit_op(voxg_edit_tree *t, const float xyz[3], int insert,
         if (e.oom)
             return fail_edit(e.err.empty() ? "host allocation failed" : e.err,
                              VOXG_ENOMEM);
+        if (e.bad_dense) return broken_dense(t);
     }
     if (changed) *changed = res;
     return 0;
@@ -469,11 +496,13 @@ int voxg_edit_delete(voxg_edit_tree *t, const float xyz[3], int *deleted) {
 
 int voxg_edit_rebuild(voxg_edit_tree *t) {
     if (!t) return fail_edit("NULL argument", VOXG_EINVAL);
+    if (!t->broken.empty()) return fail_edit(t->broken, VOXG_ERANGE);
     if (!t->root) return 0;
     Editor e;
     for (int l = 0; l < 3; l++) e.V[l] = t->V[l];
     DotBuf set(t->root->dots_num);
     e.flatten(t->root, set.data());
+    if (e.bad_dense) return broken_dense(t);
     vox_node *m = e.make(set.data(), t->root->dots_num);
     if (e.oom) return fail_edit(e.err, VOXG_ENOMEM);
     destroy_reference(t->root);
@@ -491,6 +520,7 @@ size_t voxg_edit_voxels(const voxg_edit_tree *t) {
 
 int voxg_edit_commit(const voxg_edit_tree *t, voxg_tree **out) {
     if (!t) return fail_edit("NULL argument", VOXG_EINVAL);
+    if (!t->broken.empty()) return fail_edit(t->broken, VOXG_ERANGE);
     return voxg_tree_upload(t->root, t->V, out);
 }
 
