@@ -168,3 +168,62 @@ def test_batch_of_scaled_scenes(gpu, oracle):
         assert np.array_equal(hb[:, sl][:, m].view(np.uint32),
                               ho[:, m].view(np.uint32)), name
     B.release()
+
+
+@pytest.mark.parametrize("voxel,offset", [((1.0, 0.5, 3.0), (-6e4, 17, 2502)),
+                                          ((2.0 ** -10,) * 3, (0, 0, 0))])
+def test_edited_scaled_tree_answers_like_the_oracle(gpu, oracle, voxel,
+                                                    offset):
+    """Inserts and deletes on a blob scene with a non-unit voxel size (and
+    far-off coordinates), committed to the device: rays, balls and renders
+    answer like the oracle's tree after the same edits."""
+    from voxvision_amd import scenes, voxrnd, voxtrees as VT
+    from voxvision_amd.edit import EditTree
+    V = np.asarray(voxel, np.float32)
+    v = scenes.value_noise(24, 5)
+    dots = scenes.volume_to_dots(v > np.quantile(v, 0.6), voxel) + \
+        np.asarray(offset, np.float32)
+    E = EditTree(dots, voxel=voxel)
+    T = oracle.Tree(dots, voxel)
+    rng = np.random.default_rng(9)
+    lo, hi = dots.min(0) - 2 * V, dots.max(0) + 3 * V
+    for _ in range(600):
+        p = rng.uniform(lo, hi).astype(np.float32)
+        if rng.random() < 0.5:
+            p = dots[rng.integers(0, len(dots))] + \
+                V * rng.integers(-1, 2, 3).astype(np.float32)
+        ins = rng.random() < 0.5
+        a = E.insert(p) if ins else E.delete(p)
+        b = T.insert(p) if ins else T.delete(p)
+        assert a == bool(b)
+    assert E.hash() == T.hash()
+    G = E.commit()
+    assert G.hash() == T.hash()
+    bmin, bmax = lo, hi
+    o, d = scenes.random_rays(30000, bmin, bmax, seed=10)
+    vox_o, hit_o = T.rays(o, d)
+    vox_g, hit_g = VT.ray_tree_intersection(G, o, d)
+    torch.cuda.synchronize()
+    assert np.array_equal(_u32(vox_g), vox_o)
+    m = vox_o != MISS
+    assert m.any()
+    assert np.array_equal(hit_g.cpu().numpy()[:, m].view(np.uint32),
+                          hit_o[:, m].view(np.uint32))
+    c, r = scenes.random_balls(5000, bmin, bmax, seed=11,
+                               rmin=0.25 * float(V.min()),
+                               rmax=3.0 * float(V.max()))
+    assert np.array_equal(VT.tree_ball_collidep(G, c, r).cpu().numpy(),
+                          T.balls(c, r))
+    W, H = 96, 64
+    cams = _cameras(dots, voxel, rng, W, H)
+    for quality in (0, 2):
+        out = voxrnd.render(G, cams, W, H, quality=quality, want_colors=False)
+        torch.cuda.synchronize()
+        vg = _u32(out["voxel"]).reshape(len(cams), -1)
+        hg = out["hit"].cpu().numpy().reshape(3, len(cams), -1)
+        for i, cam in enumerate(cams):
+            _, vo, ho, _ = T.render(cam, W, H, quality)
+            assert np.array_equal(vg[i], vo), (quality, i)
+            mm = vo != MISS
+            assert np.array_equal(hg[:, i, mm].view(np.uint32),
+                                  ho[:, mm].view(np.uint32)), (quality, i)

@@ -321,6 +321,27 @@ def test_edit_with_an_inexact_voxel_size_stops_at_a_bad_dense_leaf(oracle):
             call()
 
 
+@pytest.mark.parametrize("n", [2, 3])
+def test_edit_delete_off_the_dense_grid_is_refused(n):
+    """Dots that are not multiples of the voxel size (a cube of n^3 at
+    offset 0.5 makes one dense leaf): deleting an aligned point inside its
+    box, which lies on none of its voxels, is refused with VOXG_EINVAL and
+    leaves the tree unchanged (the reference asserts, tree.c:808, or
+    divides the box off its grid).  Deleting one of its own voxels works."""
+    from voxvision_amd import _lib
+    from voxvision_amd.edit import EditTree
+    g = np.arange(n, dtype=np.float32) + np.float32(0.5)
+    dots = np.stack(np.meshgrid(g, g, g, indexing="ij"), -1).reshape(-1, 3)
+    E = EditTree(dots)
+    before = E.hash()
+    with pytest.raises(_lib.VoxgError, match="off its voxel grid"):
+        E.delete((1.2, 1.2, 1.2))
+    assert E.hash() == before and E.voxels == n ** 3
+    E2 = EditTree(dots)
+    assert not E2.delete((0.5, 0.5, 0.5))   # aligns to (0, 0, 0): outside
+    assert E2.voxels == n ** 3
+
+
 def test_launch_configs_per_kind():
     """Each kind of launch keeps a configuration of its own; setting all
     kinds at once, one kind, and the restore context manager; invalid

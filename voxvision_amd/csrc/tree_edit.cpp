@@ -177,21 +177,41 @@ struct Editor {
         return root;
     }
 
-    // tree.c:437-458
-    bool contains(const vox_node *t, const float *v) const {
+    // tree.c:437-458 (voxel_in_tree): the leaf or dense leaf that holds
+    // v, or NULL when v is not in the tree
+    const vox_node *holder(const vox_node *t, const float *v) const {
         while (t) {
             for (int l = 0; l < 3; l++)
                 if (v[l] < t->bounding_box.min[l] || v[l] >= t->bounding_box.max[l])
-                    return false;
-            if (t->flags & VOX_DENSE_LEAF) return true;
+                    return nullptr;
+            if (t->flags & VOX_DENSE_LEAF) return t;
             if (t->flags & VOX_LEAF) {
                 for (unsigned i = 0; i < t->dots_num; i++)
-                    if (equal3(v, t->data.dots[i])) return true;
-                return false;
+                    if (equal3(v, t->data.dots[i])) return t;
+                return nullptr;
             }
             t = t->data.inner.children[sub_less(t->data.inner.center, v)];
         }
-        return false;
+        return nullptr;
+    }
+
+    // v is on a dense leaf's voxel grid: along each axis its minimum plus
+    // the voxel size added some number of times, as the leaf's voxels are
+    // listed (tree.c:364-420).  A dense leaf holds every point of its box,
+    // so a point off that grid passes voxel_in_tree; the reference's
+    // deletion then finds no voxel to remove (an assertion, tree.c:808) or
+    // divides the box off its grid (tree.c:683-745).
+    bool on_grid(const vox_node *t, const float *v) const {
+        for (int l = 0; l < 3; l++) {
+            float cur = t->bounding_box.min[l];
+            while (cur < v[l]) {
+                const float next = cur + V[l];
+                if (!(next > cur)) return false;
+                cur = next;
+            }
+            if (cur != v[l]) return false;
+        }
+        return true;
     }
 
     // a leaf or small dense leaf remade with one more voxel
@@ -389,6 +409,10 @@ struct Editor {
                     flatten(t, set);
                     unsigned i = 0;
                     while (i < t->dots_num && !equal3(set[i], v)) i++;
+                    if (i == t->dots_num) {  // not listed: see on_grid
+                        bad_dense = true;
+                        return;
+                    }
                     std::memmove(set + i, set + i + 1,
                                  sizeof(vox_dot) * (t->dots_num - i - 1));
                     vox_node *m = make(set, t->dots_num - 1);
@@ -458,9 +482,10 @@ int voxg_edit_create(const float *xyz, size_t n, size_t stride,
 // it fails, and so does every later call on the tree.
 static int broken_dense(voxg_edit_tree *t) {
     t->broken =
-        "an edit met a dense leaf whose box does not give its voxel count "
-        "(voxel size not exact in binary floating point); the tree is no "
-        "longer usable: rebuild it from its dots";
+        "an edit met a dense leaf whose box does not give its voxel count, "
+        "or that does not list the voxel (a voxel size not exact in binary "
+        "floating point); the tree is no longer usable: rebuild it from its "
+        "dots";
     return fail_edit(t->broken, VOXG_ERANGE);
 }
 
@@ -472,7 +497,13 @@ static int edit_op(voxg_edit_tree *t, const float xyz[3], int insert,
     for (int l = 0; l < 3; l++) e.V[l] = t->V[l];
     float v[3] = {xyz[0], xyz[1], xyz[2]};
     e.align(v);
-    const bool present = e.contains(t->root, v);
+    const vox_node *h = e.holder(t->root, v);
+    const bool present = h != nullptr;
+    if (!insert && h && (h->flags & VOX_DENSE_LEAF) && !e.on_grid(h, v))
+        return fail_edit("the voxel lies in a dense leaf but off its voxel "
+                         "grid (the tree's dots are not multiples of the "
+                         "voxel size); the tree is unchanged",
+                         VOXG_EINVAL);
     int res = insert ? !present : present;
     if (res) {
         if (insert) e.insert_at(&t->root, v);
