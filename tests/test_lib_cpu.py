@@ -342,6 +342,75 @@ def test_edit_delete_off_the_dense_grid_is_refused(n):
     assert E2.voxels == n ** 3
 
 
+def _edit_fuzz_case(seed):
+    """A random scene and 1500 random edits with a rebuild every 300: voxel
+    sizes exact and inexact in binary, dense blocks, planes, offsets up to
+    -6e4 (where 2^-10 is below the coordinates' precision)."""
+    rng = np.random.default_rng(seed)
+    choices = [(1.0,) * 3, (0.5,) * 3, (1.0, 0.5, 3.0), (1e-3,) * 3,
+               (0.1, 0.3, 0.7), (2.0 ** -10,) * 3]
+    V = np.asarray(choices[seed % 6], np.float32)
+    kind = (seed // 6) % 4
+    n = int(rng.integers(1, 400))
+    if kind == 1:
+        a = rng.integers(0, 4, 3)
+        b = a + rng.integers(1, 6, 3)
+        idx = np.stack(np.meshgrid(*[np.arange(a[i], b[i]) for i in range(3)],
+                                   indexing="ij"), -1).reshape(-1, 3)
+    else:
+        r = (-3, 6) if kind == 3 else (-5, 12)
+        idx = np.unique(rng.integers(r[0], r[1], (n, 3)), axis=0)
+        if kind == 2:
+            idx[:, rng.integers(0, 3)] = 2
+    off = (rng.choice([0.0, 0.5, 0.25, 1000.0, -6e4], 3)
+           if rng.random() < 0.5 else np.zeros(3))
+    dots = (idx.astype(np.float32) * V + off.astype(np.float32))
+    lo, hi = dots.min(0) - 3 * V, dots.max(0) + 4 * V
+    ops = []
+    for i in range(1500):
+        if rng.random() < 0.5:
+            p = dots[rng.integers(0, len(dots))] + V * rng.integers(-2, 3, 3)
+        else:
+            p = rng.uniform(lo, hi)
+        ops.append(("ins" if rng.random() < 0.5 else "del",
+                    p.astype(np.float32)))
+        if i % 300 == 299:
+            ops.append(("rebuild", None))
+    return tuple(float(x) for x in V), dots.astype(np.float32), ops
+
+
+@pytest.mark.parametrize("seed", [11, 35, 425, 473, 0, 7, 100])
+def test_edit_sequences_never_corrupt_memory(seed):
+    """Edit sequences that once overran a flattening buffer (a dense leaf
+    whose truncated dimensions multiply to more than its count: seeds 11,
+    35) or recursed without end (a new voxel that does not separate from a
+    dense leaf: 425, 473) now end in VOXG_ERANGE, after which the tree
+    refuses every call; the others run through.  A crash here is the
+    failure."""
+    from voxvision_amd import _lib
+    from voxvision_amd.edit import EditTree
+    V, dots, ops = _edit_fuzz_case(seed)
+    E = EditTree(dots, voxel=V)
+    failed = False
+    for op, p in ops:
+        try:
+            if op == "ins":
+                E.insert(p)
+            elif op == "del":
+                E.delete(p)
+            else:
+                E.rebuild()
+        except _lib.VoxgError as e:
+            if "off its voxel grid" in str(e):
+                continue   # refused, tree unchanged
+            failed = True
+            with pytest.raises(_lib.VoxgError):
+                E.insert(dots[0])
+            break
+    if seed in (11, 35, 425, 473):
+        assert failed
+
+
 def test_launch_configs_per_kind():
     """Each kind of launch keeps a configuration of its own; setting all
     kinds at once, one kind, and the restore context manager; invalid

@@ -49,7 +49,7 @@ struct Editor {
     float V[3];
     std::string err;
     bool oom = false;
-    mutable bool bad_dense = false;  // a dense leaf's dimensions disagree with its count
+    mutable bool bad_dense = false;  // a dense leaf its box does not describe
 
     vox_node *alloc_node() {
         auto *n = static_cast<vox_node *>(std::aligned_alloc(16, sizeof(vox_node)));
@@ -130,10 +130,18 @@ struct Editor {
         return n;
     }
 
-    // tree.c:364-420: dense voxels by repeated addition from the minimum
-    size_t flatten(const vox_node *t, vox_dot *set) const {
+    // tree.c:364-420: dense voxels by repeated addition from the minimum.
+    // Writes at most cap entries.  A dense leaf whose truncated dimensions
+    // (dims) do not multiply to its voxel count -- the reference asserts
+    // (tree.c:389) -- and a subtree with more voxels than cap set
+    // bad_dense instead, and what is listed then is not used.
+    size_t flatten(const vox_node *t, vox_dot *set, size_t cap) const {
         size_t count = 0;
         if (!t) return 0;
+        if (t->dots_num > cap) {
+            bad_dense = true;
+            return 0;
+        }
         if (t->flags & VOX_LEAF) {
             std::memcpy(set, t->data.dots, t->dots_num * sizeof(vox_dot));
             return t->dots_num;
@@ -141,6 +149,11 @@ struct Editor {
         if (t->flags & VOX_DENSE_LEAF) {
             size_t d[3];
             dims(t->bounding_box, d);
+            if ((double)d[0] * (double)d[1] * (double)d[2] !=
+                (double)t->dots_num) {
+                bad_dense = true;
+                return 0;
+            }
             float cur[3];
             cur[0] = t->bounding_box.min[0];
             for (size_t i = 0; i < d[0]; i++) {
@@ -159,11 +172,10 @@ struct Editor {
                 }
                 cur[0] += V[0];
             }
-            if (count != t->dots_num) bad_dense = true;  // tree.c:389
             return count;
         }
         for (int i = 0; i < 8; i++) {
-            size_t c = flatten(t->data.inner.children[i], set);
+            size_t c = flatten(t->data.inner.children[i], set, cap - count);
             set += c;
             count += c;
         }
@@ -217,7 +229,7 @@ struct Editor {
     // a leaf or small dense leaf remade with one more voxel
     vox_node *remake_with(const vox_node *t, const float *v) {
         DotBuf set(t->dots_num + 1);
-        flatten(t, set.data());
+        flatten(t, set.data(), t->dots_num);
         set.data()[t->dots_num][0] = v[0];
         set.data()[t->dots_num][1] = v[1];
         set.data()[t->dots_num][2] = v[2];
@@ -227,14 +239,7 @@ struct Editor {
 
     // tree.c:461-501
     vox_node *insert_in_big_dense(vox_node *t, const float *v) {
-        vox_node *n = alloc_node();
-        if (!n) return nullptr;
-        n->bounding_box = t->bounding_box;
-        grow(n->bounding_box, v);
-        n->flags = 0;
-        n->dots_num = t->dots_num + 1;
-        float *c = n->data.inner.center;
-        float mid[3];
+        float c[3], mid[3];
         for (int l = 0; l < 3; l++) {
             // geom-sse.c:51-62 closest_vertex
             float d1 = std::fabs(t->bounding_box.min[l] - v[l]);
@@ -243,6 +248,20 @@ struct Editor {
             mid[l] = (t->bounding_box.min[l] + t->bounding_box.max[l]) / 2.0f;
         }
         int i1 = sub_sign(c, mid), i2 = sub_sign(c, v);
+        if (i1 == i2) {
+            // the new voxel falls on the leaf's own side of every plane
+            // (coordinates too coarse for the voxel size): the reference
+            // recurses into the same leaf without end
+            bad_dense = true;
+            return nullptr;
+        }
+        vox_node *n = alloc_node();
+        if (!n) return nullptr;
+        n->bounding_box = t->bounding_box;
+        grow(n->bounding_box, v);
+        n->flags = 0;
+        n->dots_num = t->dots_num + 1;
+        for (int l = 0; l < 3; l++) n->data.inner.center[l] = c[l];
         n->data.inner.children[i1] = t;
         insert_at(&n->data.inner.children[i2], v);
         return n;
@@ -406,7 +425,7 @@ struct Editor {
                     }
                     if (oom) return;
                     vox_dot set[kMaxDots] = {};
-                    flatten(t, set);
+                    flatten(t, set, kMaxDots);
                     unsigned i = 0;
                     while (i < t->dots_num && !equal3(set[i], v)) i++;
                     if (i == t->dots_num) {  // not listed: see on_grid
@@ -482,9 +501,11 @@ int voxg_edit_create(const float *xyz, size_t n, size_t stride,
 // it fails, and so does every later call on the tree.
 static int broken_dense(voxg_edit_tree *t) {
     t->broken =
-        "an edit met a dense leaf whose box does not give its voxel count, "
-        "or that does not list the voxel (a voxel size not exact in binary "
-        "floating point); the tree is no longer usable: rebuild it from its "
+        "an edit met a dense leaf that its box and the voxel size do not "
+        "describe exactly (its box does not give its voxel count, it does "
+        "not list the voxel, or a new voxel does not separate from it: a "
+        "voxel size not exact in binary floating point, or coordinates too "
+        "large for it); the tree is no longer usable: rebuild it from its "
         "dots";
     return fail_edit(t->broken, VOXG_ERANGE);
 }
@@ -532,7 +553,7 @@ int voxg_edit_rebuild(voxg_edit_tree *t) {
     Editor e;
     for (int l = 0; l < 3; l++) e.V[l] = t->V[l];
     DotBuf set(t->root->dots_num);
-    e.flatten(t->root, set.data());
+    e.flatten(t->root, set.data(), t->root->dots_num);
     if (e.bad_dense) return broken_dense(t);
     vox_node *m = e.make(set.data(), t->root->dots_num);
     if (e.oom) return fail_edit(e.err, VOXG_ENOMEM);
