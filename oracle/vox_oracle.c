@@ -10,6 +10,7 @@
 #define _GNU_SOURCE
 #include "vox_oracle.h"
 
+#include <limits.h>
 #include <math.h>
 #include <pthread.h>
 #include <sched.h>
@@ -728,7 +729,18 @@ int orc_tree_ball_collidep(const struct orc_node *tree, const orc_dot center,
     return ball(tree, center, radius, &c);
 }
 
-/* Canonical voxel id of a search result. */
+/* An integral float to int, saturating: NaN gives 0, values beyond the int
+ * range its bounds (a plain C conversion of those is undefined). */
+static int sat_int(float f)
+{
+    if (f >= 2147483648.0f) return INT_MAX;
+    if (f >= -2147483648.0f) return (int)f;
+    return f != f ? 0 : INT_MIN;
+}
+
+/* Canonical voxel id of a search result: for a dense leaf the voxel that
+ * holds the hit point, its index clamped to the leaf's dimensions (also
+ * for a hit point that is not finite). */
 static uint32_t voxel_id_of(const struct orc_node *leaf, const orc_dot res,
                             int slot)
 {
@@ -740,7 +752,7 @@ static uint32_t voxel_id_of(const struct orc_node *leaf, const orc_dot res,
         for (l = 0; l < 3; l++) {
             float f = floorf((res[l] - leaf->bounding_box.min[l]) /
                              orc_voxel[l]);
-            int v = (int)f;
+            int v = sat_int(f);
             if (v < 0) v = 0;
             if (v > (int)dim[l] - 1) v = (int)dim[l] - 1;
             ix[l] = (uint32_t)v;

@@ -227,3 +227,45 @@ def test_edited_scaled_tree_answers_like_the_oracle(gpu, oracle, voxel,
             mm = vo != MISS
             assert np.array_equal(hg[:, i, mm].view(np.uint32),
                                   ho[:, mm].view(np.uint32)), (quality, i)
+
+
+def test_non_finite_and_zero_rays_match_the_oracle(gpu, oracle):
+    """Rays with NaN, infinite, zero, subnormal and huge components in the
+    origin or the direction (and all-zero directions) on the 64^3 blob
+    scene: hit or miss and voxel ids as the oracle's, hit points bit for bit
+    where they are numbers (NaN payloads differ between x86 and the GPU, so
+    a NaN coordinate only has to be NaN on both)."""
+    from voxvision_amd import scenes, voxtrees as VT
+    c1 = scenes.config_c1()
+    T = oracle.Tree(c1["dots"])
+    G = VT.make_tree(c1["dots"])
+    bmin, bmax = c1["bbox"]
+    rng = np.random.default_rng(3)
+    o, d = scenes.random_rays(8000, bmin, bmax, seed=21)
+    o, d = o.copy(), d.copy()
+    specials = np.array([np.nan, np.inf, -np.inf, 0.0, -0.0, 1e-45, 3e38,
+                         -3e38], np.float32)
+    for i in range(len(o)):
+        m = i % 8
+        if m == 0:
+            d[i, rng.integers(0, 3)] = rng.choice(specials)
+        elif m == 1:
+            o[i, rng.integers(0, 3)] = rng.choice(specials)
+        elif m == 2:
+            d[i] = 0.0
+        elif m == 3:
+            d[i] = rng.choice(specials, 3)
+        elif m == 4:
+            o[i] = rng.choice(specials, 3)
+    vox_o, hit_o = T.rays(o, d)
+    vox_g, hit_g = VT.ray_tree_intersection(G, o, d)
+    torch.cuda.synchronize()
+    assert np.array_equal(_u32(vox_g), vox_o), \
+        "voxel ids differ on %d rays" % int((_u32(vox_g) != vox_o).sum())
+    m = vox_o != MISS
+    hg, ho = hit_g.cpu().numpy()[:, m], hit_o[:, m]
+    nan_g, nan_o = np.isnan(hg), np.isnan(ho)
+    assert np.array_equal(nan_g, nan_o)
+    assert np.array_equal(hg[~nan_g].view(np.uint32),
+                          ho[~nan_o].view(np.uint32))
+    assert nan_o.any() and (~nan_o).all(0).any()
