@@ -269,3 +269,30 @@ def test_non_finite_and_zero_rays_match_the_oracle(gpu, oracle):
     assert np.array_equal(hg[~nan_g].view(np.uint32),
                           ho[~nan_o].view(np.uint32))
     assert nan_o.any() and (~nan_o).all(0).any()
+
+
+def test_non_finite_and_negative_balls_match_the_oracle(gpu, oracle):
+    """Ball queries with NaN, infinite, zero, subnormal and huge centre
+    components and radii (negative ones too) answer as the oracle's."""
+    from voxvision_amd import scenes, voxtrees as VT
+    c1 = scenes.config_c1()
+    T = oracle.Tree(c1["dots"])
+    G = VT.make_tree(c1["dots"])
+    rng = np.random.default_rng(4)
+    c, r = scenes.random_balls(8000, *c1["bbox"], seed=22)
+    c, r = c.copy(), r.copy()
+    specials = np.array([np.nan, np.inf, -np.inf, 0.0, -0.0, 1e-45, 3e38,
+                         -3e38, -2.0], np.float32)
+    for i in range(len(c)):
+        m = i % 4
+        if m == 0:
+            c[i, rng.integers(0, 3)] = rng.choice(specials)
+        elif m == 1:
+            r[i] = rng.choice(specials)
+        elif m == 2:
+            c[i] = rng.choice(specials, 3)
+            r[i] = rng.choice(specials)
+    want = T.balls(c, r)
+    got = VT.tree_ball_collidep(G, c, r).cpu().numpy()
+    assert np.array_equal(got, want), int((got != want).sum())
+    assert 0 < want.mean() < 1
