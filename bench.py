@@ -368,6 +368,7 @@ def serving_leg(log, seconds, rate, seed=31):
         """Submit the plan (at its arrival times when paced, else as fast
         as the result buffers come back) and collect every result."""
         tickets = queue.Queue()
+        failure = []
 
         def collect(out):
             while True:
@@ -375,22 +376,38 @@ def serving_leg(log, seconds, rate, seed=31):
                 if item is None:
                     return
                 tk, kind, px, buf, sz = item
-                info = S.wait(tk, 120.0)
+                try:
+                    info = S.wait(tk, 120.0)
+                except Exception as e:  # a request that did not complete
+                    failure.append("%s request: %s" % (kind, e))
+                    return
                 if buf is not None:
                     pools[sz].put(buf)
                 out.append((kind, px, info))
+
+        def buffer(sz):
+            while True:
+                try:
+                    return pools[sz].get(timeout=1.0)
+                except queue.Empty:
+                    if failure:
+                        return None
 
         done = []
         th = threading.Thread(target=collect, args=(done,))
         th.start()
         start = time.perf_counter()
         for item in plan:
+            if failure:
+                break
             delay = item[0] - (time.perf_counter() - start)
             if paced and delay > 0:
                 time.sleep(delay)
             if item[1] == "r":
                 _, _, tid, cam, sz, q = item
-                buf = pools[sz].get()
+                buf = buffer(sz)
+                if buf is None:
+                    break
                 tk, _ = S.submit_render(tid, cam, sz[0], sz[1], q, out=buf)
                 tickets.put((tk, "r", sz[0] * sz[1], buf, sz))
             else:
@@ -399,6 +416,8 @@ def serving_leg(log, seconds, rate, seed=31):
                 tickets.put((tk, "b", n, None, None))
         tickets.put(None)
         th.join()
+        if failure:
+            raise RuntimeError(failure[0])
         lat = np.array([i["t_done"] - i["t_submit"] for _, _, i in done]) * 1e3
         lr = np.array([i["t_done"] - i["t_submit"] for k, _, i in done
                        if k == "r"]) * 1e3
@@ -446,11 +465,17 @@ def serving_leg(log, seconds, rate, seed=31):
     # stream has grown its buffers for the largest frames before the paced
     # run measures latency (growing one waits for the stream: a cold
     # server's first half second has a p99 of about 22 ms)
-    sat = run(paced=False)
-    st = S.stats()
-    sat["peak_groups_inflight"] = st["peak_groups_inflight"]
-    sat["peak_inflight"] = st["peak_inflight"]
-    res = run(paced=True)
+    try:
+        sat = run(paced=False)
+        st = S.stats()
+        sat["peak_groups_inflight"] = st["peak_groups_inflight"]
+        sat["peak_inflight"] = st["peak_inflight"]
+        res = run(paced=True)
+    except RuntimeError as e:
+        # recorded, not hidden: the server is left as it is (closing it
+        # would wait for the request that did not complete)
+        log("[bench] serving leg failed: %s" % e)
+        return {"error": str(e)}
     res["offered_rate_per_s"] = rate
     res["groups"] = S.stats()["groups"] - st["groups"]
     res["saturation"] = sat
@@ -788,6 +813,11 @@ def main():
     line["config"]["launch"] = (tuned["config"] if tuned
                                 else default_cfgs["render_best"])
     print(json.dumps(line), flush=True)
+    if "error" in extra.get("serving", {}):
+        # a serving request never completed: end without the runtime's
+        # teardown, which would wait for it
+        sys.stderr.flush()
+        os._exit(1)
 
 
 if __name__ == "__main__":

@@ -729,18 +729,10 @@ int orc_tree_ball_collidep(const struct orc_node *tree, const orc_dot center,
     return ball(tree, center, radius, &c);
 }
 
-/* An integral float to int, saturating: NaN gives 0, values beyond the int
- * range its bounds (a plain C conversion of those is undefined). */
-static int sat_int(float f)
-{
-    if (f >= 2147483648.0f) return INT_MAX;
-    if (f >= -2147483648.0f) return (int)f;
-    return f != f ? 0 : INT_MIN;
-}
-
 /* Canonical voxel id of a search result: for a dense leaf the voxel that
- * holds the hit point, its index clamped to the leaf's dimensions (also
- * for a hit point that is not finite). */
+ * holds the hit point, its index clamped to the leaf's dimensions on the
+ * float, before the conversion (NaN goes to 0, +-inf to the last or first
+ * voxel: a plain conversion of those is undefined). */
 static uint32_t voxel_id_of(const struct orc_node *leaf, const orc_dot res,
                             int slot)
 {
@@ -752,10 +744,8 @@ static uint32_t voxel_id_of(const struct orc_node *leaf, const orc_dot res,
         for (l = 0; l < 3; l++) {
             float f = floorf((res[l] - leaf->bounding_box.min[l]) /
                              orc_voxel[l]);
-            int v = sat_int(f);
-            if (v < 0) v = 0;
-            if (v > (int)dim[l] - 1) v = (int)dim[l] - 1;
-            ix[l] = (uint32_t)v;
+            ix[l] = (uint32_t)(int)fminf(fmaxf(f, 0.0f),
+                                         (float)((int)dim[l] - 1));
         }
         return node_hdr(leaf)->id_base + (ix[0] * dim[1] + ix[1]) * dim[2] +
                ix[2];

@@ -263,17 +263,11 @@ __device__ __forceinline__ Node load_node(const NodeRec *__restrict__ nodes,
     return n;
 }
 
-// An integral float to int, saturating: NaN gives 0, values beyond the
-// int range its bounds (a plain conversion of those is undefined).
-__device__ __forceinline__ int sat_int(float f) {
-    return f >= 2147483648.0f ? INT_MAX
-                              : (f >= -2147483648.0f ? (int)f
-                                                     : (f != f ? 0 : INT_MIN));
-}
-
 // Voxel id of a dense-leaf hit: floor((res - min)/V) clamped to the leaf's
-// dimensions (get_dimensions, geom.c:192-196); a hit point that is not
-// finite (rays with infinite or NaN components) gives a clamped index too.
+// dimensions (get_dimensions, geom.c:192-196).  The clamp is taken on the
+// float, before the conversion, so that a hit point that is not finite
+// (rays with infinite or NaN components) gives a clamped index too: NaN
+// goes to 0 (maxNum), +-inf to the last or first voxel.
 // The dimensions are precomputed at flattening and stored in the record's
 // centre slot, which dense leaves do not use.
 __device__ __forceinline__ uint32_t dense_voxel(const Node &n, f3 r,
@@ -281,12 +275,12 @@ __device__ __forceinline__ uint32_t dense_voxel(const Node &n, f3 r,
     int dx = (int)__float_as_uint(n.c.x);
     int dy = (int)__float_as_uint(n.c.y);
     int dz = (int)__float_as_uint(n.c.z);
-    int ix = sat_int(floorf(div_v(r.x - n.bmin.x, t.vx, t.ivx)));
-    int iy = sat_int(floorf(div_v(r.y - n.bmin.y, t.vy, t.ivy)));
-    int iz = sat_int(floorf(div_v(r.z - n.bmin.z, t.vz, t.ivz)));
-    ix = ix < 0 ? 0 : (ix > dx - 1 ? dx - 1 : ix);
-    iy = iy < 0 ? 0 : (iy > dy - 1 ? dy - 1 : iy);
-    iz = iz < 0 ? 0 : (iz > dz - 1 ? dz - 1 : iz);
+    auto index = [](float q, int dim) {
+        return (int)fminf(fmaxf(floorf(q), 0.0f), (float)(dim - 1));
+    };
+    int ix = index(div_v(r.x - n.bmin.x, t.vx, t.ivx), dx);
+    int iy = index(div_v(r.y - n.bmin.y, t.vy, t.ivy), dy);
+    int iz = index(div_v(r.z - n.bmin.z, t.vz, t.ivz), dz);
     return n.ref + (uint32_t)((ix * dy + iy) * dz + iz);
 }
 
