@@ -343,3 +343,25 @@ def test_extreme_cameras_match_the_oracle(gpu, oracle, quality):
         assert mm.any(), i
         assert np.array_equal(hg[:, i, mm].view(np.uint32),
                               ho[:, mm].view(np.uint32)), i
+
+
+def test_voxel_coords_on_coarse_coordinates(gpu):
+    """Dots 6e4 from the origin with a voxel of 2^-10, below their
+    precision, and dots with NaN and infinite components: the tree builds,
+    and every voxel id maps to coordinates (a dense leaf whose truncated
+    dimensions hold no voxel once made the lookup divide by zero)."""
+    from voxvision_amd import voxtrees as VT
+    rng = np.random.default_rng(8)
+    V = 2.0 ** -10
+    idx = rng.integers(0, 4, (600, 3)).astype(np.float32)
+    cases = [idx * np.float32(V) - np.float32(6e4)]
+    bad = idx.copy()
+    bad[rng.integers(0, 600, 12), rng.integers(0, 3, 12)] = np.nan
+    bad[rng.integers(0, 600, 12), rng.integers(0, 3, 12)] = np.inf
+    cases.append(bad)
+    for dots in cases:
+        G = VT.make_tree(dots, (V, V, V) if dots is cases[0] else (1, 1, 1))
+        n = int(G.info["voxel_ids"])
+        xyz = G.voxel_coords(np.arange(n + 2, dtype=np.uint32))
+        assert xyz.shape == (n + 2, 3)
+        assert np.isnan(xyz[n:]).all()

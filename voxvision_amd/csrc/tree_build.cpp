@@ -776,10 +776,16 @@ uint64_t tree_hash(const HostTree &t) {
 
 int voxel_coords(const HostTree &t, size_t n, const uint32_t *ids,
                  float *xyz) {
-    // dense leaves in id order for the lookup
+    // dense leaves in id order for the lookup (a leaf whose truncated
+    // dimensions hold no voxel, possible only when coordinates are too
+    // coarse for the voxel size, has no ids and is left out)
     std::vector<uint32_t> dense;
-    for (uint32_t i = 0; i < t.nodes.size(); i++)
-        if ((t.nodes[i].info & NT_TYPE_MASK) == NT_DENSE) dense.push_back(i);
+    for (uint32_t i = 0; i < t.nodes.size(); i++) {
+        if ((t.nodes[i].info & NT_TYPE_MASK) != NT_DENSE) continue;
+        uint32_t dim[3];
+        dense_dims(t.nodes[i].bmin, t.nodes[i].bmax, t.voxel, dim);
+        if (dim[0] && dim[1] && dim[2]) dense.push_back(i);
+    }
     std::sort(dense.begin(), dense.end(), [&](uint32_t a, uint32_t b) {
         return t.nodes[a].ref < t.nodes[b].ref;
     });
@@ -797,6 +803,10 @@ int voxel_coords(const HostTree &t, size_t n, const uint32_t *ids,
                                    [&](uint32_t v, uint32_t nd) {
                                        return v < t.nodes[nd].ref;
                                    });
+        if (it == dense.begin()) {
+            xyz[3 * k] = xyz[3 * k + 1] = xyz[3 * k + 2] = NAN;
+            continue;
+        }
         const NodeRec &r = t.nodes[*(it - 1)];
         uint32_t dim[3];
         dense_dims(r.bmin, r.bmax, t.voxel, dim);
